@@ -1,0 +1,315 @@
+"""fsmeld_amd -- FSMELD's exact z-normalised subsequence search on MI355X.
+
+The engine is the HIP library ``lib/libfsmeld_hip.so`` (C ABI in
+``include/fsmeld.h``); this module is a thin ctypes layer over it that mirrors
+the reference's query-level entry points (include/funcs.h:64-72):
+
+* :func:`search`  -- ``FSMELD``: every window whose exact elastic distance to
+  the query is <= cutoff (LB_IP -> GLB -> ELD cascade);
+* :func:`cutoff`  -- ``FSMELD_find_threshold``: the k-th smallest exact
+  distance and the k best windows in (distance, index) order;
+* :func:`exact`   -- ``FSMELD_exact``: the brute-force scan.
+
+Importing the package loads the library and raises if it is missing: there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+__all__ = [
+    "METRICS", "Target", "QueryResult", "FsmeldError", "search", "cutoff", "exact",
+    "synth_target", "synth_query", "device_count", "lib_path",
+]
+
+METRICS = {"dtw": 1, "edr": 2, "erp": 3, "lcss": 4, "msm": 5}
+MODES = {"search": 0, "cutoff": 1, "exact": 2}
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "lib", "libfsmeld_hip.so")
+
+if not os.path.exists(lib_path):
+    raise ImportError(
+        f"fsmeld_amd: engine library not built ({lib_path}); run __graft_entry__.build()")
+
+_lib = ctypes.CDLL(lib_path)
+
+
+class FsmeldError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("metric", ctypes.c_int), ("m", ctypes.c_int),
+                ("w", ctypes.c_int), ("arg", ctypes.c_double), ("cutoff", ctypes.c_double),
+                ("k", ctypes.c_int), ("exclude_begin", ctypes.c_int64),
+                ("exclude_end", ctypes.c_int64)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("valid", ctypes.c_uint64), ("invalid", ctypes.c_uint64),
+                ("after_lbip", ctypes.c_uint64), ("after_glb", ctypes.c_uint64),
+                ("exact_evals", ctypes.c_uint64), ("matches", ctypes.c_uint64),
+                ("threshold", ctypes.c_double), ("ts_n", ctypes.c_int64),
+                ("k_m_l", ctypes.c_int64), ("forsize", ctypes.c_int64), ("tail", ctypes.c_int64),
+                ("m_k", ctypes.c_int), ("ms_stats", ctypes.c_double),
+                ("ms_lbip", ctypes.c_double), ("ms_glb", ctypes.c_double),
+                ("ms_eld", ctypes.c_double), ("ms_total", ctypes.c_double)]
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_U8 = ctypes.POINTER(ctypes.c_uint8)
+_VP = ctypes.c_void_p
+
+
+def _sig(name, res, args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_sig("fsm_device_count", ctypes.c_int, [])
+_sig("fsm_last_error", ctypes.c_char_p, [])
+_sig("fsm_target_open", ctypes.c_int, [_D, ctypes.c_int64, ctypes.POINTER(_VP)])
+_sig("fsm_target_open_file", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_VP)])
+_sig("fsm_target_length", ctypes.c_int64, [_VP])
+_sig("fsm_target_close", None, [_VP])
+_sig("fsm_query", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, ctypes.c_int64,
+                                 ctypes.POINTER(_VP)])
+_sig("fsm_result_stats", ctypes.c_int, [_VP, ctypes.POINTER(_Stats)])
+_sig("fsm_result_count", ctypes.c_int64, [_VP])
+_sig("fsm_result_indices", _I64, [_VP])
+_sig("fsm_result_distances", _D, [_VP])
+_sig("fsm_result_free", None, [_VP])
+_sig("fsm_stage_window_stats", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int64,
+                                              ctypes.c_int64, _D, _D, _U8])
+_sig("fsm_stage_lbip", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _D, _D, _D])
+_sig("fsm_stage_glb", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _I64, ctypes.c_int64, _D])
+_sig("fsm_stage_eld", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _I64, ctypes.c_int64, _D])
+_sig("fsm_eld_pairs", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     _D, _D, ctypes.c_int64, _D])
+_sig("fsm_synth_target", ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, _D])
+_sig("fsm_synth_query", ctypes.c_int, [ctypes.c_uint64, _D, ctypes.c_int64, ctypes.c_int,
+                                       ctypes.c_double, _D, _I64])
+
+
+def _ptr(a: np.ndarray, t=_D):
+    return a.ctypes.data_as(t)
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise FsmeldError(rc, (_lib.fsm_last_error() or b"").decode(errors="replace"))
+
+
+def device_count() -> int:
+    return int(_lib.fsm_device_count())
+
+
+def _f64(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+@dataclass
+class QueryResult:
+    """Indices (ascending for search/exact, rank order for cutoff), their exact
+    distances and the engine's counters (kk1/kk2/kk3 of src/fsmeld.cpp:981-996)."""
+    indices: np.ndarray
+    distances: np.ndarray
+    stats: dict = field(default_factory=dict)
+
+    @property
+    def threshold(self) -> float:
+        return self.stats.get("threshold", float("nan"))
+
+
+def _desc(mode, metric, m, w, arg, cutoff_value=0.0, k=0, exclude=None) -> _Desc:
+    d = _Desc()
+    d.mode = MODES[mode]
+    d.metric = METRICS[metric] if isinstance(metric, str) else int(metric)
+    d.m, d.w, d.arg = int(m), int(w), float(arg)
+    d.cutoff, d.k = float(cutoff_value), int(k)
+    if exclude is None:
+        d.exclude_begin = d.exclude_end = -1
+    else:
+        d.exclude_begin, d.exclude_end = int(exclude[0]), int(exclude[1])
+    return d
+
+
+class Target:
+    """A target series resident in GPU memory (HBM) for repeated queries."""
+
+    def __init__(self, values=None, path: Optional[str] = None):
+        self._h = _VP()
+        if path is not None:
+            _check(_lib.fsm_target_open_file(os.fsencode(path), ctypes.byref(self._h)))
+            self.values = None
+        else:
+            v = _f64(values)
+            self.values = v
+            _check(_lib.fsm_target_open(_ptr(v), int(v.size), ctypes.byref(self._h)))
+
+    def __len__(self):
+        return int(_lib.fsm_target_length(self._h))
+
+    def close(self):
+        if self._h:
+            _lib.fsm_target_close(self._h)
+            self._h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- query-level ----
+    def query(self, mode, query, m, metric, w, arg, cutoff_value=0.0, k=0,
+              exclude=None) -> QueryResult:
+        q = _f64(query)
+        d = _desc(mode, metric, m, w, arg, cutoff_value, k, exclude)
+        r = _VP()
+        _check(_lib.fsm_query(self._h, ctypes.byref(d), _ptr(q), int(q.size), ctypes.byref(r)))
+        try:
+            st = _Stats()
+            _check(_lib.fsm_result_stats(r, ctypes.byref(st)))
+            n = int(_lib.fsm_result_count(r))
+            idx = np.ctypeslib.as_array(_lib.fsm_result_indices(r), shape=(n,)).copy() if n else \
+                np.zeros(0, np.int64)
+            dist = np.ctypeslib.as_array(_lib.fsm_result_distances(r), shape=(n,)).copy() if n \
+                else np.zeros(0, np.float64)
+            stats = {name: getattr(st, name) for name, _ in _Stats._fields_}
+        finally:
+            _lib.fsm_result_free(r)
+        return QueryResult(idx, dist, stats)
+
+    def search(self, query, m, cutoff_value, metric, w, arg, exclude=None) -> QueryResult:
+        return self.query("search", query, m, metric, w, arg, cutoff_value=cutoff_value,
+                          exclude=exclude)
+
+    def cutoff(self, query, m, k, metric, w, arg, exclude=None) -> QueryResult:
+        return self.query("cutoff", query, m, metric, w, arg, k=k, exclude=exclude)
+
+    def exact(self, query, m, cutoff_value, metric, w, arg, exclude=None) -> QueryResult:
+        return self.query("exact", query, m, metric, w, arg, cutoff_value=cutoff_value,
+                          exclude=exclude)
+
+    # ---- stage-level (parity tests) ----
+    def window_stats(self, m, exclude=None):
+        n = len(self) - m + 1
+        mu, sg = np.empty(n), np.empty(n)
+        inv = np.empty(n, np.uint8)
+        xb, xe = (-1, -1) if exclude is None else exclude
+        _check(_lib.fsm_stage_window_stats(self._h, int(m), int(xb), int(xe), _ptr(mu),
+                                           _ptr(sg), _ptr(inv, _U8)))
+        return mu, sg, inv.astype(bool)
+
+    def lbip(self, query, m, metric, w, arg):
+        q = _f64(query)
+        d = _desc("search", metric, m, w, arg)
+        n = len(self) - m + 1
+        lb = np.empty(n)
+        seg = _geometry(len(self), m)["S"]
+        tu, tl = np.empty(seg), np.empty(seg)
+        _check(_lib.fsm_stage_lbip(self._h, ctypes.byref(d), _ptr(q), _ptr(lb), _ptr(tu),
+                                   _ptr(tl)))
+        return lb, tu, tl
+
+    def glb(self, query, m, metric, w, arg, windows, bsf):
+        q = _f64(query)
+        win = np.ascontiguousarray(np.asarray(windows, dtype=np.int64))
+        d = _desc("search", metric, m, w, arg, cutoff_value=bsf)
+        out = np.empty(win.size)
+        _check(_lib.fsm_stage_glb(self._h, ctypes.byref(d), _ptr(q), _ptr(win, _I64),
+                                  int(win.size), _ptr(out)))
+        return out
+
+    def eld(self, query, m, metric, w, arg, windows):
+        q = _f64(query)
+        win = np.ascontiguousarray(np.asarray(windows, dtype=np.int64))
+        d = _desc("search", metric, m, w, arg)
+        out = np.empty(win.size)
+        _check(_lib.fsm_stage_eld(self._h, ctypes.byref(d), _ptr(q), _ptr(win, _I64),
+                                  int(win.size), _ptr(out)))
+        return out
+
+
+def eld_pairs(metric, q, cands, w, arg) -> np.ndarray:
+    """Exact distance of each normalised row of ``cands`` to normalised ``q``."""
+    q = _f64(q)
+    c = _f64(cands).reshape(-1, q.size)
+    out = np.empty(c.shape[0])
+    m = METRICS[metric] if isinstance(metric, str) else int(metric)
+    _check(_lib.fsm_eld_pairs(m, int(q.size), int(w), float(arg), _ptr(q), _ptr(c),
+                              int(c.shape[0]), _ptr(out)))
+    return out
+
+
+def _geometry(N: int, m: int) -> dict:
+    """Macro-segmentation of src/fsmeld.cpp:28-37, :647-661."""
+    mk = 1
+    while mk < 8 * m:
+        mk *= 2
+    mk = min(mk, N)
+    kml = mk - m + 1
+    ts_n = N - m + 1
+    forsize, tail = divmod(ts_n, kml)
+    return dict(m_k=mk, k_m_l=kml, ts_n=ts_n, forsize=forsize, tail=tail,
+                tail_first=kml - tail, S=forsize + (1 if tail else 0))
+
+
+geometry = _geometry
+
+
+# ---- module-level wrappers named after the reference's entry points ----
+def search(target, query, m, cutoff_value, metric, w, arg, exclude=None) -> QueryResult:
+    """FSMELD (include/funcs.h:67-68)."""
+    t = target if isinstance(target, Target) else Target(target)
+    return t.search(query, m, cutoff_value, metric, w, arg, exclude)
+
+
+def cutoff(target, query, m, k, metric, w, arg, exclude=None) -> QueryResult:
+    """FSMELD_find_threshold (include/funcs.h:64-65)."""
+    t = target if isinstance(target, Target) else Target(target)
+    return t.cutoff(query, m, k, metric, w, arg, exclude)
+
+
+def exact(target, query, m, cutoff_value, metric, w, arg, exclude=None) -> QueryResult:
+    """FSMELD_exact with top_k = 0 (include/funcs.h:70-72)."""
+    t = target if isinstance(target, Target) else Target(target)
+    return t.exact(query, m, cutoff_value, metric, w, arg, exclude)
+
+
+def synth_target(seed: int, n: int, spike_frac=0.0, spike_sd=0.0, n_const_runs=0,
+                 const_len=300, n_nans=0) -> np.ndarray:
+    out = np.empty(int(n))
+    _check(_lib.fsm_synth_target(int(seed), int(n), float(spike_frac), float(spike_sd),
+                                 int(n_const_runs), int(const_len), int(n_nans), _ptr(out)))
+    return out
+
+
+def synth_query(seed: int, target: np.ndarray, m: int, noise_sd=0.1):
+    t = _f64(target)
+    q = np.empty(int(m))
+    off = ctypes.c_int64(0)
+    _check(_lib.fsm_synth_query(int(seed), _ptr(t), int(t.size), int(m), float(noise_sd),
+                                _ptr(q), ctypes.byref(off)))
+    return q, int(off.value)

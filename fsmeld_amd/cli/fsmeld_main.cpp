@@ -1,0 +1,273 @@
+// fsmeld_main.cpp -- `fsmeld search|cutoff|exact` on the gfx950 engine.
+//
+// Same command line, validation messages, exit codes and stdout / CSV format
+// as the reference's src/main.cpp:12-110 and the report blocks of
+// src/fsmeld.cpp (search :972-1026, cutoff :519-553, exact :1292-1334).
+// Differences: no per-percent progress lines (the GPU evaluates all segments
+// at once), and the cutoff report's pruning counters describe the GPU's
+// representative-then-search schedule rather than the reference's
+// sequential dynamic threshold.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fsmeld.h"
+#include "search_result.h"
+
+namespace {
+
+void usage() {
+    std::cout << "FSMELD: exact z-normalized subsequence search\n\n"
+                 "Usage:\n"
+                 "  fsmeld search TARGET.bin QUERY.txt m cutoff metric w arg [exclude_start exclude_end]\n"
+                 "  fsmeld cutoff TARGET.bin QUERY.txt m k      metric w arg [exclude_start exclude_end]\n"
+                 "  fsmeld exact  TARGET.bin QUERY.txt m cutoff metric w arg [exclude_start exclude_end]\n\n"
+                 "Metrics: dtw, erp, edr, lcss, msm\n"
+                 "arg: 0 for DTW; gap g for ERP; match epsilon for EDR/LCSS; cost c for MSM.\n"
+                 "TARGET.bin contains native-endian IEEE-754 doubles; QUERY.txt contains text numbers.\n"
+                 "Query and candidate windows are z-normalized. Matches use zero-based indices.\n"
+                 "Optional exclusion is a half-open interval [start,end) of candidate starts.\n"
+                 "No exclusion is applied by default (self-matches are included).\n";
+}
+
+long long integer(const char* text, const char* name) {
+    size_t end = 0;
+    const auto v = std::stoll(text, &end);
+    if (text[end] != '\0') throw std::invalid_argument(std::string("Invalid ") + name);
+    return v;
+}
+
+double number(const char* text, const char* name) {
+    size_t end = 0;
+    const double v = std::stod(text, &end);
+    if (text[end] != '\0' || !std::isfinite(v))
+        throw std::invalid_argument(std::string("Invalid finite ") + name);
+    return v;
+}
+
+int metric_code(const std::string& s) {
+    if (s == "dtw") return FSM_METRIC_DTW;
+    if (s == "erp") return FSM_METRIC_ERP;
+    if (s == "edr") return FSM_METRIC_EDR;
+    if (s == "lcss") return FSM_METRIC_LCSS;
+    if (s == "msm") return FSM_METRIC_MSM;
+    return 0;
+}
+
+bool quiet_matches() {  // src/fsmeld.cpp:998-1001
+    const char* e = std::getenv("FSMELD_QUIET_MATCHES");
+    return e && std::string(e) != "0" && std::string(e) != "false";
+}
+
+void check(int rc) {
+    if (rc != FSM_OK) throw std::runtime_error(fsm_last_error());
+}
+
+struct Run {
+    fsm_target* target = nullptr;
+    fsm_result* result = nullptr;
+    ~Run() {
+        if (result) fsm_result_free(result);
+        if (target) fsm_target_close(target);
+    }
+};
+
+void print_geometry(const fsm_stats& s, int m) {
+    std::cout << "ts_n=" << s.ts_n << std::endl;
+    std::cout << "m=" << m << std::endl;
+    std::cout << "k_m_l=" << s.k_m_l << std::endl;
+    std::cout << "m_k=" << s.m_k << std::endl;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc == 1 ||
+        (argc == 2 && (std::string(argv[1]) == "--help" || std::string(argv[1]) == "-h"))) {
+        usage();
+        return 0;
+    }
+    try {
+        if (argc != 9 && argc != 11) {
+            usage();
+            throw std::invalid_argument("Expected eight arguments, plus optional exclusion bounds.");
+        }
+        const std::string command = argv[1];
+        if (command != "search" && command != "cutoff" && command != "exact")
+            throw std::invalid_argument("Unknown command: " + command);
+        const auto length = integer(argv[4], "m");
+        if (length < 2 || length > std::numeric_limits<int>::max() / 8)
+            throw std::invalid_argument("m must be at least 2 and fit the FFT indexing range.");
+        const int m = (int)length;
+        const std::string metric = argv[6];
+        if (!metric_code(metric)) throw std::invalid_argument("Unsupported metric: " + metric);
+        const auto radius = integer(argv[7], "w");
+        if (radius < 0 || radius >= m) throw std::invalid_argument("w must satisfy 0 <= w < m.");
+        const int w = (int)radius;
+        const double arg = number(argv[8], "metric parameter");
+        if ((metric == "edr" || metric == "lcss" || metric == "msm") && arg < 0)
+            throw std::invalid_argument("Matching tolerance and MSM cost must be non-negative.");
+        const auto bytes = std::filesystem::file_size(argv[2]);
+        if (bytes % sizeof(double) || bytes / sizeof(double) < 2ULL * m - 1)
+            throw std::invalid_argument("Target must contain at least 2*m-1 complete double values.");
+        {
+            std::ifstream target(argv[2], std::ios::binary);
+            if (!target) throw std::invalid_argument("Cannot open target file.");
+        }
+        std::vector<double> query;
+        {
+            std::ifstream qf(argv[3]);
+            double sum = 0, squares = 0;
+            for (int i = 0; i < m; ++i) {
+                double v;
+                if (!(qf >> v) || !std::isfinite(v))
+                    throw std::invalid_argument("Query must begin with m finite numeric values.");
+                sum += v;
+                squares += v * v;
+                query.push_back(v);
+            }
+            const double mean = sum / m;
+            const double sigma = std::sqrt(squares / m - mean * mean);
+            if (!std::isfinite(sigma) || sigma <= 1e-12)
+                throw std::invalid_argument(
+                    "Query cannot be z-normalized: non-finite or near-zero standard deviation.");
+        }
+        long long begin = -1, end = -1;
+        if (argc == 11) {
+            begin = integer(argv[9], "exclude_start");
+            end = integer(argv[10], "exclude_end");
+            if (begin < 0 || end <= begin)
+                throw std::invalid_argument("Exclusion requires 0 <= start < end.");
+        }
+        fsm_query_desc d{};
+        d.metric = metric_code(metric);
+        d.m = m;
+        d.w = w;
+        d.arg = arg;
+        d.exclude_begin = begin;
+        d.exclude_end = end;
+        if (command == "cutoff") {
+            const auto k = integer(argv[5], "k");
+            if (k < 1 || k > std::numeric_limits<int>::max() ||
+                (unsigned long long)k > bytes / sizeof(double) - m + 1)
+                throw std::invalid_argument(
+                    "k exceeds the candidate count or supported integer range.");
+            d.mode = FSM_MODE_CUTOFF;
+            d.k = (int)k;
+        } else {
+            const double cutoff = number(argv[5], "cutoff");
+            if (cutoff < 0) throw std::invalid_argument("cutoff must be non-negative.");
+            d.mode = command == "search" ? FSM_MODE_SEARCH : FSM_MODE_EXACT;
+            d.cutoff = cutoff;
+        }
+
+        Run run;
+        check(fsm_target_open_file(argv[2], &run.target));
+        check(fsm_query(run.target, &d, query.data(), (int64_t)query.size(), &run.result));
+        fsm_stats s{};
+        check(fsm_result_stats(run.result, &s));
+        const int64_t n = fsm_result_count(run.result);
+        const int64_t* idx = fsm_result_indices(run.result);
+        const double* dist = fsm_result_distances(run.result);
+        const bool quiet = quiet_matches();
+
+        if (d.mode == FSM_MODE_CUTOFF) {
+            print_geometry(s, m);
+            std::cout << "\n\n" << std::string(60, '=') << std::endl;
+            std::cout << " [FSMELD Top-K Threshold Results]" << std::endl;
+            std::cout << "  - Candidates checked by LB:   " << s.valid << std::endl;
+            std::cout << "  - Pruned by LBIP:             " << (s.valid - s.after_lbip) << std::endl;
+            std::cout << "  - Pruned by GLB:              " << (s.after_lbip - s.after_glb) << std::endl;
+            std::cout << "  - Exact distances computed:   " << s.exact_evals << std::endl;
+            std::cout << std::setprecision(17);
+            std::cout << "  - Final Top-K Threshold:      " << s.threshold << std::endl;
+            std::cout << std::string(60, '=') << "\n" << std::endl;
+            for (int64_t k = 0; k < n; ++k)
+                std::cout << "Rank " << k + 1 << ": Index A[" << k + 1 << "]=" << idx[k] << std::endl;
+            return 0;
+        }
+
+        print_geometry(s, m);
+        std::printf("tail=%lld\n", (long long)s.tail);
+        std::printf("forsize=%lld\n", (long long)s.forsize);
+        ExperimentRow row;
+        row.dataset = dataset_name_of(argv[2]);
+        row.metric = metric;
+        row.m = m;
+        row.w = w;
+        row.xi = d.cutoff;
+        row.total_time_seconds = s.ms_total / 1000.0;
+        row.valid_candidates = s.valid;
+        row.invalid_candidates = s.invalid;
+        row.matches = (uint64_t)n;
+
+        if (d.mode == FSM_MODE_SEARCH) {
+            const double lb_seconds = (s.ms_stats + s.ms_lbip + s.ms_glb) / 1000.0;
+            const double lbip_seconds = (s.ms_stats + s.ms_lbip) / 1000.0;
+            const double lbip_rate =
+                s.valid == 0 ? 0.0 : (double)(s.valid - s.after_lbip) / (double)s.valid;
+            const double rate =
+                s.valid == 0 ? 0.0 : (double)(s.valid - s.after_glb) / (double)s.valid;
+            std::cout << "kk1=" << s.valid << std::endl;
+            std::cout << "kk2=" << s.after_lbip << std::endl;
+            std::cout << "kk2=" << s.after_lbip << std::endl;
+            std::cout << "Pruning Power (guolv)=" << lbip_rate << std::endl;
+            std::cout << "Pruning Power (guolv2)=" << rate << std::endl;
+            std::cout << "LP_IP Lower Bound Time taken: " << lbip_seconds * 1000.0 << " ms" << std::endl;
+            std::cout << "LP_IP Lower Bound Time taken: " << lbip_seconds << " s" << std::endl;
+            std::cout << "T_LB: " << std::setprecision(17) << lb_seconds << " s" << std::endl;
+            std::cout << "T_total: " << s.ms_total / 1000.0 << " s" << std::endl;
+            std::cout << "Invalid candidates: " << s.invalid << std::endl;
+            std::cout << "Final dynamic threshold: " << d.cutoff << std::endl;
+            std::cout << "Total Valid Candidates: " << s.valid << std::endl;
+            std::cout << "Candidates after LP/IP: " << s.after_lbip << std::endl;
+            std::cout << "Cascade: LB_IP+GLB+ELD" << std::endl;
+            std::cout << "Candidates after GLB: " << s.after_glb << std::endl;
+            std::cout << "Final Matches: " << n << std::endl;
+            if (!quiet)
+                for (int64_t i = 0; i < n; ++i) std::printf("A[%lld]=%lld\n", (long long)i, (long long)idx[i]);
+            std::fflush(stdout);
+            row.method = "FSMELD";
+            row.lb_time_seconds = lb_seconds;
+            row.pruned_candidates = s.valid - s.after_glb;
+            row.survivors = s.after_glb;
+            row.has_lbip = true;
+            row.lbip_time_seconds = lbip_seconds;
+            row.lbip_pruning_rate = lbip_rate;
+        } else {
+            if (!quiet)
+                for (int64_t i = 0; i < n; ++i)
+                    std::printf("\nGET %s: %lf (index: %lld)\n", metric.c_str(), dist[i], (long long)idx[i]);
+            std::fflush(stdout);
+            std::cout << "\n\n--- Statistics ---" << std::endl;
+            std::cout << "Total comparisons (kk1) = " << s.valid << std::endl;
+            std::cout << "Exact ELD evaluations = " << s.exact_evals << std::endl;
+            std::cout << "Final matches found: " << n << std::endl;
+            std::cout << "Total Valid Candidates: " << s.valid << std::endl;
+            std::cout << "Invalid candidates: " << s.invalid << std::endl;
+            std::cout << "Exact ELD Evaluations: " << s.exact_evals << std::endl;
+            std::cout << "Final Matches: " << n << std::endl;
+            if (!quiet)
+                for (int64_t k = 0; k < n; ++k) std::printf("A[%lld] = %lld\n", (long long)k, (long long)idx[k]);
+            std::fflush(stdout);
+            std::cout << "T_total: " << std::setprecision(17) << s.ms_total / 1000.0 << " s" << std::endl;
+            row.method = "ELD";
+            row.lb_time_seconds = 0.0;
+            row.pruned_candidates = 0;
+            row.survivors = s.valid;
+        }
+        emit_experiment_row(row);
+        return 0;
+    } catch (const std::exception& error) {
+        std::cerr << "Error: " << error.what() << '\n';
+        return 1;
+    }
+}

@@ -1,0 +1,745 @@
+// engine.hip -- host side of the C ABI in include/fsmeld.h: device-resident
+// targets, per-query orchestration of the kernels, result objects.
+//
+// Query flow (one HIP stream per target, one host synchronisation per call):
+//   search: window stats (cached per m) -> envelope (cached per w) -> LB_IP
+//           -> compact by the LB_IP test -> GLB -> ELD -> sorted matches
+//   exact : window stats -> compact valid windows -> ELD
+//   cutoff: search's stages plus a representative pass: the smallest-LB
+//           window of every segment is verified first; the k-th smallest of
+//           those exact distances is an upper bound xi0 of the k-th smallest
+//           distance overall, a search at xi0 returns every window that can
+//           rank, and the k smallest by (distance, index) are the answer --
+//           the order the reference's max-heap leaves (src/fsmeld.cpp:376-418).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fsmeld.h"
+#include "engine_internal.h"
+
+using namespace fsm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? FSM_ERR_OOM : FSM_ERR_HIP,        \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+bool make_geometry(int64_t N, int m, Geometry* G) {
+    long long sel = 1;
+    while (sel < 8LL * m) sel *= 2;  // src/fsmeld.cpp:28-37
+    if (N < 2LL * m - 1) return false;
+    if (N < sel) sel = N;
+    if (sel > (1LL << 30)) return false;
+    G->N = N;
+    G->m = m;
+    G->m_k = (int)sel;
+    G->kml = G->m_k - m + 1;
+    G->ts_n = N - m + 1;
+    G->forsize = G->ts_n / G->kml;
+    G->tail = G->ts_n % G->kml;
+    G->tail_first = (int)(G->kml - G->tail);
+    G->S = G->forsize + (G->tail > 0 ? 1 : 0);
+    return true;
+}
+
+// Centred clipped envelope on the host for the query (lower_upper_lemire,
+// src/lower_upper_lemire.cpp:8-44; the query holds only finite values).
+void host_envelope(const std::vector<double>& a, int r, std::vector<double>& L,
+                   std::vector<double>& U) {
+    const int n = (int)a.size();
+    L.assign(n, 0.0);
+    U.assign(n, 0.0);
+    std::vector<int> ql(n), qu(n);
+    int hl = 0, tl = -1, hu = 0, tu = -1, out = 0;
+    auto push = [&](int i) {
+        while (hl <= tl && a[ql[tl]] >= a[i]) tl--;
+        ql[++tl] = i;
+        while (hu <= tu && a[qu[tu]] <= a[i]) tu--;
+        qu[++tu] = i;
+    };
+    const int k = 2 * r + 1;
+    for (int i = 0; i < n + r; i++) {
+        if (i < n) push(i);
+        if (i >= r) {
+            while (ql[hl] <= i - k) hl++;
+            while (qu[hu] <= i - k) hu++;
+            L[out] = a[ql[hl]];
+            U[out] = a[qu[hu]];
+            out++;
+        }
+    }
+}
+
+struct HostQuery {
+    std::vector<double> qn, LQ, UQ, qs, S1, S2, SD, TU, TL;
+};
+
+// src/fsmeld.cpp:622-643 (normalisation and sorted copy), :721 (envelope).
+void prepare_query(const double* q_raw, int m, int w, double arg, HostQuery* H) {
+    double ex = 0.0, ex2 = 0.0;
+    for (int i = 0; i < m; i++) {
+        ex += q_raw[i];
+        ex2 += q_raw[i] * q_raw[i];
+    }
+    const double mean = ex / m;
+    double sd = ex2 / m;
+    sd = std::sqrt(sd - mean * mean);
+    H->qn.resize(m);
+    for (int i = 0; i < m; i++) H->qn[i] = (q_raw[i] - mean) / sd;
+    host_envelope(H->qn, w, H->LQ, H->UQ);
+    H->qs = H->qn;
+    std::sort(H->qs.begin(), H->qs.end());
+    H->S1.assign(m + 1, 0.0);
+    H->S2.assign(m + 1, 0.0);
+    H->SD.assign(m + 1, 0.0);
+    H->TU.assign(m, 0.0);
+    H->TL.assign(m, 0.0);
+    for (int i = 0; i < m; i++) {
+        const double q = H->qs[i];
+        H->S1[i + 1] = H->S1[i] + q;
+        H->S2[i + 1] = H->S2[i] + q * q;
+        H->SD[i + 1] = H->SD[i] + (q - arg) * (q - arg);
+        H->TU[i] = q - arg;
+        H->TL[i] = q + arg;
+    }
+}
+
+// Argument checks of src/main.cpp:46-99 that concern the engine.
+int validate(const fsm_query_desc* d, int64_t N, const double* q, int64_t qlen, bool need_mode) {
+    if (!d) return fail(FSM_ERR_ARG, "null query descriptor");
+    if (need_mode && (d->mode < FSM_MODE_SEARCH || d->mode > FSM_MODE_EXACT))
+        return fail(FSM_ERR_ARG, "Unknown command");
+    if (d->m < 2 || d->m > (1 << 27))
+        return fail(FSM_ERR_ARG, "m must be at least 2 and fit the FFT indexing range.");
+    if (d->metric < FSM_METRIC_DTW || d->metric > FSM_METRIC_MSM)
+        return fail(FSM_ERR_ARG, "Unsupported metric");
+    if (d->w < 0 || d->w >= d->m) return fail(FSM_ERR_ARG, "w must satisfy 0 <= w < m.");
+    if (!std::isfinite(d->arg)) return fail(FSM_ERR_ARG, "Invalid finite metric parameter");
+    if ((d->metric == FSM_METRIC_EDR || d->metric == FSM_METRIC_LCSS ||
+         d->metric == FSM_METRIC_MSM) && d->arg < 0)
+        return fail(FSM_ERR_ARG, "Matching tolerance and MSM cost must be non-negative.");
+    if (N < 2LL * d->m - 1)
+        return fail(FSM_ERR_ARG, "Target must contain at least 2*m-1 complete double values.");
+    if (!q || qlen < d->m) return fail(FSM_ERR_ARG, "Query must begin with m finite numeric values.");
+    double sum = 0, sq = 0;
+    for (int i = 0; i < d->m; i++) {
+        if (!std::isfinite(q[i]))
+            return fail(FSM_ERR_ARG, "Query must begin with m finite numeric values.");
+        sum += q[i];
+        sq += q[i] * q[i];
+    }
+    const double mean = sum / d->m;
+    const double sigma = std::sqrt(sq / d->m - mean * mean);
+    if (!std::isfinite(sigma) || sigma <= 1e-12)
+        return fail(FSM_ERR_ARG,
+                    "Query cannot be z-normalized: non-finite or near-zero standard deviation.");
+    const bool excl = !(d->exclude_begin == -1 && d->exclude_end == -1);
+    if (excl && (d->exclude_begin < 0 || d->exclude_end <= d->exclude_begin))
+        return fail(FSM_ERR_ARG, "Exclusion requires 0 <= start < end.");
+    if (need_mode && d->mode == FSM_MODE_CUTOFF) {
+        if (d->k < 1 || (int64_t)d->k > N - d->m + 1)
+            return fail(FSM_ERR_ARG, "k exceeds the candidate count or supported integer range.");
+    } else if (need_mode) {
+        if (!std::isfinite(d->cutoff)) return fail(FSM_ERR_ARG, "Invalid finite cutoff");
+        if (d->cutoff < 0) return fail(FSM_ERR_ARG, "cutoff must be non-negative.");
+    }
+    return FSM_OK;
+}
+
+double ms_between(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return (double)ms;
+}
+
+}  // namespace
+
+struct fsm_target {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t N = 0;
+    DevBuf T;
+    int stats_m = -1;
+    DevBuf MU, SG, INV;
+    int env_w = -1;
+    DevBuf Lr, Ur;
+    DevBuf lb, list_a, list_b, out_idx, out_dist, slab, rep_lb, rep_idx, tu, tl, qtab,
+        scratch, counters;
+    hipEvent_t ev[8] = {};
+};
+
+struct fsm_result {
+    std::vector<int64_t> idx;
+    std::vector<double> dist;
+    fsm_stats st{};
+};
+
+namespace {
+
+int upload_query(fsm_target* t, const HostQuery& H, int m, QueryTables* Q) {
+    const size_t stride = (size_t)m + 1;
+    HIP_TRY(t->qtab.ensure(sizeof(double) * stride * 9));
+    std::vector<double> buf(stride * 9, 0.0);
+    const std::vector<double>* parts[9] = {&H.qn, &H.LQ, &H.UQ, &H.qs, &H.S1,
+                                           &H.S2, &H.SD, &H.TU, &H.TL};
+    for (int k = 0; k < 9; k++)
+        std::copy(parts[k]->begin(), parts[k]->end(), buf.begin() + k * stride);
+    HIP_TRY(hipMemcpyAsync(t->qtab.p, buf.data(), sizeof(double) * buf.size(),
+                           hipMemcpyHostToDevice, t->stream));
+    double* d = t->qtab.as<double>();
+    Q->qn = d;
+    Q->LQ = d + stride;
+    Q->UQ = d + 2 * stride;
+    Q->qs = d + 3 * stride;
+    Q->S1 = d + 4 * stride;
+    Q->S2 = d + 5 * stride;
+    Q->SD = d + 6 * stride;
+    Q->TU = d + 7 * stride;
+    Q->TL = d + 8 * stride;
+    // the staging vector must outlive the async copy
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    return FSM_OK;
+}
+
+int ensure_stats(fsm_target* t, const Geometry& G, bool* computed) {
+    *computed = false;
+    if (t->stats_m == G.m) return FSM_OK;
+    const size_t n = (size_t)G.S * (size_t)G.kml;
+    HIP_TRY(t->MU.ensure(sizeof(double) * n));
+    HIP_TRY(t->SG.ensure(sizeof(double) * n));
+    HIP_TRY(t->INV.ensure(n));
+    HIP_TRY(launch_window_stats(t->stream, t->T.as<double>(), G, t->MU.as<double>(),
+                                t->SG.as<double>(), t->INV.as<uint8_t>()));
+    t->stats_m = G.m;
+    *computed = true;
+    return FSM_OK;
+}
+
+int ensure_env(fsm_target* t, int w) {
+    if (t->env_w == w) return FSM_OK;
+    HIP_TRY(t->Lr.ensure(sizeof(double) * (size_t)t->N));
+    HIP_TRY(t->Ur.ensure(sizeof(double) * (size_t)t->N));
+    HIP_TRY(launch_envelope(t->stream, t->T.as<double>(), t->N, w, t->Lr.as<double>(),
+                            t->Ur.as<double>()));
+    t->env_w = w;
+    return FSM_OK;
+}
+
+int run_lbip(fsm_target* t, const Geometry& G, const QueryTables& Q, const QueryParams& P,
+             bool with_rep) {
+    HIP_TRY(t->lb.ensure(sizeof(double) * (size_t)G.ts_n));
+    HIP_TRY(t->tu.ensure(sizeof(double) * (size_t)G.S));
+    HIP_TRY(t->tl.ensure(sizeof(double) * (size_t)G.S));
+    HIP_TRY(t->rep_lb.ensure(sizeof(double) * (size_t)G.S));
+    HIP_TRY(t->rep_idx.ensure(sizeof(int64_t) * (size_t)G.S));
+    const size_t per = lbip_slab_bytes(G);
+    int blocks = (int)std::min<int64_t>(G.S, per > (1u << 20) ? 1024 : 2048);
+    HIP_TRY(t->slab.ensure(per * (size_t)blocks));
+    LbipArgs a{};
+    a.T = t->T.as<double>();
+    a.MU = t->MU.as<double>();
+    a.SG = t->SG.as<double>();
+    a.INV = t->INV.as<uint8_t>();
+    a.Lraw = t->Lr.as<double>();
+    a.Uraw = t->Ur.as<double>();
+    a.q = Q;
+    a.p = P;
+    a.G = G;
+    a.LB = t->lb.as<double>();
+    a.TUo = t->tu.as<double>();
+    a.TLo = t->tl.as<double>();
+    a.rep_lb = with_rep ? t->rep_lb.as<double>() : nullptr;
+    a.rep_idx = with_rep ? t->rep_idx.as<int64_t>() : nullptr;
+    a.slab = t->slab.p;
+    a.slab_bytes_per_block = per;
+    a.blocks = blocks;
+    HIP_TRY(launch_lbip(t->stream, a));
+    return FSM_OK;
+}
+
+VerifyArgs verify_args(fsm_target* t, const Geometry& G, const QueryTables& Q,
+                       const QueryParams& P) {
+    VerifyArgs v{};
+    v.T = t->T.as<double>();
+    v.MU = t->MU.as<double>();
+    v.SG = t->SG.as<double>();
+    v.Lraw = t->Lr.as<double>();
+    v.Uraw = t->Ur.as<double>();
+    v.q = Q;
+    v.p = P;
+    v.G = G;
+    return v;
+}
+
+int ensure_eld_scratch(fsm_target* t, int w) {
+    if (w <= 32) return FSM_OK;
+    const int64_t thr = eld_scratch_threads();
+    HIP_TRY(t->scratch.ensure(sizeof(double) * 2 * (size_t)(2 * w + 2) * (size_t)thr));
+    return FSM_OK;
+}
+
+QueryParams params_of(const fsm_query_desc* d) {
+    QueryParams P{};
+    P.metric = d->metric;
+    P.m = d->m;
+    P.w = d->w;
+    P.arg = d->arg;
+    P.cutoff = d->cutoff;
+    const bool excl = !(d->exclude_begin == -1 && d->exclude_end == -1);
+    P.xb = excl ? d->exclude_begin : -1;
+    P.xe = excl ? d->exclude_end : -1;
+    return P;
+}
+
+// counters: [0] compacted, [1] valid, [2] after GLB, [3] ELD kept
+int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
+                      const QueryParams& P, bool use_lb, bool use_glb) {
+    unsigned long long* C = t->counters.as<unsigned long long>();
+    HIP_TRY(hipMemsetAsync(C, 0, sizeof(unsigned long long) * 8, t->stream));
+    HIP_TRY(t->list_a.ensure(sizeof(int64_t) * (size_t)G.ts_n));
+    HIP_TRY(t->list_b.ensure(sizeof(int64_t) * (size_t)G.ts_n));
+    HIP_TRY(t->out_idx.ensure(sizeof(int64_t) * (size_t)G.ts_n));
+    HIP_TRY(t->out_dist.ensure(sizeof(double) * (size_t)G.ts_n));
+    HIP_TRY(launch_compact(t->stream, G, P, t->INV.as<uint8_t>(),
+                           use_lb ? t->lb.as<double>() : nullptr, use_lb,
+                           t->list_a.as<int64_t>(), C));
+    HIP_TRY(hipEventRecord(t->ev[3], t->stream));
+    VerifyArgs v = verify_args(t, G, Q, P);
+    const int64_t* eld_in = t->list_a.as<int64_t>();
+    const unsigned long long* eld_cnt = C;
+    if (use_glb) {
+        HIP_TRY(launch_glb(t->stream, v, t->list_a.as<int64_t>(), C, t->list_b.as<int64_t>(),
+                           C + 2, nullptr));
+        eld_in = t->list_b.as<int64_t>();
+        eld_cnt = C + 2;
+    }
+    HIP_TRY(hipEventRecord(t->ev[4], t->stream));
+    int rc = ensure_eld_scratch(t, P.w);
+    if (rc) return rc;
+    HIP_TRY(launch_eld(t->stream, v, eld_in, eld_cnt, false, t->out_idx.as<int64_t>(),
+                       t->out_dist.as<double>(), C + 3, t->scratch.as<double>(),
+                       eld_scratch_threads()));
+    HIP_TRY(hipEventRecord(t->ev[5], t->stream));
+    return FSM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char* fsm_last_error(void) { return g_last_error.c_str(); }
+
+int fsm_target_open(const double* host, int64_t n, fsm_target** out) {
+    if (!out || !host || n <= 0) return fail(FSM_ERR_ARG, "invalid target");
+    *out = nullptr;
+    if (fsm_device_count() < 1) return fail(FSM_ERR_NODEV, "no HIP device");
+    auto* t = new fsm_target();
+    t->N = n;
+    auto cleanup = [&](int rc) {
+        fsm_target_close(t);
+        return rc;
+    };
+    if (hipSetDevice(0) != hipSuccess) return cleanup(fail(FSM_ERR_HIP, "hipSetDevice failed"));
+    if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(FSM_ERR_HIP, "hipStreamCreate failed"));
+    for (auto& e : t->ev)
+        if (hipEventCreate(&e) != hipSuccess)
+            return cleanup(fail(FSM_ERR_HIP, "hipEventCreate failed"));
+    if (t->T.ensure(sizeof(double) * (size_t)n) != hipSuccess)
+        return cleanup(fail(FSM_ERR_OOM, "device allocation of the target failed"));
+    if (t->counters.ensure(sizeof(unsigned long long) * 8) != hipSuccess)
+        return cleanup(fail(FSM_ERR_OOM, "device allocation failed"));
+    if (hipMemcpy(t->T.p, host, sizeof(double) * (size_t)n, hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return cleanup(fail(FSM_ERR_HIP, "target upload failed"));
+    *out = t;
+    return FSM_OK;
+}
+
+int fsm_target_open_file(const char* path, fsm_target** out) {
+    if (!path || !out) return fail(FSM_ERR_ARG, "invalid path");
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) return fail(FSM_ERR_IO, "Cannot open target file.");
+    const std::streamoff bytes = f.tellg();
+    if (bytes <= 0 || bytes % (std::streamoff)sizeof(double))
+        return fail(FSM_ERR_ARG, "Target must contain at least 2*m-1 complete double values.");
+    std::vector<double> v((size_t)(bytes / (std::streamoff)sizeof(double)));
+    f.seekg(0);
+    f.read(reinterpret_cast<char*>(v.data()), bytes);
+    if (f.gcount() != bytes) return fail(FSM_ERR_IO, "short read of target file");
+    return fsm_target_open(v.data(), (int64_t)v.size(), out);
+}
+
+int64_t fsm_target_length(const fsm_target* t) { return t ? t->N : 0; }
+
+void fsm_target_close(fsm_target* t) {
+    if (!t) return;
+    if (t->stream) (void)hipStreamSynchronize(t->stream);
+    DevBuf* bufs[] = {&t->T, &t->MU, &t->SG, &t->INV, &t->Lr, &t->Ur, &t->lb,
+                      &t->list_a, &t->list_b, &t->out_idx, &t->out_dist, &t->slab,
+                      &t->rep_lb, &t->rep_idx, &t->tu, &t->tl, &t->qtab, &t->scratch,
+                      &t->counters};
+    for (DevBuf* b : bufs) b->release();
+    for (auto& e : t->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+    delete t;
+}
+
+int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t qlen,
+              fsm_result** out) {
+    const auto wall0 = std::chrono::steady_clock::now();
+    if (!t || !out) return fail(FSM_ERR_ARG, "null argument");
+    *out = nullptr;
+    int rc = validate(d, t->N, q, qlen, true);
+    if (rc) return rc;
+    Geometry G;
+    if (!make_geometry(t->N, d->m, &G))
+        return fail(FSM_ERR_ARG, "Macro segment exceeds the supported FFT length.");
+    HostQuery H;
+    prepare_query(q, d->m, d->w, d->arg, &H);
+    QueryTables Q{};
+    rc = upload_query(t, H, d->m, &Q);
+    if (rc) return rc;
+    QueryParams P = params_of(d);
+    std::unique_ptr<fsm_result> R(new fsm_result());
+    fsm_stats& st = R->st;
+    st.ts_n = G.ts_n;
+    st.k_m_l = G.kml;
+    st.forsize = G.forsize;
+    st.tail = G.tail;
+    st.m_k = G.m_k;
+    auto bail = [&](int code) { return code; };
+    unsigned long long C[8] = {};
+
+    HIP_TRY(hipEventRecord(t->ev[0], t->stream));
+    bool stats_new = false;
+    rc = ensure_stats(t, G, &stats_new);
+    if (rc) return bail(rc);
+    HIP_TRY(hipEventRecord(t->ev[1], t->stream));
+
+    if (d->mode == FSM_MODE_EXACT) {
+        HIP_TRY(hipEventRecord(t->ev[2], t->stream));
+        rc = run_filter_verify(t, G, Q, P, false, false);
+        if (rc) return bail(rc);
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        HIP_TRY(hipMemcpy(C, t->counters.p, sizeof(C), hipMemcpyDeviceToHost));
+        st.valid = C[1];
+        st.invalid = (uint64_t)G.ts_n - C[1];
+        st.after_lbip = C[0];
+        st.after_glb = C[0];
+        st.exact_evals = C[0];
+    } else {
+        rc = ensure_env(t, d->w);
+        if (rc) return bail(rc);
+        rc = run_lbip(t, G, Q, P, d->mode == FSM_MODE_CUTOFF);
+        if (rc) return bail(rc);
+        HIP_TRY(hipEventRecord(t->ev[2], t->stream));
+        uint64_t rep_evals = 0;
+        if (d->mode == FSM_MODE_CUTOFF) {
+            // valid count and representatives
+            unsigned long long* Cd = t->counters.as<unsigned long long>();
+            HIP_TRY(hipMemsetAsync(Cd, 0, sizeof(unsigned long long) * 8, t->stream));
+            HIP_TRY(t->list_a.ensure(sizeof(int64_t) * (size_t)G.ts_n));
+            HIP_TRY(launch_compact(t->stream, G, P, t->INV.as<uint8_t>(), nullptr, false,
+                                   t->list_a.as<int64_t>(), Cd));
+            std::vector<int64_t> reps((size_t)G.S);
+            HIP_TRY(hipMemcpyAsync(reps.data(), t->rep_idx.p, sizeof(int64_t) * G.S,
+                                   hipMemcpyDeviceToHost, t->stream));
+            HIP_TRY(hipMemcpyAsync(C, t->counters.p, sizeof(C), hipMemcpyDeviceToHost,
+                                   t->stream));
+            HIP_TRY(hipStreamSynchronize(t->stream));
+            const uint64_t nvalid = C[1];
+            if (nvalid < (uint64_t)d->k)
+                return bail(fail(FSM_ERR_FEWER_THAN_K,
+                                 "Fewer valid, non-excluded candidates than requested k."));
+            std::vector<int64_t> rl;
+            rl.reserve(reps.size());
+            for (int64_t r : reps)
+                if (r >= 0) rl.push_back(r);
+            double xi0 = std::numeric_limits<double>::infinity();
+            if ((int64_t)rl.size() >= d->k) {
+                HIP_TRY(t->list_b.ensure(sizeof(int64_t) * (size_t)G.ts_n));
+                HIP_TRY(t->out_dist.ensure(sizeof(double) * (size_t)G.ts_n));
+                unsigned long long nrep = rl.size();
+                HIP_TRY(hipMemcpyAsync(t->list_b.p, rl.data(), sizeof(int64_t) * rl.size(),
+                                       hipMemcpyHostToDevice, t->stream));
+                HIP_TRY(hipMemcpyAsync(Cd + 4, &nrep, sizeof(nrep), hipMemcpyHostToDevice,
+                                       t->stream));
+                rc = ensure_eld_scratch(t, d->w);
+                if (rc) return bail(rc);
+                VerifyArgs v = verify_args(t, G, Q, P);
+                HIP_TRY(launch_eld(t->stream, v, t->list_b.as<int64_t>(), Cd + 4, true, nullptr,
+                                   t->out_dist.as<double>(), nullptr, t->scratch.as<double>(),
+                                   eld_scratch_threads()));
+                std::vector<double> rd(rl.size());
+                HIP_TRY(hipMemcpyAsync(rd.data(), t->out_dist.p, sizeof(double) * rd.size(),
+                                       hipMemcpyDeviceToHost, t->stream));
+                HIP_TRY(hipStreamSynchronize(t->stream));
+                rep_evals = rl.size();
+                std::vector<double> fin;
+                for (double x : rd)
+                    if (std::isfinite(x)) fin.push_back(x);
+                if ((int64_t)fin.size() >= d->k) {
+                    std::nth_element(fin.begin(), fin.begin() + (d->k - 1), fin.end());
+                    xi0 = fin[d->k - 1];
+                }
+            }
+            P.cutoff = xi0;
+        }
+        rc = run_filter_verify(t, G, Q, P, true, true);
+        if (rc) return bail(rc);
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        unsigned long long C2[8] = {};
+        HIP_TRY(hipMemcpy(C2, t->counters.p, sizeof(C2), hipMemcpyDeviceToHost));
+        std::memcpy(C, C2, sizeof(C));
+        st.valid = C[1];
+        st.invalid = (uint64_t)G.ts_n - C[1];
+        st.after_lbip = C[0];
+        st.after_glb = C[2];
+        st.exact_evals = C[2] + rep_evals;
+    }
+    const uint64_t nres = C[3];
+    R->idx.resize(nres);
+    R->dist.resize(nres);
+    if (nres) {
+        HIP_TRY(hipMemcpy(R->idx.data(), t->out_idx.p, sizeof(int64_t) * nres,
+                          hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(R->dist.data(), t->out_dist.p, sizeof(double) * nres,
+                          hipMemcpyDeviceToHost));
+    }
+    // order: ascending index (search / exact), (distance, index) for cutoff
+    std::vector<size_t> ord(nres);
+    for (size_t i = 0; i < nres; i++) ord[i] = i;
+    if (d->mode == FSM_MODE_CUTOFF) {
+        std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) {
+            if (R->dist[a] != R->dist[b]) return R->dist[a] < R->dist[b];
+            return R->idx[a] < R->idx[b];
+        });
+        if (ord.size() > (size_t)d->k) ord.resize(d->k);
+    } else {
+        std::sort(ord.begin(), ord.end(),
+                  [&](size_t a, size_t b) { return R->idx[a] < R->idx[b]; });
+    }
+    std::vector<int64_t> idx(ord.size());
+    std::vector<double> dist(ord.size());
+    for (size_t i = 0; i < ord.size(); i++) {
+        idx[i] = R->idx[ord[i]];
+        dist[i] = R->dist[ord[i]];
+    }
+    R->idx.swap(idx);
+    R->dist.swap(dist);
+    if (d->mode == FSM_MODE_CUTOFF) {
+        if ((int64_t)R->idx.size() < d->k)
+            return bail(fail(FSM_ERR_FEWER_THAN_K,
+                             "Fewer valid, non-excluded candidates than requested k."));
+        st.threshold = R->dist.back();
+    }
+    st.matches = R->idx.size();
+    st.ms_stats = stats_new ? ms_between(t->ev[0], t->ev[1]) : 0.0;
+    st.ms_lbip = d->mode == FSM_MODE_EXACT ? 0.0 : ms_between(t->ev[1], t->ev[2]);
+    if (d->mode == FSM_MODE_EXACT) {
+        st.ms_glb = 0.0;
+        st.ms_eld = ms_between(t->ev[2], t->ev[5]);
+    } else {
+        st.ms_glb = ms_between(t->ev[3], t->ev[4]);
+        st.ms_eld = ms_between(t->ev[4], t->ev[5]);
+    }
+    st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                           wall0).count();
+    *out = R.release();
+    return FSM_OK;
+}
+
+int fsm_result_stats(const fsm_result* r, fsm_stats* out) {
+    if (!r || !out) return fail(FSM_ERR_ARG, "null argument");
+    *out = r->st;
+    return FSM_OK;
+}
+int64_t fsm_result_count(const fsm_result* r) { return r ? (int64_t)r->idx.size() : 0; }
+const int64_t* fsm_result_indices(const fsm_result* r) { return r ? r->idx.data() : nullptr; }
+const double* fsm_result_distances(const fsm_result* r) { return r ? r->dist.data() : nullptr; }
+void fsm_result_free(fsm_result* r) { delete r; }
+
+// ------------------------------------------------------ stage entry points ---
+int fsm_stage_window_stats(fsm_target* t, int m, int64_t xb, int64_t xe, double* mu,
+                           double* sigma, uint8_t* invalid) {
+    if (!t || !mu || !sigma || !invalid) return fail(FSM_ERR_ARG, "null argument");
+    Geometry G;
+    if (m < 2 || !make_geometry(t->N, m, &G)) return fail(FSM_ERR_ARG, "invalid m");
+    bool fresh;
+    int rc = ensure_stats(t, G, &fresh);
+    if (rc) return rc;
+    const size_t n = (size_t)G.S * (size_t)G.kml;
+    std::vector<double> hm(n), hs(n);
+    std::vector<uint8_t> hi(n);
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    HIP_TRY(hipMemcpy(hm.data(), t->MU.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hs.data(), t->SG.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hi.data(), t->INV.p, n, hipMemcpyDeviceToHost));
+    const bool excl = xb >= 0 && xe > xb;
+    for (int64_t g = 0; g < G.ts_n; g++) {
+        const int64_t s = stat_index(G, g);
+        mu[g] = hm[s];
+        sigma[g] = hs[s];
+        invalid[g] = hi[s] || (excl && g >= xb && g < xe);
+    }
+    return FSM_OK;
+}
+
+int fsm_stage_lbip(fsm_target* t, const fsm_query_desc* d, const double* q, double* lb,
+                   double* t_u, double* t_l) {
+    if (!t || !lb) return fail(FSM_ERR_ARG, "null argument");
+    int rc = validate(d, t->N, q, d ? d->m : 0, false);
+    if (rc) return rc;
+    Geometry G;
+    if (!make_geometry(t->N, d->m, &G)) return fail(FSM_ERR_ARG, "invalid m");
+    HostQuery H;
+    prepare_query(q, d->m, d->w, d->arg, &H);
+    QueryTables Q{};
+    if ((rc = upload_query(t, H, d->m, &Q))) return rc;
+    QueryParams P = params_of(d);
+    bool fresh;
+    if ((rc = ensure_stats(t, G, &fresh))) return rc;
+    if ((rc = ensure_env(t, d->w))) return rc;
+    if ((rc = run_lbip(t, G, Q, P, false))) return rc;
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    HIP_TRY(hipMemcpy(lb, t->lb.p, sizeof(double) * (size_t)G.ts_n, hipMemcpyDeviceToHost));
+    if (t_u) HIP_TRY(hipMemcpy(t_u, t->tu.p, sizeof(double) * (size_t)G.S, hipMemcpyDeviceToHost));
+    if (t_l) HIP_TRY(hipMemcpy(t_l, t->tl.p, sizeof(double) * (size_t)G.S, hipMemcpyDeviceToHost));
+    return FSM_OK;
+}
+
+static int stage_verify(fsm_target* t, const fsm_query_desc* d, const double* q,
+                        const int64_t* win, int64_t n, double* out, bool glb) {
+    if (!t || !out || (n > 0 && !win) || n < 0) return fail(FSM_ERR_ARG, "null argument");
+    int rc = validate(d, t->N, q, d ? d->m : 0, false);
+    if (rc) return rc;
+    Geometry G;
+    if (!make_geometry(t->N, d->m, &G)) return fail(FSM_ERR_ARG, "invalid m");
+    for (int64_t i = 0; i < n; i++)
+        if (win[i] < 0 || win[i] >= G.ts_n) return fail(FSM_ERR_ARG, "window index out of range");
+    if (n == 0) return FSM_OK;
+    HostQuery H;
+    prepare_query(q, d->m, d->w, d->arg, &H);
+    QueryTables Q{};
+    if ((rc = upload_query(t, H, d->m, &Q))) return rc;
+    QueryParams P = params_of(d);
+    bool fresh;
+    if ((rc = ensure_stats(t, G, &fresh))) return rc;
+    if (glb && (rc = ensure_env(t, d->w))) return rc;
+    if ((rc = ensure_eld_scratch(t, d->w))) return rc;
+    HIP_TRY(t->list_a.ensure(sizeof(int64_t) * (size_t)n));
+    HIP_TRY(t->out_dist.ensure(sizeof(double) * (size_t)n));
+    unsigned long long cnt = (unsigned long long)n;
+    unsigned long long* Cd = t->counters.as<unsigned long long>();
+    HIP_TRY(hipMemcpyAsync(t->list_a.p, win, sizeof(int64_t) * n, hipMemcpyHostToDevice,
+                           t->stream));
+    HIP_TRY(hipMemcpyAsync(Cd + 6, &cnt, sizeof(cnt), hipMemcpyHostToDevice, t->stream));
+    VerifyArgs v = verify_args(t, G, Q, P);
+    if (glb)
+        HIP_TRY(launch_glb(t->stream, v, t->list_a.as<int64_t>(), Cd + 6, nullptr, nullptr,
+                           t->out_dist.as<double>()));
+    else
+        HIP_TRY(launch_eld(t->stream, v, t->list_a.as<int64_t>(), Cd + 6, true, nullptr,
+                           t->out_dist.as<double>(), nullptr, t->scratch.as<double>(),
+                           eld_scratch_threads()));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    HIP_TRY(hipMemcpy(out, t->out_dist.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return FSM_OK;
+}
+
+int fsm_stage_glb(fsm_target* t, const fsm_query_desc* d, const double* q, const int64_t* win,
+                  int64_t n, double* out) {
+    return stage_verify(t, d, q, win, n, out, true);
+}
+
+int fsm_stage_eld(fsm_target* t, const fsm_query_desc* d, const double* q, const int64_t* win,
+                  int64_t n, double* out) {
+    return stage_verify(t, d, q, win, n, out, false);
+}
+
+int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const double* cands,
+                  int64_t n, double* out) {
+    if (!q || !cands || !out || n < 0) return fail(FSM_ERR_ARG, "null argument");
+    if (m < 2 || w < 0 || w >= m || metric < FSM_METRIC_DTW || metric > FSM_METRIC_MSM)
+        return fail(FSM_ERR_ARG, "invalid m, w or metric");
+    if (n == 0) return FSM_OK;
+    if (fsm_device_count() < 1) return fail(FSM_ERR_NODEV, "no HIP device");
+    HIP_TRY(hipSetDevice(0));
+    DevBuf dq, dc, dout, dscr;
+    const int64_t thr = eld_scratch_threads();
+    auto done = [&](int rc) {
+        dq.release();
+        dc.release();
+        dout.release();
+        dscr.release();
+        return rc;
+    };
+    if (dq.ensure(sizeof(double) * m) || dc.ensure(sizeof(double) * (size_t)m * n) ||
+        dout.ensure(sizeof(double) * n))
+        return done(fail(FSM_ERR_OOM, "device allocation failed"));
+    if (w > 32 && dscr.ensure(sizeof(double) * 2 * (size_t)(2 * w + 2) * (size_t)thr))
+        return done(fail(FSM_ERR_OOM, "device allocation failed"));
+    if (hipMemcpy(dq.p, q, sizeof(double) * m, hipMemcpyHostToDevice) ||
+        hipMemcpy(dc.p, cands, sizeof(double) * (size_t)m * n, hipMemcpyHostToDevice))
+        return done(fail(FSM_ERR_HIP, "upload failed"));
+    if (launch_eld_pairs(nullptr, metric, m, w, arg, dq.as<double>(), dc.as<double>(), n,
+                         dout.as<double>(), dscr.as<double>(), thr) != hipSuccess)
+        return done(fail(FSM_ERR_HIP, "ELD launch failed"));
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(out, dout.p, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        return done(fail(FSM_ERR_HIP, "ELD failed"));
+    return done(FSM_OK);
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+// engine_internal.h -- types shared by the gfx950 kernels and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fsm {
+
+enum Metric : int { DTW = 1, EDR = 2, ERP = 3, LCSS = 4, MSM = 5 };
+
+// src/ElasticMeasures.cpp:7 -- value of out-of-band DP cells.
+constexpr double kEldSentinel = 1e18;
+// include/def.h:7 -- value glb_elastic returns when it abandons early.
+constexpr double kGlbAbandon = 1008610086.0;
+
+// Macro-segmentation of one query length over a target of N samples
+// (src/fsmeld.cpp:28-37, :647-661).  Segment s < forsize starts at s*kml;
+// the tail segment (index forsize, present when tail > 0) starts at N - m_k
+// and evaluates only its windows [tail_first, kml).
+struct Geometry {
+    int64_t N;
+    int m;
+    int m_k;
+    int64_t kml;
+    int64_t ts_n;
+    int64_t forsize;
+    int64_t tail;
+    int tail_first;
+    int64_t S;  // forsize + (tail > 0)
+};
+
+// Device-side query tables, uploaded once per query.
+struct QueryTables {
+    const double* qn;   // z-normalised query [m]
+    const double* LQ;   // query envelope, radius w [m]
+    const double* UQ;
+    const double* qs;   // sorted qn [m]
+    const double* S1;   // prefix sums over qs of q, q^2, (q - arg)^2 [m + 1]
+    const double* S2;
+    const double* SD;
+    const double* TU;   // fl(qs - arg), fl(qs + arg) [m]
+    const double* TL;
+};
+
+struct QueryParams {
+    int metric;
+    int m;
+    int w;
+    double arg;
+    double cutoff;
+    int64_t xb, xe;  // exclusion [xb, xe), xb < 0: none
+};
+
+// Window statistics live segment-major: entry s*kml + j is window j of
+// segment s.  Global window g maps to that layout by stat_index().
+__host__ __device__ inline int64_t stat_index(const Geometry& G, int64_t g) {
+    const int64_t full = G.forsize * G.kml;
+    return g < full ? g : g + G.tail_first;
+}
+
+__host__ __device__ inline bool is_excluded(const QueryParams& P, int64_t g) {
+    return P.xb >= 0 && P.xe > P.xb && g >= P.xb && g < P.xe;
+}
+
+// NaN-propagating min / max (any NaN operand gives NaN).
+__host__ __device__ inline double nmin(double a, double b) { return (a < b || a != a) ? a : b; }
+__host__ __device__ inline double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
+// ------------------------------------------------------------ launchers ---
+hipError_t launch_window_stats(hipStream_t st, const double* T, const Geometry& G, double* MU,
+                               double* SG, uint8_t* INV);
+hipError_t launch_envelope(hipStream_t st, const double* T, int64_t N, int w, double* Lraw,
+                           double* Uraw);
+
+struct LbipArgs {
+    const double* T;
+    const double* MU;
+    const double* SG;
+    const uint8_t* INV;
+    const double* Lraw;
+    const double* Uraw;
+    QueryTables q;
+    QueryParams p;
+    Geometry G;
+    double* LB;        // [ts_n]
+    double* TUo;       // [S] or null
+    double* TLo;       // [S] or null
+    double* rep_lb;    // [S] or null: smallest LB of the segment's valid windows
+    int64_t* rep_idx;  // [S] or null
+    void* slab;        // per-block global scratch
+    size_t slab_bytes_per_block;
+    int blocks;
+};
+size_t lbip_slab_bytes(const Geometry& G);
+size_t lbip_lds_bytes(const Geometry& G);
+hipError_t launch_lbip(hipStream_t st, const LbipArgs& a);
+
+// Compaction: windows g in [0, ts_n) that are valid, not excluded and (for
+// use_lb) not pruned by the LB_IP test at P.cutoff.  counters[0] += kept,
+// counters[1] += valid.
+hipError_t launch_compact(hipStream_t st, const Geometry& G, const QueryParams& P,
+                          const uint8_t* INV, const double* LB, bool use_lb, int64_t* out,
+                          unsigned long long* counters);
+
+struct VerifyArgs {
+    const double* T;
+    const double* MU;
+    const double* SG;
+    const double* Lraw;
+    const double* Uraw;
+    QueryTables q;
+    QueryParams p;
+    Geometry G;
+};
+// GLB filter over in[0 .. *in_count): survivors appended to out (out_count);
+// values written to vals[i] when vals != null (stage-level entry point).
+hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
+                      const unsigned long long* in_count, int64_t* out,
+                      unsigned long long* out_count, double* vals);
+// Exact distances over in[0 .. *in_count).  keep_all: write dist[i] for every
+// input; else append (index, distance) for distances <= P.cutoff.
+hipError_t launch_eld(hipStream_t st, const VerifyArgs& a, const int64_t* in,
+                      const unsigned long long* in_count, bool keep_all, int64_t* out_idx,
+                      double* out_dist, unsigned long long* out_count, double* scratch,
+                      int64_t scratch_threads);
+// Exact distances of normalised rows (no target).
+hipError_t launch_eld_pairs(hipStream_t st, int metric, int m, int w, double arg,
+                            const double* q, const double* cands, int64_t n, double* out,
+                            double* scratch, int64_t scratch_threads);
+int64_t eld_scratch_threads();
+
+}  // namespace fsm

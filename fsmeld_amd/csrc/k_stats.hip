@@ -1,0 +1,141 @@
+// k_stats.hip -- per-window mean / std / invalid flag, and the target's
+// sliding envelope, on gfx950.
+//
+// Window statistics must equal the reference's bits: every window's mean and
+// std come from the running-sum recurrence of the macro-segment that
+// evaluates it (src/mvmean.cpp:9-46), so one lane owns one segment and runs
+// that recurrence in the reference's order.  Build with -ffp-contract=off.
+#include "engine_internal.h"
+
+namespace fsm {
+
+// One lane per macro-segment (src/mvmean.cpp:9-46 and the invalid test of
+// src/fsmeld.cpp:110-129).  Outputs are segment-major [S][kml].
+__global__ void __launch_bounds__(64) k_window_stats(const double* __restrict__ T, Geometry G,
+                                                     double* __restrict__ MU,
+                                                     double* __restrict__ SG,
+                                                     uint8_t* __restrict__ INV) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= G.S) return;
+    const double* a = (s < G.forsize) ? T + s * G.kml : T + (G.N - G.m_k);
+    double* mu = MU + s * G.kml;
+    double* si = SG + s * G.kml;
+    uint8_t* inv = INV + s * G.kml;
+    const int l = G.m;
+    const double dl = (double)l;
+    const double nan = __builtin_nan("");
+    double sum1 = 0.0, sum2 = 0.0;
+    int bad = 0;
+    for (int i = 0; i < l; i++) {
+        const double v = a[i];
+        if (__builtin_isfinite(v)) {
+            sum1 += v;
+            sum2 += v * v;
+        } else {
+            ++bad;
+        }
+    }
+    {
+        const double m0 = bad ? nan : sum1 / dl;
+        const double s0 = bad ? nan : sqrt(sum2 / dl - m0 * m0);
+        mu[0] = m0;
+        si[0] = s0;
+        inv[0] = (bad > 0) || !__builtin_isfinite(s0) || s0 <= 1e-12;
+    }
+    const int64_t kml = G.kml;
+    for (int64_t i = 1; i < kml; i++) {
+        const double vo = a[i - 1];
+        const double vi = a[i + l - 1];
+        const bool of = __builtin_isfinite(vo);
+        const bool nf = __builtin_isfinite(vi);
+        const double out = of ? vo : 0.0;
+        const double in = nf ? vi : 0.0;
+        bad += (int)!nf - (int)!of;
+        sum1 = sum1 - out + in;
+        sum2 = sum2 - out * out + in * in;
+        const double mi = bad ? nan : sum1 / dl;
+        const double sg = bad ? nan : sqrt(sum2 / dl - mi * mi);
+        mu[i] = mi;
+        si[i] = sg;
+        inv[i] = (bad > 0) || !__builtin_isfinite(sg) || sg <= 1e-12;
+    }
+}
+
+hipError_t launch_window_stats(hipStream_t st, const double* T, const Geometry& G, double* MU,
+                               double* SG, uint8_t* INV) {
+    const int bs = 64;
+    const int64_t blocks = (G.S + bs - 1) / bs;
+    hipLaunchKernelGGL(k_window_stats, dim3((unsigned)blocks), dim3(bs), 0, st, T, G, MU, SG,
+                       INV);
+    return hipGetLastError();
+}
+
+// Centred envelope of the whole target, radius w, clipped at its ends and
+// NaN-propagating: L[p] = min T[max(0,p-w) .. min(N-1,p+w)], U = max.
+// For NaN-free ranges this equals lower_upper_lemire (src/lower_upper_lemire.cpp:8-44).
+// A block stages a tile plus halo in LDS and builds the min/max over blocks
+// of 2^K samples by K doubling passes; each output combines two overlapping
+// 2^K blocks.
+constexpr int kEnvTile = 1024;
+constexpr int kEnvThreads = 256;
+
+__global__ void __launch_bounds__(kEnvThreads) k_envelope(const double* __restrict__ T,
+                                                          int64_t N, int w, int K,
+                                                          double* __restrict__ L,
+                                                          double* __restrict__ U) {
+    extern __shared__ double lds[];
+    const int span = kEnvTile + 2 * w;
+    double* A = lds;         // running min over [x, x + 2^p)
+    double* B = lds + span;  // running max
+    const int64_t p0 = (int64_t)blockIdx.x * kEnvTile;
+    const double inf = __builtin_inf();
+    for (int x = threadIdx.x; x < span; x += blockDim.x) {
+        const int64_t src = p0 - w + x;
+        const bool in = src >= 0 && src < N;
+        const double v = in ? T[src] : 0.0;
+        A[x] = in ? v : inf;
+        B[x] = in ? v : -inf;
+    }
+    __syncthreads();
+    for (int p = 0; p < K; p++) {
+        const int d = 1 << p;
+        double ra[8], rb[8];  // span <= 8 * kEnvThreads is checked by the launcher
+        int c = 0;
+        for (int x = threadIdx.x; x < span; x += blockDim.x, c++) {
+            const int y = x + d;
+            ra[c] = (y < span) ? nmin(A[x], A[y]) : A[x];
+            rb[c] = (y < span) ? nmax(B[x], B[y]) : B[x];
+        }
+        __syncthreads();
+        c = 0;
+        for (int x = threadIdx.x; x < span; x += blockDim.x, c++) {
+            A[x] = ra[c];
+            B[x] = rb[c];
+        }
+        __syncthreads();
+    }
+    const int len = 2 * w + 1;
+    const int off = len - (1 << K);
+    for (int x = threadIdx.x; x < kEnvTile; x += blockDim.x) {
+        const int64_t pos = p0 + x;
+        if (pos >= N) break;
+        // window [pos - w, pos + w] is LDS [x, x + 2w]
+        L[pos] = nmin(A[x], A[x + off]);
+        U[pos] = nmax(B[x], B[x + off]);
+    }
+}
+
+hipError_t launch_envelope(hipStream_t st, const double* T, int64_t N, int w, double* L,
+                           double* U) {
+    const int span = kEnvTile + 2 * w;
+    if (span > 8 * kEnvThreads) return hipErrorInvalidValue;
+    int K = 0;
+    while ((2 << K) <= 2 * w + 1) K++;  // 2^K <= 2w+1 < 2^(K+1)
+    const int64_t blocks = (N + kEnvTile - 1) / kEnvTile;
+    const size_t lds = sizeof(double) * 2 * (size_t)span;
+    hipLaunchKernelGGL(k_envelope, dim3((unsigned)blocks), dim3(kEnvThreads), lds, st, T, N, w,
+                       K, L, U);
+    return hipGetLastError();
+}
+
+}  // namespace fsm

@@ -1,0 +1,150 @@
+/*
+ * fsmeld.h -- C ABI of the MI355X (gfx950) FSMELD engine: exact z-normalised
+ * subsequence range search under elastic distances with the
+ * LB_IP -> GLB -> ELD cascade, computed by HIP kernels on one GPU.
+ *
+ * The reference is one executable with a C++ free-function API
+ * (include/funcs.h, include/fsmeld_internal.h of lzzt0123/FSMELD); each entry
+ * point below names the reference function it replaces.  Conventions:
+ *   - plain pointers and sizes, host memory on both sides of the boundary;
+ *   - int status codes (FSM_OK = 0, negative on failure); fsm_last_error()
+ *     returns a message for the calling thread's last failure;
+ *   - result objects are library owned and released with fsm_result_free();
+ *   - a target handle is reentrant for sequential callers; concurrent calls on
+ *     one handle must be serialised by the caller.
+ * Window indices are 0-based candidate start positions in the target.
+ */
+#ifndef FSMELD_H
+#define FSMELD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FSM_OK 0
+#define FSM_ERR_ARG (-1)          /* invalid argument (message in fsm_last_error) */
+#define FSM_ERR_OOM (-2)          /* device or host allocation failed */
+#define FSM_ERR_HIP (-3)          /* HIP runtime error */
+#define FSM_ERR_FEWER_THAN_K (-4) /* cutoff: fewer valid candidates than k */
+#define FSM_ERR_NODEV (-5)        /* no usable GPU */
+#define FSM_ERR_IO (-6)           /* target file unreadable */
+
+/* metric codes: include/funcs.h:10-18 of the reference */
+#define FSM_METRIC_DTW 1
+#define FSM_METRIC_EDR 2
+#define FSM_METRIC_ERP 3
+#define FSM_METRIC_LCSS 4
+#define FSM_METRIC_MSM 5
+
+/* query modes: the reference's three commands (src/main.cpp:90-104) */
+#define FSM_MODE_SEARCH 0 /* FSMELD()                 include/funcs.h:67-68 */
+#define FSM_MODE_CUTOFF 1 /* FSMELD_find_threshold()  include/funcs.h:64-65 */
+#define FSM_MODE_EXACT 2  /* FSMELD_exact(top_k = 0)  include/funcs.h:70-72 */
+
+typedef struct fsm_target fsm_target;
+typedef struct fsm_result fsm_result;
+
+typedef struct {
+    int mode;              /* FSM_MODE_* */
+    int metric;            /* FSM_METRIC_* */
+    int m;                 /* query / window length, m >= 2 */
+    int w;                 /* Sakoe-Chiba radius, 0 <= w < m */
+    double arg;            /* ERP gap g, EDR/LCSS epsilon, MSM cost c; ignored by DTW */
+    double cutoff;         /* search / exact: distance cutoff xi >= 0 */
+    int k;                 /* cutoff: rank k >= 1 */
+    int64_t exclude_begin; /* candidate exclusion [begin, end); -1 = none */
+    int64_t exclude_end;
+} fsm_query_desc;
+
+typedef struct {
+    uint64_t valid;       /* kk1: valid, non-excluded windows */
+    uint64_t invalid;     /* windows skipped as invalid or excluded */
+    uint64_t after_lbip;  /* kk2: survivors of LB_IP */
+    uint64_t after_glb;   /* kk3: survivors of GLB */
+    uint64_t exact_evals; /* exact elastic distances computed */
+    uint64_t matches;     /* result count */
+    double threshold;     /* cutoff mode: k-th smallest distance */
+    int64_t ts_n, k_m_l, forsize, tail;
+    int m_k;
+    /* device time per stage from HIP events on the engine's stream, ms */
+    double ms_stats, ms_lbip, ms_glb, ms_eld;
+    double ms_total; /* host wall time of the call, ms */
+} fsm_stats;
+
+/* ---- device and errors ---- */
+int fsm_device_count(void);
+const char* fsm_last_error(void);
+
+/* ---- device-resident target (replaces the reference's per-segment ifstream
+ *      reads, src/fsmeld.cpp:740-746 and read_tail_segment :159-168) ---- */
+int fsm_target_open(const double* host_values, int64_t n, fsm_target** out);
+int fsm_target_open_file(const char* path, fsm_target** out);
+int64_t fsm_target_length(const fsm_target* t);
+void fsm_target_close(fsm_target* t);
+
+/* ---- query: search / cutoff / exact over the whole target ----
+ * query_raw holds at least m raw (un-normalised) query values; the engine
+ * z-normalises them as src/fsmeld.cpp:622-639 does.
+ * search/exact: indices ascending, distances are the exact ELD values.
+ * cutoff: the k (distance, index)-smallest windows in rank order, as printed
+ *         by src/fsmeld.cpp:544-553; stats.threshold is the k-th distance. */
+int fsm_query(fsm_target* t, const fsm_query_desc* desc, const double* query_raw,
+              int64_t query_len, fsm_result** out);
+int fsm_result_stats(const fsm_result* r, fsm_stats* out);
+int64_t fsm_result_count(const fsm_result* r);
+const int64_t* fsm_result_indices(const fsm_result* r);
+const double* fsm_result_distances(const fsm_result* r);
+void fsm_result_free(fsm_result* r);
+
+/* ---- stage-level entry points (parity tests; outputs in host memory) ---- */
+
+/* Per-window mean, std and invalid flag in global window order (ts_n each),
+ * each from the recurrence of the macro-segment that evaluates the window.
+ * Replaces mvmean() include/funcs.h:39 + precompute_invalid_mask_fast()
+ * include/fsmeld_internal.h:22-25 (exclusion [xb, xe) applied as there). */
+int fsm_stage_window_stats(fsm_target* t, int m, int64_t xb, int64_t xe, double* mu,
+                           double* sigma, uint8_t* invalid);
+
+/* LB_IP value per window (ts_n; NaN for invalid windows) and the per-segment
+ * thresholds t_u, t_l (forsize + [tail > 0] each; may be NULL).  Replaces the
+ * per-segment LB_IP stage of src/fsmeld.cpp:756-808 (min_max_window,
+ * lower_upper_lemire, compute_normalized_arrays, find_t*_lp*, q_to_h_*,
+ * s_to_g_*, inv_and_padding_y / FFT_x / cov_IFFT, trans). */
+int fsm_stage_lbip(fsm_target* t, const fsm_query_desc* desc, const double* query_raw,
+                   double* lb, double* t_u, double* t_l);
+
+/* GLB value for each given window (glb_elastic, include/funcs.h:84-85, called
+ * as src/fsmeld.cpp:827-829 with bsf = desc->cutoff). */
+int fsm_stage_glb(fsm_target* t, const fsm_query_desc* desc, const double* query_raw,
+                  const int64_t* windows, int64_t n, double* out);
+
+/* Exact elastic distance for each given window
+ * (exact_elastic_distance, include/fsmeld_internal.h:27). */
+int fsm_stage_eld(fsm_target* t, const fsm_query_desc* desc, const double* query_raw,
+                  const int64_t* windows, int64_t n, double* out);
+
+/* Exact elastic distance of n already-normalised rows cands[n][m] against q[m]
+ * (dtw/erp/edr/lcss/msm, include/funcs.h:76-81, argument order of
+ * src/fsmeld.cpp:170-183).  No target needed. */
+int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const double* cands,
+                  int64_t n, double* out);
+
+/* ---- seeded synthetic workloads (host only; no GPU needed) ----
+ * SplitMix64-seeded xoshiro256** with Box-Muller normals.
+ * Target: random walk of N(0,1) increments; spike_frac of positions get an
+ * added N(0, spike_sd) outlier; n_const_runs runs t[a..a+const_len) = t[a];
+ * n_nans single NaN samples.  Query: a copy of m target values from a seeded
+ * offset whose window is finite and free of repeated samples, plus
+ * N(0, noise_sd) noise; *offset receives the source position. */
+int fsm_synth_target(uint64_t seed, int64_t n, double spike_frac, double spike_sd,
+                     int n_const_runs, int const_len, int n_nans, double* out);
+int fsm_synth_query(uint64_t seed, const double* target, int64_t n, int m, double noise_sd,
+                    double* out_query, int64_t* offset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

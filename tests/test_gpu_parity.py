@@ -1,0 +1,207 @@
+"""GPU parity: every stage and every command of the HIP engine against the
+CPU oracle on the same seeded inputs.
+
+Bar: bit-exact for window statistics, invalid flags, GLB values, exact
+distances, match sets, cutoff thresholds and rankings.  LB_IP values are
+compared where both sides chose the same thresholds (within 1e-9 relative,
+the FFT guard of include/fsmeld_internal.h:9-19) and are checked to be
+admissible (never above an exact distance)."""
+import numpy as np
+import pytest
+
+import fsmeld_amd as F
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+METRICS = [("dtw", 0.0), ("erp", 0.0), ("edr", 0.25), ("lcss", 0.25), ("msm", 0.5)]
+
+
+def bits(x):
+    return np.asarray(x, dtype=np.float64).view(np.uint64)
+
+
+def same_bits(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    both_nan = np.isnan(a) & np.isnan(b)
+    return np.all((bits(a) == bits(b)) | both_nan)
+
+
+def make_target(seed, n, nans=3, runs=4, spikes=0.001):
+    return F.synth_target(seed, n, spikes, 5.0, runs, 200, nans)
+
+
+@pytest.fixture(scope="module")
+def small():
+    t = make_target(3, 1 << 15)
+    return t, F.Target(t)
+
+
+@pytest.mark.parametrize("m", [16, 64, 128])
+def test_window_stats_bitexact(gpu, small, m):
+    t, T = small
+    mu, sg, inv = T.window_stats(m)
+    omu, osg, oinv = O.window_stats(t, m)
+    assert same_bits(mu, omu)
+    assert same_bits(sg, osg)
+    assert np.array_equal(inv, oinv)
+    assert inv.sum() > 0  # the NaNs and constant runs produce invalid windows
+
+
+def test_window_stats_exclusion(gpu, small):
+    t, T = small
+    mu, sg, inv = T.window_stats(64, exclude=(1000, 5000))
+    _, _, oinv = O.window_stats(t, 64, exclude=(1000, 5000))
+    assert np.array_equal(inv, oinv)
+    assert inv[1000:5000].all()
+
+
+@pytest.mark.parametrize("metric,arg", METRICS)
+@pytest.mark.parametrize("m,w", [(2, 0), (7, 1), (33, 0), (40, 6), (64, 25), (50, 33), (24, 23)])
+def test_eld_pairs_bitexact(gpu, metric, arg, m, w):
+    rng = np.random.default_rng(m * 100 + w)
+    q = rng.normal(size=m)
+    c = rng.normal(size=(97, m))
+    c[:10] = q + rng.normal(size=(10, m)) * 1e-3   # near matches
+    c[10:13] = q                                    # identical
+    c[13:20] = np.round(c[13:20], 1)                # ties
+    got = F.eld_pairs(metric, q, c, w, arg)
+    want = np.array([O.eld(metric, q, row, w, arg) for row in c])
+    assert same_bits(got, want)
+
+
+@pytest.mark.parametrize("metric,arg", METRICS)
+def test_eld_and_glb_windows_bitexact(gpu, small, metric, arg):
+    t, T = small
+    m, w = 64, 6
+    q, _ = F.synth_query(5, t, m, 0.1)
+    mu, sg, inv = O.window_stats(t, m)
+    win = np.flatnonzero(~inv)[::97][:300]
+    d = T.eld(q, m, metric, w, arg, win)
+    qn = O.normalize_query(q, m)
+    LQ, UQ = O.lemire(qn, w)
+    want_d = np.array([O.eld(metric, qn, (t[g:g + m] - mu[g]) / sg[g], w, arg) for g in win])
+    assert same_bits(d, want_d)
+    for bsf in (np.median(want_d), np.quantile(want_d, 0.1), 1e30):
+        g = T.glb(q, m, metric, w, arg, win, bsf)
+        want_g = np.array([O.glb(metric, qn, (t[x:x + m] - mu[x]) / sg[x], arg, LQ, UQ, w, bsf)
+                           for x in win])
+        assert same_bits(g, want_g)
+        assert np.all(want_g[want_g < 1e9] <= want_d[want_g < 1e9] + 1e-12)
+
+
+@pytest.mark.parametrize("metric,arg", METRICS)
+def test_lbip_values(gpu, metric, arg):
+    t = make_target(8, 1 << 14, nans=0, runs=0, spikes=0.0)  # NaN-free: deques exact
+    T = F.Target(t)
+    m, w = 32, 3
+    q, _ = F.synth_query(8, t, m, 0.1)
+    lb, tu, tl = T.lbip(q, m, metric, w, arg)
+    olb, otu, otl = O.lbip_all(t, q, m, metric, w, arg)
+    same_t = (tu == otu) & (tl == otl)
+    assert same_t.mean() >= 0.9, f"thresholds agree in {same_t.mean():.3f} of segments"
+    G = F.geometry(t.size, m)
+    seg = np.minimum(np.arange(G["ts_n"]) // G["k_m_l"], G["S"] - 1)
+    sel = same_t[seg]
+    a, b = lb[sel], olb[sel]
+    assert np.all(np.abs(a - b) <= 1e-9 * np.maximum(1.0, np.abs(b)))
+    # admissibility: LB_IP never exceeds the exact distance (beyond the guard)
+    _, _, exact_inv = O.window_stats(t, m)
+    win = np.flatnonzero(~exact_inv)[::37]
+    d = T.eld(q, m, metric, w, arg, win)
+    assert np.all(lb[win] <= d + 1e-9)
+
+
+def run_all_modes(t, q, m, metric, w, arg, k, exclude=None):
+    T = F.Target(t)
+    thr, ranks, rd, _ = O.cutoff(t, q, m, k, metric, w, arg, exclude=exclude)
+    g = T.cutoff(q, m, k, metric, w, arg, exclude=exclude)
+    assert bits(g.threshold) == bits(thr)
+    assert np.array_equal(g.indices, ranks)
+    assert same_bits(g.distances, rd)
+    s = T.search(q, m, thr, metric, w, arg, exclude=exclude)
+    e = T.exact(q, m, thr, metric, w, arg, exclude=exclude)
+    os_ids, ost = O.search(t, q, m, thr, metric, w, arg, exclude=exclude)
+    oe_ids, oe_d, oest = O.exact(t, q, m, thr, metric, w, arg, exclude=exclude)
+    assert np.array_equal(s.indices, os_ids)
+    assert np.array_equal(e.indices, oe_ids)
+    assert np.array_equal(s.indices, e.indices)
+    assert same_bits(e.distances, oe_d)
+    assert same_bits(s.distances, oe_d)
+    assert s.stats["valid"] == ost["valid"] == oest["valid"]
+    assert s.stats["invalid"] == ost["invalid"]
+    return s, ost
+
+
+@pytest.mark.parametrize("metric,arg", METRICS)
+def test_commands_match_oracle(gpu, metric, arg):
+    t = make_target(21, 1 << 15)
+    q, _ = F.synth_query(21, t, 64, 0.1)
+    s, ost = run_all_modes(t, q, 64, metric, 4, arg, 10)
+    # LB_IP / GLB survivor counts track the reference restatement closely
+    assert abs(int(s.stats["after_lbip"]) - int(ost["after_lbip"])) <= 0.02 * ost["valid"] + 5
+
+
+@pytest.mark.parametrize("case", ["tail0", "single_segment", "w0", "m2", "exclusion", "big_w"])
+def test_command_edge_cases(gpu, case):
+    if case == "tail0":      # ts_n a multiple of k_m_l: no tail segment
+        m = 32
+        G = F.geometry(1 << 14, m)
+        n = G["k_m_l"] * 20 + m - 1
+        t = make_target(31, n)
+        q, _ = F.synth_query(31, t, m, 0.1)
+        assert F.geometry(n, m)["tail"] == 0
+        run_all_modes(t, q, m, "dtw", 3, 0.0, 5)
+    elif case == "single_segment":   # N < 8m: one macro-segment of length N
+        t = make_target(32, 700, nans=1, runs=1)
+        q, _ = F.synth_query(32, t, 128, 0.1)
+        run_all_modes(t, q, 128, "erp", 6, 0.0, 5)
+    elif case == "w0":
+        t = make_target(33, 1 << 13)
+        q, _ = F.synth_query(33, t, 40, 0.1)
+        for metric, arg in METRICS:
+            run_all_modes(t, q, 40, metric, 0, arg, 3)
+    elif case == "m2":
+        t = make_target(34, 4000, nans=0, runs=0)
+        q = np.array([0.0, 1.0])
+        run_all_modes(t, q, 2, "msm", 1, 0.5, 4)
+    elif case == "exclusion":
+        t = make_target(35, 1 << 14)
+        q, off = F.synth_query(35, t, 64, 0.0)
+        run_all_modes(t, q, 64, "dtw", 3, 0.0, 8, exclude=(off - 100, off + 100))
+    elif case == "big_w":   # radius above the register band variants
+        t = make_target(36, 1 << 13, nans=0, runs=1)
+        q, _ = F.synth_query(36, t, 96, 0.1)
+        for metric, arg in METRICS:
+            run_all_modes(t, q, 96, metric, 40, arg, 3)
+
+
+def test_fewer_than_k(gpu):
+    t = make_target(40, 600, nans=0, runs=0)
+    t[:] = np.nan
+    t[100:400] = np.cumsum(np.random.default_rng(0).normal(size=300))
+    q, _ = F.synth_query(40, t[100:400].copy(), 64, 0.1)
+    T = F.Target(t)
+    with pytest.raises(F.FsmeldError) as e:
+        T.cutoff(q, 64, 300, "dtw", 3, 0.0)   # 237 valid windows, k <= N - m + 1
+    assert e.value.code == -4
+    with pytest.raises(LookupError):
+        O.cutoff(t, q, 64, 300, "dtw", 3, 0.0)
+
+
+def test_plain_small_config(gpu):
+    """The plain small configuration at full size: 2^20 random walk, 10
+    constant runs, 10 NaNs, m=128, DTW w=6, cutoff from k=10."""
+    from fsmeld_amd import workloads
+    c = workloads.plain_small()
+    t, q = c["target"], c["query"]
+    T = F.Target(t)
+    thr, ranks, rd, _ = O.cutoff(t, q, 128, 10, "dtw", 6, 0.0)
+    g = T.cutoff(q, 128, 10, "dtw", 6, 0.0)
+    assert bits(g.threshold) == bits(thr) and np.array_equal(g.indices, ranks)
+    s = T.search(q, 128, thr, "dtw", 6, 0.0)
+    e = T.exact(q, 128, thr, "dtw", 6, 0.0)
+    os_ids, _ = O.search(t, q, 128, thr, "dtw", 6, 0.0, threads=8)
+    assert np.array_equal(s.indices, os_ids)
+    assert np.array_equal(e.indices, os_ids)
+    assert c["offset"] in set(s.indices.tolist())
