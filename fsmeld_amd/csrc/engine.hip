@@ -114,7 +114,7 @@ void host_envelope(const std::vector<double>& a, int r, std::vector<double>& L,
 }
 
 struct HostQuery {
-    std::vector<double> qn, LQ, UQ, qs, S1, S2, SD, TU, TL;
+    std::vector<double> qn, LQ, UQ, qs, S1, S2, SD, TU, TL, XG;
 };
 
 // src/fsmeld.cpp:622-643 (normalisation and sorted copy), :721 (envelope).
@@ -130,6 +130,8 @@ void prepare_query(const double* q_raw, int m, int w, double arg, HostQuery* H) 
     H->qn.resize(m);
     for (int i = 0; i < m; i++) H->qn[i] = (q_raw[i] - mean) / sd;
     host_envelope(H->qn, w, H->LQ, H->UQ);
+    H->XG.resize(m);
+    for (int i = 0; i < m; i++) H->XG[i] = (H->qn[i] - arg) * (H->qn[i] - arg);  // dist(x, g)
     H->qs = H->qn;
     std::sort(H->qs.begin(), H->qs.end());
     H->S1.assign(m + 1, 0.0);
@@ -221,11 +223,11 @@ namespace {
 
 int upload_query(fsm_target* t, const HostQuery& H, int m, QueryTables* Q) {
     const size_t stride = (size_t)m + 1;
-    HIP_TRY(t->qtab.ensure(sizeof(double) * stride * 9));
-    std::vector<double> buf(stride * 9, 0.0);
-    const std::vector<double>* parts[9] = {&H.qn, &H.LQ, &H.UQ, &H.qs, &H.S1,
-                                           &H.S2, &H.SD, &H.TU, &H.TL};
-    for (int k = 0; k < 9; k++)
+    HIP_TRY(t->qtab.ensure(sizeof(double) * stride * 10));
+    std::vector<double> buf(stride * 10, 0.0);
+    const std::vector<double>* parts[10] = {&H.qn, &H.LQ, &H.UQ, &H.qs, &H.S1,
+                                            &H.S2, &H.SD, &H.TU, &H.TL, &H.XG};
+    for (int k = 0; k < 10; k++)
         std::copy(parts[k]->begin(), parts[k]->end(), buf.begin() + k * stride);
     HIP_TRY(hipMemcpyAsync(t->qtab.p, buf.data(), sizeof(double) * buf.size(),
                            hipMemcpyHostToDevice, t->stream));
@@ -239,6 +241,7 @@ int upload_query(fsm_target* t, const HostQuery& H, int m, QueryTables* Q) {
     Q->SD = d + 6 * stride;
     Q->TU = d + 7 * stride;
     Q->TL = d + 8 * stride;
+    Q->XG = d + 9 * stride;
     // the staging vector must outlive the async copy
     HIP_TRY(hipStreamSynchronize(t->stream));
     return FSM_OK;
@@ -716,25 +719,37 @@ int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const d
     if (n == 0) return FSM_OK;
     if (fsm_device_count() < 1) return fail(FSM_ERR_NODEV, "no HIP device");
     HIP_TRY(hipSetDevice(0));
-    DevBuf dq, dc, dout, dscr;
+    DevBuf dq, dxg, dc, dout, dscr;
     const int64_t thr = eld_scratch_threads();
     auto done = [&](int rc) {
         dq.release();
+        dxg.release();
         dc.release();
         dout.release();
         dscr.release();
         return rc;
     };
-    if (dq.ensure(sizeof(double) * m) || dc.ensure(sizeof(double) * (size_t)m * n) ||
+    if (dq.ensure(sizeof(double) * m) || dxg.ensure(sizeof(double) * m) ||
+        dc.ensure(sizeof(double) * (size_t)m * n) ||
         dout.ensure(sizeof(double) * n))
         return done(fail(FSM_ERR_OOM, "device allocation failed"));
     if (w > 32 && dscr.ensure(sizeof(double) * 2 * (size_t)(2 * w + 2) * (size_t)thr))
         return done(fail(FSM_ERR_OOM, "device allocation failed"));
+    std::vector<double> xg(m);
+    for (int i = 0; i < m; i++) xg[i] = (q[i] - arg) * (q[i] - arg);
     if (hipMemcpy(dq.p, q, sizeof(double) * m, hipMemcpyHostToDevice) ||
+        hipMemcpy(dxg.p, xg.data(), sizeof(double) * m, hipMemcpyHostToDevice) ||
         hipMemcpy(dc.p, cands, sizeof(double) * (size_t)m * n, hipMemcpyHostToDevice))
         return done(fail(FSM_ERR_HIP, "upload failed"));
-    if (launch_eld_pairs(nullptr, metric, m, w, arg, dq.as<double>(), dc.as<double>(), n,
-                         dout.as<double>(), dscr.as<double>(), thr) != hipSuccess)
+    VerifyArgs va{};
+    va.q.qn = dq.as<double>();
+    va.q.XG = dxg.as<double>();
+    va.p.metric = metric;
+    va.p.m = m;
+    va.p.w = w;
+    va.p.arg = arg;
+    if (launch_eld_pairs(nullptr, va, dc.as<double>(), n, dout.as<double>(), dscr.as<double>(),
+                         thr) != hipSuccess)
         return done(fail(FSM_ERR_HIP, "ELD launch failed"));
     if (hipDeviceSynchronize() != hipSuccess ||
         hipMemcpy(out, dout.p, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)

@@ -40,6 +40,7 @@ struct QueryTables {
     const double* SD;
     const double* TU;   // fl(qs - arg), fl(qs + arg) [m]
     const double* TL;
+    const double* XG;   // (qn - arg)^2 [m], the ERP gap cost of each query sample
 };
 
 struct QueryParams {
@@ -65,6 +66,37 @@ __host__ __device__ inline bool is_excluded(const QueryParams& P, int64_t g) {
 // NaN-propagating min / max (any NaN operand gives NaN).
 __host__ __device__ inline double nmin(double a, double b) { return (a < b || a != a) ? a : b; }
 __host__ __device__ inline double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
+// Block-aggregated append: every thread of the block calls it (uniform
+// control flow); lanes with keep = true write value (and dval) to
+// consecutive slots of out / outd claimed with one atomicAdd per block.
+// sh must hold blockDim.x / 64 + 1 entries.
+__device__ inline void block_append(bool keep, int64_t value, double dval, int64_t* out,
+                                    double* outd, unsigned long long* count,
+                                    unsigned long long* sh) {
+    const uint64_t mask = __ballot(keep);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = (int)(blockDim.x >> 6);
+    if (lane == 0) sh[wid] = (unsigned long long)__builtin_popcountll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int k = 0; k < nw; k++) {
+            const unsigned long long c = sh[k];
+            sh[k] = tot;
+            tot += c;
+        }
+        sh[nw] = tot ? atomicAdd(count, tot) : 0ull;
+    }
+    __syncthreads();
+    if (keep) {
+        const uint64_t below = lane ? (mask & ((1ull << lane) - 1)) : 0ull;
+        const unsigned long long pos = sh[nw] + sh[wid] + (unsigned long long)__builtin_popcountll(below);
+        out[pos] = value;
+        if (outd) outd[pos] = dval;
+    }
+    __syncthreads();
+}
 
 // ------------------------------------------------------------ launchers ---
 hipError_t launch_window_stats(hipStream_t st, const double* T, const Geometry& G, double* MU,
@@ -123,10 +155,10 @@ hipError_t launch_eld(hipStream_t st, const VerifyArgs& a, const int64_t* in,
                       const unsigned long long* in_count, bool keep_all, int64_t* out_idx,
                       double* out_dist, unsigned long long* out_count, double* scratch,
                       int64_t scratch_threads);
-// Exact distances of normalised rows (no target).
-hipError_t launch_eld_pairs(hipStream_t st, int metric, int m, int w, double arg,
-                            const double* q, const double* cands, int64_t n, double* out,
-                            double* scratch, int64_t scratch_threads);
+// Exact distances of normalised rows cands[n][m] (no target); a.q.qn / a.q.XG
+// and a.p.{metric, m, w, arg} describe the query.
+hipError_t launch_eld_pairs(hipStream_t st, const VerifyArgs& a, const double* cands,
+                            int64_t n, double* out, double* scratch, int64_t scratch_threads);
 int64_t eld_scratch_threads();
 
 }  // namespace fsm
