@@ -717,6 +717,11 @@ int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const d
     if (m < 2 || w < 0 || w >= m || metric < FSM_METRIC_DTW || metric > FSM_METRIC_MSM)
         return fail(FSM_ERR_ARG, "invalid m, w or metric");
     if (n == 0) return FSM_OK;
+    if (!std::isfinite(arg)) return fail(FSM_ERR_ARG, "Invalid finite metric parameter");
+    for (int i = 0; i < m; i++)
+        if (!std::isfinite(q[i])) return fail(FSM_ERR_ARG, "query values must be finite");
+    for (int64_t i = 0; i < (int64_t)m * n; i++)
+        if (!std::isfinite(cands[i])) return fail(FSM_ERR_ARG, "candidate values must be finite");
     if (fsm_device_count() < 1) return fail(FSM_ERR_NODEV, "no HIP device");
     HIP_TRY(hipSetDevice(0));
     DevBuf dq, dxg, dc, dout, dscr;

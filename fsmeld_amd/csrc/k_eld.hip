@@ -25,18 +25,13 @@
 
 namespace fsm {
 
-__device__ inline double dmin3(double a, double b, double c) {  // std::min({a,b,c})
-    double r = a;
-    if (b < r) r = b;
-    if (c < r) r = c;
-    return r;
-}
-__device__ inline double dmax3(double a, double b, double c) {  // std::max({a,b,c})
-    double r = a;
-    if (r < b) r = b;
-    if (r < c) r = c;
-    return r;
-}
+// DP values are sums of non-negative costs (or LCSS counts): never NaN and
+// never -0, so fmin / fmax (v_min_f64 / v_max_f64) return exactly what
+// std::min({...}) / std::max({...}) return.  This file is compiled with
+// -fno-honor-nans (all inputs are finite: valid windows, a validated query),
+// which lets them lower to single v_min_f64 / v_max_f64 instructions.
+__device__ inline double dmin3(double a, double b, double c) { return fmin(fmin(a, b), c); }
+__device__ inline double dmax3(double a, double b, double c) { return fmax(fmax(a, b), c); }
 __device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
 __device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
 
@@ -45,13 +40,9 @@ __device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
 // lo - np, 0): the two differences are the same rounded values the reference
 // takes the min of, and c + 0 = c.
 __device__ inline double msm_cost(double np, double x, double y, double c) {
-    const double hi = x > y ? x : y;
-    const double lo = x > y ? y : x;
-    double e = np - hi;
-    const double f = lo - np;
-    if (f > e) e = f;
-    if (e < 0.0) e = 0.0;
-    return c + e;
+    const double hi = fmax(x, y);
+    const double lo = fmin(x, y);
+    return c + fmax(fmax(np - hi, lo - np), 0.0);
 }
 
 // Candidate sample source: raw target samples normalised on the fly, or rows
@@ -73,40 +64,53 @@ __device__ inline double qat(const double* __restrict__ q, int i, int m) {
 
 // One cell.  DTW / LCSS / MSM index samples by (i, j); ERP / EDR use the
 // (m+1)^2 grid with gap row / column and samples x[i-1], y[j-1] (passed as
-// xv / yv, with xg = (x[i-1]-g)^2, yg = (y[j-1]-g)^2).
-template <int METRIC>
+// xp / yp, with xg = (x[i-1]-g)^2, yg = (y[j-1]-g)^2).  EDGE = false: the
+// caller guarantees i > 0 and j > 0.
+template <int METRIC, bool EDGE>
 __device__ inline double cell(int i, int j, double up, double left, double diag, double xi,
                               double xp, double yj, double yp, double xg, double yg,
                               double arg) {
     const double S = kEldSentinel;
     if (METRIC == DTW) {
         const double d = (xi - yj) * (xi - yj);
-        if (i == 0 && j == 0) return d;
-        const double u = (i > 0) ? up : S;
-        const double l = (j > 0) ? left : S;
-        const double g = (i > 0 && j > 0) ? diag : S;
-        return dmin3(u, l, g) + d;
+        if (EDGE) {
+            if (i == 0 && j == 0) return d;
+            const double u = (i > 0) ? up : S;
+            const double l = (j > 0) ? left : S;
+            const double g = (i > 0 && j > 0) ? diag : S;
+            return dmin3(u, l, g) + d;
+        }
+        return dmin3(up, left, diag) + d;
     } else if (METRIC == ERP) {
-        if (i == 0 && j == 0) return 0.0;
-        if (i == 0) return left + yg;
-        if (j == 0) return up + xg;
+        if (EDGE) {
+            if (i == 0 && j == 0) return 0.0;
+            if (i == 0) return left + yg;
+            if (j == 0) return up + xg;
+        }
         const double match = diag + (xp - yp) * (xp - yp);
         return dmin3(match, left + yg, up + xg);
     } else if (METRIC == EDR) {
-        if (i == 0 && j == 0) return 0.0;
-        if (i == 0) return (double)j;
-        if (j == 0) return (double)i;
+        if (EDGE) {
+            if (i == 0 && j == 0) return 0.0;
+            if (i == 0) return (double)j;
+            if (j == 0) return (double)i;
+        }
         const double sub = (fabs(xp - yp) <= arg) ? 0.0 : 1.0;
         return dmin3(up + 1.0, left + 1.0, diag + sub);
     } else if (METRIC == LCSS) {
-        const double d1 = (i > 0 && j > 0) ? diag : 0.0;
-        const double u = (i > 0) ? up : 0.0;
-        const double l = (j > 0) ? left : 0.0;
+        double d1 = diag, u = up, l = left;
+        if (EDGE) {
+            d1 = (i > 0 && j > 0) ? diag : 0.0;
+            u = (i > 0) ? up : 0.0;
+            l = (j > 0) ? left : 0.0;
+        }
         return (fabs(xi - yj) <= arg) ? dmax3(d1 + 1, u, l) : dmax3(d1, u, l);
     } else {  // MSM
-        if (i == 0 && j == 0) return fabs(xi - yj);
-        if (i == 0) return left + msm_cost(yj, xi, yp, arg);
-        if (j == 0) return up + msm_cost(xi, xp, yj, arg);
+        if (EDGE) {
+            if (i == 0 && j == 0) return fabs(xi - yj);
+            if (i == 0) return left + msm_cost(yj, xi, yp, arg);
+            if (j == 0) return up + msm_cost(xi, xp, yj, arg);
+        }
         const double c1 = diag + fabs(xi - yj);
         const double c2 = up + msm_cost(xi, xp, yj, arg);
         const double c3 = left + msm_cost(yj, xi, yp, arg);
@@ -114,22 +118,104 @@ __device__ inline double cell(int i, int j, double up, double left, double diag,
     }
 }
 
-// Anti-diagonal DP for one lane, compile-time band half-width W >= w.
-// Iteration b handles anti-diagonals dA = 2b + W (even slots 2r, i = b + r,
-// j = b + W - r) and dB = dA + 1 (odd slots 2r+1, i = b + 1 + r, j = b + W - r).
-// Rings: X[k] = x[b - 1 + k], Y[k] = y[b - 1 + k], k in [0, W+1]; a cell reads
-// x[i] = X[i - b + 1] and x[i-1] = X[i - b] (likewise y), ERP/EDR the latter.
-template <int METRIC, int W>
-__device__ double eld_ad(const double* __restrict__ q, const double* __restrict__ qg,
-                         const CandSrc& c, int m, int w, double arg, double thr, bool active,
-                         bool* dead_out) {
-    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
-    const int M = GRID1 ? m : m - 1;  // last grid index; ERP/EDR cells read x[i-1], y[j-1]
+// Query staging in LDS: xs[kXPad + i] = x[i] for i in [0, m), 0 elsewhere in
+// [-kXPad, m + kXPad); for ERP xg holds (x[i] - g)^2 likewise.
+constexpr int kXPad = 40;
+
+// Anti-diagonal DP for one lane, compile-time band half-width W >= w
+// (EXACT: w == W, so every slot is in band).  Iteration b handles
+// anti-diagonals dA = 2b + W (even slots 2r, cells (b + r, b + W - r)) and
+// dB = dA + 1 (odd slots 2r+1, cells (b + 1 + r, b + W - r)).  The candidate
+// ring holds Y[k] = y[b - 1 + k], k in [0, W+1]; a cell reads y[j] =
+// Y[j - b + 1] and y[j-1] = Y[j - b]; query samples come from LDS at
+// wave-uniform addresses.
+// One iteration (two anti-diagonals) of the band update; EDGE selects the
+// boundary-checked form.
+template <int METRIC, int W, bool EXACT, bool EDGE>
+__device__ __forceinline__ void ad_step(int b, int M, int w, double arg, double* A, double* B,
+                                        const double* X, const double* Y) {
     const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;
-    double A[W + 1], B[W + 1];  // B[W] is a padding slot (always S)
-    double X[W + 2], Y[W + 2], XG[W + 2], YG[W + 2];
+#pragma unroll
+    for (int r = 0; r <= W; r++) {
+        const int i = b + r, j = b + W - r;
+        const int delta = 2 * r - W;
+        const bool inb = EXACT || (delta <= w && delta >= -w);
+        const bool inr = !EDGE || (i >= 0 && j >= 0 && i <= M && j <= M);
+        if (inb && inr) {
+            const double up = (r >= 1) ? B[r - 1] : S;
+            const double left = (r <= W - 1) ? B[r] : S;
+            A[r] = cell<METRIC, EDGE>(i, j, up, left, A[r], X[r + 1], X[r], Y[W - r + 1],
+                                      Y[W - r],
+                                      METRIC == ERP ? (X[r] - arg) * (X[r] - arg) : 0.0,
+                                      METRIC == ERP ? (Y[W - r] - arg) * (Y[W - r] - arg) : 0.0,
+                                      arg);
+        } else if (EDGE) {
+            A[r] = S;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < W; r++) {
+        const int i = b + 1 + r, j = b + W - r;
+        const int delta = 2 * r + 1 - W;
+        const bool inb = EXACT || (delta <= w && delta >= -w);
+        const bool inr = !EDGE || (i >= 0 && j >= 0 && i <= M && j <= M);
+        if (inb && inr) {
+            B[r] = cell<METRIC, EDGE>(i, j, A[r], A[r + 1], B[r], X[r + 2], X[r + 1],
+                                      Y[W - r + 1], Y[W - r],
+                                      METRIC == ERP ? (X[r + 1] - arg) * (X[r + 1] - arg) : 0.0,
+                                      METRIC == ERP ? (Y[W - r] - arg) * (Y[W - r] - arg) : 0.0,
+                                      arg);
+        } else if (EDGE) {
+            B[r] = S;
+        }
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void ring_shift(double* X, double* Y, double xn, double yn) {
+#pragma unroll
+    for (int k = 0; k < W + 1; k++) {
+        Y[k] = Y[k + 1];
+        X[k] = X[k + 1];
+    }
+    Y[W + 1] = yn;
+    X[W + 1] = xn;
+}
+
+template <int METRIC, int W>
+__device__ __forceinline__ bool ad_alive(const double* A, const double* B, int b, int M, int m,
+                                         double thr) {
+    if (METRIC == LCSS) {
+        double mx = A[0];
+#pragma unroll
+        for (int k = 1; k <= W; k++) mx = fmax(mx, A[k]);
+#pragma unroll
+        for (int k = 0; k < W; k++) mx = fmax(mx, B[k]);
+        // matches still possible after a cell of anti-diagonal >= 2b+W
+        const int rem = (2 * M - (2 * b + W)) / 2 + 1;
+        return !(1.0 - (mx + (double)rem) / (double)m > thr);
+    }
+    double mn = A[0];
+#pragma unroll
+    for (int k = 1; k <= W; k++) mn = fmin(mn, A[k]);
+#pragma unroll
+    for (int k = 0; k < W; k++) mn = fmin(mn, B[k]);
+    return !(mn > thr);
+}
+
+template <int METRIC, int W, bool EXACT>
+__device__ double eld_ad(const double* xs, const double* xgs, const CandSrc& c, int m, int w,
+                         double arg, double thr, bool active, bool* dead_out) {
+    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
+    const int M = GRID1 ? m : m - 1;
+    const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;
+    double A[W + 1], B[W + 1];
+    double X[W + 2], Y[W + 2];
     const int b0 = -((W + 1) / 2) - 1;
     const int bend = (2 * M - W) >= 0 ? (2 * M - W) / 2 : -((W - 2 * M + 1) / 2);
+    // interior iterations [1, M - W]: every cell in range with i, j >= 1
+    const int bi0 = 1;
+    const int bi1 = M - W;
 #pragma unroll
     for (int k = 0; k <= W; k++) {
         A[k] = S;
@@ -137,79 +223,35 @@ __device__ double eld_ad(const double* __restrict__ q, const double* __restrict_
     }
 #pragma unroll
     for (int k = 0; k < W + 2; k++) {
-        const int idx = b0 - 1 + k;
-        X[k] = qat(q, idx, m);
-        Y[k] = c.at(idx);
-        if (METRIC == ERP) {
-            XG[k] = qat(qg, idx, m);
-            YG[k] = (Y[k] - arg) * (Y[k] - arg);
-        }
+        Y[k] = c.at(b0 - 1 + k);
+        X[k] = xs[kXPad + b0 - 1 + k];
     }
     bool alive = active;
-    double result = S;
-    const bool check = thr < 1e300 && METRIC != LCSS;
+    const bool check = thr < 1e300;
+    int b = b0;
+    // head
+    for (; b <= bend && b < bi0; b++) {
+        ad_step<METRIC, W, EXACT, true>(b, M, w, arg, A, B, X, Y);
+        ring_shift<W>(X, Y, xs[kXPad + b + W + 1], c.at(b + W + 1));
+    }
+    // interior, early abandon every 4 iterations
     int it = 0;
-    for (int b = b0; b <= bend; b++, it++) {
-        // phase A: slots 2r, cells (b + r, b + W - r)
-#pragma unroll
-        for (int r = 0; r <= W; r++) {
-            const int i = b + r, j = b + W - r;
-            const int delta = 2 * r - W;
-            double v = S;
-            if (i >= 0 && j >= 0 && i <= M && j <= M && delta <= w && delta >= -w) {
-                const double up = (r >= 1) ? B[r - 1] : S;
-                const double left = (r <= W - 1) ? B[r] : S;
-                v = cell<METRIC>(i, j, up, left, A[r], X[r + 1], X[r], Y[W - r + 1], Y[W - r],
-                                 METRIC == ERP ? XG[r] : 0.0, METRIC == ERP ? YG[W - r] : 0.0,
-                                 arg);
-            }
-            A[r] = v;
-        }
-        if (!(W & 1) && b == bend) result = A[W / 2];
-        // phase B: slots 2r+1, cells (b + 1 + r, b + W - r)
-#pragma unroll
-        for (int r = 0; r < W; r++) {
-            const int i = b + 1 + r, j = b + W - r;
-            const int delta = 2 * r + 1 - W;
-            double v = S;
-            if (i >= 0 && j >= 0 && i <= M && j <= M && delta <= w && delta >= -w) {
-                v = cell<METRIC>(i, j, A[r], A[r + 1], B[r], X[r + 2], X[r + 1], Y[W - r + 1],
-                                 Y[W - r], METRIC == ERP ? XG[r + 1] : 0.0,
-                                 METRIC == ERP ? YG[W - r] : 0.0, arg);
-            }
-            B[r] = v;
-        }
-        if ((W & 1) && b == bend) result = B[(W - 1) / 2];
-        // shift the rings: X[k] <- x[b + k]
-        {
-            const int idx = b + W + 1;
-            const double xn = qat(q, idx, m);
-            const double yn = c.at(idx);
-#pragma unroll
-            for (int k = 0; k < W + 1; k++) {
-                X[k] = X[k + 1];
-                Y[k] = Y[k + 1];
-                if (METRIC == ERP) {
-                    XG[k] = XG[k + 1];
-                    YG[k] = YG[k + 1];
-                }
-            }
-            X[W + 1] = xn;
-            Y[W + 1] = yn;
-            if (METRIC == ERP) {
-                XG[W + 1] = qat(qg, idx, m);
-                YG[W + 1] = (yn - arg) * (yn - arg);
-            }
-        }
-        if (check && (it & 7) == 7) {
-            double mn = A[0];
-#pragma unroll
-            for (int k = 1; k <= W; k++) mn = fmin(mn, A[k]);
-#pragma unroll
-            for (int k = 0; k < W; k++) mn = fmin(mn, B[k]);
-            if (mn > thr) alive = false;
+    for (; b <= bi1; b++, it++) {
+        ad_step<METRIC, W, EXACT, false>(b, M, w, arg, A, B, X, Y);
+        ring_shift<W>(X, Y, xs[kXPad + b + W + 1], c.at(b + W + 1));
+        if (check && (it & 3) == 3) {
+            alive = alive && ad_alive<METRIC, W>(A, B, b, M, m, thr);
             if (!__any(alive)) break;
         }
+    }
+    double result = S;
+    if (__any(alive) || !check) {
+        // tail (the final cell (M, M) sits in slot W of iteration bend)
+        for (b = (b > bi1 ? b : bi1 + 1); b <= bend; b++) {
+            ad_step<METRIC, W, EXACT, true>(b, M, w, arg, A, B, X, Y);
+            ring_shift<W>(X, Y, xs[kXPad + b + W + 1], c.at(b + W + 1));
+        }
+        result = (W & 1) ? B[(W - 1) / 2] : A[W / 2];
     }
     *dead_out = !alive;
     if (METRIC == DTW || METRIC == ERP) {
@@ -254,10 +296,10 @@ __device__ double eld_rows(const double* __restrict__ q, const double* __restric
                 const double up = (k < w2) ? b[k + 1] : S;
                 const double left = (k > 0) ? b[k - 1] : S;
                 if (GRID1)
-                    v = cell<METRIC>(i, j, up, left, b[k], 0.0, xi, 0.0, yj, xg,
+                    v = cell<METRIC, true>(i, j, up, left, b[k], 0.0, xi, 0.0, yj, xg,
                                      (yj - arg) * (yj - arg), arg);
                 else
-                    v = cell<METRIC>(i, j, up, left, b[k], xi, xp, yj, yp, 0.0, 0.0, arg);
+                    v = cell<METRIC, true>(i, j, up, left, b[k], xi, xp, yj, yp, 0.0, 0.0, arg);
             }
             b[k] = v;
         }
@@ -276,8 +318,8 @@ __device__ double eld_rows(const double* __restrict__ q, const double* __restric
 
 // ----------------------------------------------------------------- kernel ---
 // in[] lists window indices (target mode) or, with rows != null, row numbers
-// 0..n_direct-1 of normalised candidates.
-template <int METRIC, int W>
+// 0..n_direct-1 of normalised candidates.  W == 0: generic row-band path.
+template <int METRIC, int W, bool EXACT>
 __global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const int64_t* __restrict__ in,
                                              const unsigned long long* __restrict__ in_count,
                                              int64_t n_direct, const double* __restrict__ rows,
@@ -285,6 +327,7 @@ __global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const int64_t* __rest
                                              double* __restrict__ out_dist,
                                              unsigned long long* out_count,
                                              double* __restrict__ scratch) {
+    extern __shared__ double xlds[];
     const int64_t n = in_count ? (int64_t)*in_count : n_direct;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,6 +338,16 @@ __global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const int64_t* __rest
     if (!keep_all) {
         if (METRIC == DTW || METRIC == ERP) thr = cut * cut * (1.0 + 1e-12) + 1e-300;
         else thr = cut * (1.0 + 1e-12) + 1e-300;
+    }
+    const int span = m + 2 * kXPad;
+    if (W > 0) {
+        for (int i = threadIdx.x; i < span; i += blockDim.x) {
+            const int qi = i - kXPad;
+            const bool in_q = qi >= 0 && qi < m;
+            xlds[i] = in_q ? a.q.qn[qi] : 0.0;
+            if (METRIC == ERP) xlds[span + i] = in_q ? a.q.XG[qi] : 0.0;
+        }
+        __syncthreads();
     }
     int64_t ci = tid;
     for (int64_t itx = 0; itx < iters; itx++, ci += stride) {
@@ -319,7 +372,7 @@ __global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const int64_t* __rest
         bool dead = false;
         double d;
         if constexpr (W > 0) {
-            d = eld_ad<METRIC, W>(a.q.qn, a.q.XG, c, m, w, a.p.arg, thr, have, &dead);
+            d = eld_ad<METRIC, W, EXACT>(xlds, xlds + span, c, m, w, a.p.arg, thr, have, &dead);
         } else {
             MemArr b{scratch + tid, stride * 2};
             MemArr y{scratch + tid + stride, stride * 2};
@@ -359,18 +412,20 @@ static hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64
     const int w = a.p.w;
     const dim3 blk(256), grid(2048);
     const int ka = keep_all ? 1 : 0;
-    if (w <= 6)
-        hipLaunchKernelGGL((k_eld<METRIC, 6>), grid, blk, 0, st, a, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch);
-    else if (w <= 8)
-        hipLaunchKernelGGL((k_eld<METRIC, 8>), grid, blk, 0, st, a, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch);
-    else if (w <= 16)
-        hipLaunchKernelGGL((k_eld<METRIC, 16>), grid, blk, 0, st, a, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch);
-    else if (w <= 25)
-        hipLaunchKernelGGL((k_eld<METRIC, 25>), grid, blk, 0, st, a, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch);
+    const size_t lds = sizeof(double) * (size_t)(a.p.m + 2 * kXPad) * (METRIC == ERP ? 2 : 1);
+#define FSM_ELD(WW, EX) \
+    hipLaunchKernelGGL((k_eld<METRIC, WW, EX>), grid, blk, lds, st, a, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch)
+    if (w == 6) FSM_ELD(6, true);
+    else if (w == 25) FSM_ELD(25, true);
+    else if (w <= 8) FSM_ELD(8, false);
+    else if (w <= 16) FSM_ELD(16, false);
+    else if (w <= 25) FSM_ELD(25, false);
+    else if (w <= 32 && METRIC != ERP) FSM_ELD(32, false);
     else {
         if (!scratch) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_eld<METRIC, 0>), dim3((unsigned)(scratch_threads / 256)), blk, 0, st, a, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch);
+        hipLaunchKernelGGL((k_eld<METRIC, 0, false>), dim3((unsigned)(scratch_threads / 256)), blk, 0, st, a, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch);
     }
+#undef FSM_ELD
     return hipGetLastError();
 }
 

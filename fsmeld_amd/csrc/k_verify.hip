@@ -147,6 +147,24 @@ struct CandEnvelope {
     }
 };
 
+// Normalised samples (src/fsmeld.cpp:822-826).  A position whose normalised
+// value lies strictly inside the other series' envelope adds nothing to any
+// GLB sum, so the correctly rounded quotient (t - mu) / sd is computed only
+// where that cannot be decided from a = fl(fl(t - mu) * fl(1 / sd)).  With
+// u = 2^-53, |a - fl((t - mu) / sd)| <= 3.02 u |a|; decisions use 8 u |a|.
+struct Norm {
+    double mu, sd, rinv;
+    __device__ double approx(double x) const { return (x - mu) * rinv; }
+    __device__ double exact(double x) const { return (x - mu) / sd; }
+};
+__device__ inline double nmargin(double a) { return fabs(a) * 8.881784197001252e-16 + 1e-300; }
+
+// T strictly inside [lo, hi] for every rounding of the approximate value a
+__device__ inline bool surely_inside(double a, double lo, double hi) {
+    const double d = nmargin(a);
+    return a - d > lo && a + d < hi;
+}
+
 template <int METRIC>
 __device__ double glb_one(const double* __restrict__ t, double mu, double sd,
                           const double* __restrict__ Q, const double* __restrict__ LQ,
@@ -154,40 +172,57 @@ __device__ double glb_one(const double* __restrict__ t, double mu, double sd,
                           int m, int w, double arg, double bsf) {
     if (m < 2) return 0.0;
     const int n = m;
-    auto T = [&](int i) { return (t[i] - mu) / sd; };  // src/fsmeld.cpp:822-826
+    const Norm N{mu, sd, 1.0 / sd};
     double boundary = 0.0, t2q = 0.0, lcss_t = 0.0;
-    const double Tl = T(n - 1);
+    const double Tl = N.exact(t[n - 1]);
+    // The reference tests early abandon after every position; positions that
+    // add nothing leave the test unchanged, so it is evaluated once before the
+    // loop (its first test) and after every contributing position.
     if (METRIC == DTW) {
-        boundary = sqd(T(0), Q[0]) + sqd(Tl, Q[n - 1]);
+        boundary = sqd(N.exact(t[0]), Q[0]) + sqd(Tl, Q[n - 1]);
+        if (n > 2 && boundary + t2q > bsf * bsf) return kGlbAbandon;
         for (int i = 1; i < n - 1; ++i) {
-            const double v = T(i);
+            if (surely_inside(N.approx(t[i]), LQ[i], UQ[i])) continue;
+            const double v = N.exact(t[i]);
             if (v < LQ[i]) t2q += sqd(v, LQ[i]);
             else if (v > UQ[i]) t2q += sqd(v, UQ[i]);
             if (boundary + t2q > bsf * bsf) return kGlbAbandon;
         }
     } else if (METRIC == ERP) {
         boundary = smin3(sqd(Q[n - 1], Tl), sqd(Q[n - 1], arg), sqd(Tl, arg));
+        if (n > 2 && boundary + t2q > bsf * bsf) return kGlbAbandon;
         for (int i = 1; i < n - 1; ++i) {
-            const double v = T(i);
+            if (surely_inside(N.approx(t[i]), LQ[i], UQ[i])) continue;
+            const double v = N.exact(t[i]);
             if (v > UQ[i]) t2q += mn(sqd(v, UQ[i]), sqd(v, arg));
             else if (v < LQ[i]) t2q += mn(sqd(v, LQ[i]), sqd(v, arg));
             if (boundary + t2q > bsf * bsf) return kGlbAbandon;
         }
     } else if (METRIC == EDR) {
         boundary = mn(edr_subcost(Q[n - 1], Tl, arg), 1.0);
+        if (n > 2 && boundary + t2q > bsf) return kGlbAbandon;
         for (int i = 1; i < n - 1; ++i) {
-            const double v = T(i);
+            if (surely_inside(N.approx(t[i]), LQ[i], UQ[i])) continue;
+            const double v = N.exact(t[i]);
             if (v > UQ[i]) t2q += edr_subcost(v, UQ[i], arg);
             else if (v < LQ[i]) t2q += edr_subcost(v, LQ[i], arg);
             if (boundary + t2q > bsf) return kGlbAbandon;
         }
     } else if (METRIC == LCSS) {
         const double min_len = (double)n;
-        for (int i = 0; i < n; ++i)
-            if (env_match(T(i), LQ[i], UQ[i], arg)) lcss_t += 1.0;
+        for (int i = 0; i < n; ++i) {
+            const double a = N.approx(t[i]);
+            if (surely_inside(a, LQ[i], UQ[i])) {
+                lcss_t += 1.0;
+                continue;
+            }
+            const double far = nmargin(a) + 8.881784197001252e-16 * (fabs(LQ[i]) + fabs(UQ[i]) + arg);
+            if (a + far < LQ[i] - arg || a - far > UQ[i] + arg) continue;  // surely no match
+            if (env_match(N.exact(t[i]), LQ[i], UQ[i], arg)) lcss_t += 1.0;
+        }
         if (smax(0.0, 1.0 - lcss_t / min_len) > bsf) return kGlbAbandon;
     } else {  // MSM
-        const double T2 = T(n - 2), T0 = T(0);
+        const double T2 = N.exact(t[n - 2]), T0 = N.exact(t[0]);
         const int b = n - 1;
         if ((T2 >= Tl && Tl >= Q[b]) || (T2 <= Tl && Tl <= Q[b]) ||
             (Q[b - 1] <= Q[b] && Q[b] <= Tl) || (Q[b - 1] >= Q[b] && Q[b] >= Tl)) {
@@ -196,71 +231,63 @@ __device__ double glb_one(const double* __restrict__ t, double mu, double sd,
             boundary = fabs(Q[0] - T0) + smin3(fabs(Q[b] - Tl), arg + fabs(Tl - T2),
                                                arg + fabs(Q[b] - Q[b - 1]));
         }
+        if (n > 2 && boundary + t2q > bsf) return kGlbAbandon;
         for (int i = 1; i < n - 1; ++i) {
-            const double v = T(i);
+            if (surely_inside(N.approx(t[i]), LQ[i], UQ[i])) continue;
+            const double v = N.exact(t[i]);
             if (v > UQ[i]) t2q += mn(fabs(v - UQ[i]), arg);
             else if (v < LQ[i]) t2q += mn(fabs(v - LQ[i]), arg);
             if (boundary + t2q > bsf) return kGlbAbandon;
         }
     }
 
+    // candidate envelope (raw), normalised only where a decision needs it;
+    // the pass's first abandon test equals pass 1's and has already passed
     CandEnvelope E;
     E.init(t, Lr, Ur, m, w);
-    auto env = [&](int i, double* lo, double* hi) {
-        double a, b;
-        E.at(i, &a, &b);
-        *lo = (a - mu) / sd;
-        *hi = (b - mu) / sd;
-    };
-    if (METRIC == DTW) {
-        double q2t = 0.0;
-        for (int i = 1; i < n - 1; ++i) {
-            double LT, UT;
-            env(i, &LT, &UT);
-            if (Q[i] < LT) q2t += sqd(Q[i], LT);
-            else if (Q[i] > UT) q2t += sqd(Q[i], UT);
-            if (boundary + q2t > bsf * bsf) return kGlbAbandon;
+    double q2t = 0.0, qr = 0.0;
+    const int i0 = (METRIC == LCSS) ? 0 : 1;
+    const int i1 = (METRIC == LCSS) ? n : n - 1;
+    for (int i = i0; i < i1; ++i) {
+        double lo, hi;
+        E.at(i, &lo, &hi);
+        const double aL = N.approx(lo), aU = N.approx(hi);
+        const double q = Q[i];
+        const bool inside = q - nmargin(aL) > aL && q + nmargin(aU) < aU;
+        if (METRIC == LCSS) {
+            if (inside) {
+                qr += 1.0;
+                continue;
+            }
+            const double far = nmargin(aL) + nmargin(aU) +
+                               8.881784197001252e-16 * (fabs(q) + arg);
+            if (q + far < aL - arg || q - far > aU + arg) continue;
+            if (env_match(q, N.exact(lo), N.exact(hi), arg)) qr += 1.0;
+            continue;
         }
-        return sqrt(smax(0.0, boundary + mx(t2q, q2t)));
-    } else if (METRIC == ERP) {
-        double q2t = 0.0;
-        for (int i = 1; i < n - 1; ++i) {
-            double LT, UT;
-            env(i, &LT, &UT);
-            if (Q[i] > UT) q2t += mn(sqd(Q[i], UT), sqd(Q[i], arg));
-            else if (Q[i] < LT) q2t += mn(sqd(Q[i], LT), sqd(Q[i], arg));
+        if (inside) continue;
+        const double LT = N.exact(lo), UT = N.exact(hi);
+        if (METRIC == DTW) {
+            if (q < LT) q2t += sqd(q, LT);
+            else if (q > UT) q2t += sqd(q, UT);
             if (boundary + q2t > bsf * bsf) return kGlbAbandon;
-        }
-        return sqrt(smax(0.0, boundary + mx(t2q, q2t)));
-    } else if (METRIC == EDR) {
-        double q2t = 0.0;
-        for (int i = 1; i < n - 1; ++i) {
-            double LT, UT;
-            env(i, &LT, &UT);
-            if (Q[i] > UT) q2t += edr_subcost(Q[i], UT, arg);
-            else if (Q[i] < LT) q2t += edr_subcost(Q[i], LT, arg);
+        } else if (METRIC == ERP) {
+            if (q > UT) q2t += mn(sqd(q, UT), sqd(q, arg));
+            else if (q < LT) q2t += mn(sqd(q, LT), sqd(q, arg));
+            if (boundary + q2t > bsf * bsf) return kGlbAbandon;
+        } else if (METRIC == EDR) {
+            if (q > UT) q2t += edr_subcost(q, UT, arg);
+            else if (q < LT) q2t += edr_subcost(q, LT, arg);
+            if (boundary + q2t > bsf) return kGlbAbandon;
+        } else {  // MSM
+            if (q > UT) q2t += mn(fabs(q - UT), arg);
+            else if (q < LT) q2t += mn(fabs(q - LT), arg);
             if (boundary + q2t > bsf) return kGlbAbandon;
         }
-        return boundary + mx(t2q, q2t);
-    } else if (METRIC == LCSS) {
-        double qr = 0.0;
-        for (int i = 0; i < n; ++i) {
-            double LT, UT;
-            env(i, &LT, &UT);
-            if (env_match(Q[i], LT, UT, arg)) qr += 1.0;
-        }
-        return smax(0.0, 1.0 - mn(lcss_t, qr) / (double)n);
-    } else {
-        double q2t = 0.0;
-        for (int i = 1; i < n - 1; ++i) {
-            double LT, UT;
-            env(i, &LT, &UT);
-            if (Q[i] > UT) q2t += mn(fabs(Q[i] - UT), arg);
-            else if (Q[i] < LT) q2t += mn(fabs(Q[i] - LT), arg);
-            if (boundary + q2t > bsf) return kGlbAbandon;
-        }
-        return boundary + mx(t2q, q2t);
     }
+    if (METRIC == DTW || METRIC == ERP) return sqrt(smax(0.0, boundary + mx(t2q, q2t)));
+    if (METRIC == LCSS) return smax(0.0, 1.0 - mn(lcss_t, qr) / (double)n);
+    return boundary + mx(t2q, q2t);
 }
 
 template <int METRIC>

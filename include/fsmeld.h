@@ -128,7 +128,8 @@ int fsm_stage_eld(fsm_target* t, const fsm_query_desc* desc, const double* query
 
 /* Exact elastic distance of n already-normalised rows cands[n][m] against q[m]
  * (dtw/erp/edr/lcss/msm, include/funcs.h:76-81, argument order of
- * src/fsmeld.cpp:170-183).  No target needed. */
+ * src/fsmeld.cpp:170-183).  No target needed; all values must be finite
+ * (FSM_ERR_ARG otherwise). */
 int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const double* cands,
                   int64_t n, double* out);
 
