@@ -95,6 +95,8 @@ _sig("fsm_stage_window_stats", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int64,
                                               ctypes.c_int64, _D, _D, _U8])
 _sig("fsm_stage_lbip", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _D, _D, _D])
 _sig("fsm_stage_glb", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _I64, ctypes.c_int64, _D])
+_sig("fsm_stage_glb_filter", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _I64, ctypes.c_int64,
+                                            _U8])
 _sig("fsm_stage_eld", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _I64, ctypes.c_int64, _D])
 _sig("fsm_eld_pairs", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      _D, _D, ctypes.c_int64, _D])
@@ -241,6 +243,16 @@ class Target:
         _check(_lib.fsm_stage_glb(self._h, ctypes.byref(d), _ptr(q), _ptr(win, _I64),
                                   int(win.size), _ptr(out)))
         return out
+
+    def glb_keep(self, query, m, metric, w, arg, windows, bsf):
+        """GLB keep decision (``glb <= bsf``) per window from the search filter kernel."""
+        q = _f64(query)
+        win = np.ascontiguousarray(np.asarray(windows, dtype=np.int64))
+        d = _desc("search", metric, m, w, arg, cutoff_value=bsf)
+        out = np.empty(win.size, dtype=np.uint8)
+        _check(_lib.fsm_stage_glb_filter(self._h, ctypes.byref(d), _ptr(q), _ptr(win, _I64),
+                                         int(win.size), _ptr(out, _U8)))
+        return out.astype(bool)
 
     def eld(self, query, m, metric, w, arg, windows):
         q = _f64(query)

@@ -242,6 +242,8 @@ int upload_query(fsm_target* t, const HostQuery& H, int m, QueryTables* Q) {
     Q->TU = d + 7 * stride;
     Q->TL = d + 8 * stride;
     Q->XG = d + 9 * stride;
+    Q->qabs = 0.0;
+    for (double v : H.qn) Q->qabs = std::max(Q->qabs, std::fabs(v));
     // the staging vector must outlive the async copy
     HIP_TRY(hipStreamSynchronize(t->stream));
     return FSM_OK;
@@ -317,8 +319,8 @@ VerifyArgs verify_args(fsm_target* t, const Geometry& G, const QueryTables& Q,
     return v;
 }
 
-int ensure_eld_scratch(fsm_target* t, int w) {
-    if (w <= 32) return FSM_OK;
+int ensure_eld_scratch(fsm_target* t, int metric, int m, int w) {
+    if (!eld_uses_scratch(metric, m, w)) return FSM_OK;
     const int64_t thr = eld_scratch_threads();
     HIP_TRY(t->scratch.ensure(sizeof(double) * 2 * (size_t)(2 * w + 2) * (size_t)thr));
     return FSM_OK;
@@ -360,7 +362,7 @@ int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
         eld_cnt = C + 2;
     }
     HIP_TRY(hipEventRecord(t->ev[4], t->stream));
-    int rc = ensure_eld_scratch(t, P.w);
+    int rc = ensure_eld_scratch(t, P.metric, P.m, P.w);
     if (rc) return rc;
     HIP_TRY(launch_eld(t->stream, v, eld_in, eld_cnt, false, t->out_idx.as<int64_t>(),
                        t->out_dist.as<double>(), C + 3, t->scratch.as<double>(),
@@ -518,7 +520,7 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
                                        hipMemcpyHostToDevice, t->stream));
                 HIP_TRY(hipMemcpyAsync(Cd + 4, &nrep, sizeof(nrep), hipMemcpyHostToDevice,
                                        t->stream));
-                rc = ensure_eld_scratch(t, d->w);
+                rc = ensure_eld_scratch(t, d->metric, d->m, d->w);
                 if (rc) return bail(rc);
                 VerifyArgs v = verify_args(t, G, Q, P);
                 HIP_TRY(launch_eld(t->stream, v, t->list_b.as<int64_t>(), Cd + 4, true, nullptr,
@@ -662,8 +664,11 @@ int fsm_stage_lbip(fsm_target* t, const fsm_query_desc* d, const double* q, doub
     return FSM_OK;
 }
 
+enum StageKind { STAGE_GLB_VALUES, STAGE_ELD_VALUES, STAGE_GLB_KEEP };
+
 static int stage_verify(fsm_target* t, const fsm_query_desc* d, const double* q,
-                        const int64_t* win, int64_t n, double* out, bool glb) {
+                        const int64_t* win, int64_t n, void* out, StageKind kind) {
+    const bool glb = kind != STAGE_ELD_VALUES;
     if (!t || !out || (n > 0 && !win) || n < 0) return fail(FSM_ERR_ARG, "null argument");
     int rc = validate(d, t->N, q, d ? d->m : 0, false);
     if (rc) return rc;
@@ -680,35 +685,55 @@ static int stage_verify(fsm_target* t, const fsm_query_desc* d, const double* q,
     bool fresh;
     if ((rc = ensure_stats(t, G, &fresh))) return rc;
     if (glb && (rc = ensure_env(t, d->w))) return rc;
-    if ((rc = ensure_eld_scratch(t, d->w))) return rc;
+    if ((rc = ensure_eld_scratch(t, d->metric, d->m, d->w))) return rc;
     HIP_TRY(t->list_a.ensure(sizeof(int64_t) * (size_t)n));
+    HIP_TRY(t->list_b.ensure(sizeof(int64_t) * (size_t)n));
     HIP_TRY(t->out_dist.ensure(sizeof(double) * (size_t)n));
-    unsigned long long cnt = (unsigned long long)n;
+    unsigned long long cnt[2] = {(unsigned long long)n, 0ull};
     unsigned long long* Cd = t->counters.as<unsigned long long>();
     HIP_TRY(hipMemcpyAsync(t->list_a.p, win, sizeof(int64_t) * n, hipMemcpyHostToDevice,
                            t->stream));
-    HIP_TRY(hipMemcpyAsync(Cd + 6, &cnt, sizeof(cnt), hipMemcpyHostToDevice, t->stream));
+    HIP_TRY(hipMemcpyAsync(Cd + 6, cnt, sizeof(cnt), hipMemcpyHostToDevice, t->stream));
     VerifyArgs v = verify_args(t, G, Q, P);
-    if (glb)
+    if (kind == STAGE_GLB_VALUES)
         HIP_TRY(launch_glb(t->stream, v, t->list_a.as<int64_t>(), Cd + 6, nullptr, nullptr,
                            t->out_dist.as<double>()));
+    else if (kind == STAGE_GLB_KEEP)
+        HIP_TRY(launch_glb(t->stream, v, t->list_a.as<int64_t>(), Cd + 6, t->list_b.as<int64_t>(),
+                           Cd + 7, nullptr));
     else
         HIP_TRY(launch_eld(t->stream, v, t->list_a.as<int64_t>(), Cd + 6, true, nullptr,
                            t->out_dist.as<double>(), nullptr, t->scratch.as<double>(),
                            eld_scratch_threads()));
     HIP_TRY(hipStreamSynchronize(t->stream));
-    HIP_TRY(hipMemcpy(out, t->out_dist.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (kind != STAGE_GLB_KEEP) {
+        HIP_TRY(hipMemcpy(out, t->out_dist.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        return FSM_OK;
+    }
+    HIP_TRY(hipMemcpy(cnt, Cd + 6, sizeof(cnt), hipMemcpyDeviceToHost));
+    std::vector<int64_t> kept(cnt[1]);
+    if (cnt[1])
+        HIP_TRY(hipMemcpy(kept.data(), t->list_b.p, sizeof(int64_t) * cnt[1], hipMemcpyDeviceToHost));
+    std::sort(kept.begin(), kept.end());
+    uint8_t* flags = static_cast<uint8_t*>(out);
+    for (int64_t i = 0; i < n; i++)
+        flags[i] = std::binary_search(kept.begin(), kept.end(), win[i]) ? 1 : 0;
     return FSM_OK;
 }
 
 int fsm_stage_glb(fsm_target* t, const fsm_query_desc* d, const double* q, const int64_t* win,
                   int64_t n, double* out) {
-    return stage_verify(t, d, q, win, n, out, true);
+    return stage_verify(t, d, q, win, n, out, STAGE_GLB_VALUES);
+}
+
+int fsm_stage_glb_filter(fsm_target* t, const fsm_query_desc* d, const double* q,
+                         const int64_t* win, int64_t n, uint8_t* keep) {
+    return stage_verify(t, d, q, win, n, keep, STAGE_GLB_KEEP);
 }
 
 int fsm_stage_eld(fsm_target* t, const fsm_query_desc* d, const double* q, const int64_t* win,
                   int64_t n, double* out) {
-    return stage_verify(t, d, q, win, n, out, false);
+    return stage_verify(t, d, q, win, n, out, STAGE_ELD_VALUES);
 }
 
 int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const double* cands,
@@ -738,7 +763,8 @@ int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const d
         dc.ensure(sizeof(double) * (size_t)m * n) ||
         dout.ensure(sizeof(double) * n))
         return done(fail(FSM_ERR_OOM, "device allocation failed"));
-    if (w > 32 && dscr.ensure(sizeof(double) * 2 * (size_t)(2 * w + 2) * (size_t)thr))
+    if (eld_uses_scratch(metric, m, w) &&
+        dscr.ensure(sizeof(double) * 2 * (size_t)(2 * w + 2) * (size_t)thr))
         return done(fail(FSM_ERR_OOM, "device allocation failed"));
     std::vector<double> xg(m);
     for (int i = 0; i < m; i++) xg[i] = (q[i] - arg) * (q[i] - arg);

@@ -41,6 +41,7 @@ struct QueryTables {
     const double* TU;   // fl(qs - arg), fl(qs + arg) [m]
     const double* TL;
     const double* XG;   // (qn - arg)^2 [m], the ERP gap cost of each query sample
+    double qabs;        // max |qn|, which also bounds |LQ| and |UQ|
 };
 
 struct QueryParams {
@@ -160,5 +161,8 @@ hipError_t launch_eld(hipStream_t st, const VerifyArgs& a, const int64_t* in,
 hipError_t launch_eld_pairs(hipStream_t st, const VerifyArgs& a, const double* cands,
                             int64_t n, double* out, double* scratch, int64_t scratch_threads);
 int64_t eld_scratch_threads();
+// true when (metric, m, w) runs the row-band path, which needs the scratch
+// buffer of 2 (2w + 2) doubles per thread of eld_scratch_threads()
+bool eld_uses_scratch(int metric, int m, int w);
 
 }  // namespace fsm

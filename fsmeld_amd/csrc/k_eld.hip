@@ -15,12 +15,19 @@
 // normalised once each ((t - mu) / sigma, src/fsmeld.cpp:822-826) into a
 // register ring.
 //
-// Early abandon (search / exact / cutoff collection only): every optimal path
-// crosses one of any two consecutive anti-diagonals and its prefix cost is at
-// most the final cost (all step costs are >= 0), so once every band cell of
-// two consecutive anti-diagonals exceeds the cutoff (squared for DTW/ERP, with
-// a relative margin of 1e-12) the window cannot match and is dropped.  The
-// distances of surviving windows are unchanged.
+// Early abandon (search / exact / cutoff collection only): every path crosses
+// one of any two consecutive anti-diagonals, and after that cell it still has
+// to consume every candidate sample beyond the cell's column.  Consuming
+// sample y_k costs at least its distance to the query envelope over the band
+// (GLB's per-position term: d^2 for DTW, min(d^2, (y-g)^2) for ERP, [d > eps]
+// for EDR, min(d, c) for MSM).  So the final cost is at least
+//   min over the two anti-diagonals of the cell value + sum of those terms
+//   over the samples beyond the largest column of the two anti-diagonals,
+// and once that exceeds the cutoff (squared for DTW/ERP, relative margin
+// 1e-12 plus the rounding terms below) the window cannot match and is
+// dropped.  LCSS counts matches instead: at most one per remaining sample
+// that lies within eps of the envelope.  The distances of surviving windows
+// are unchanged.
 #include "engine_internal.h"
 
 namespace fsm {
@@ -121,6 +128,7 @@ __device__ inline double cell(int i, int j, double up, double left, double diag,
 // Query staging in LDS: xs[kXPad + i] = x[i] for i in [0, m), 0 elsewhere in
 // [-kXPad, m + kXPad); for ERP xg holds (x[i] - g)^2 likewise.
 constexpr int kXPad = 40;
+constexpr size_t kEldLdsCap = 64 * 1024;
 
 // Anti-diagonal DP for one lane, compile-time band half-width W >= w
 // (EXACT: w == W, so every slot is in band).  Iteration b handles
@@ -182,9 +190,28 @@ __device__ __forceinline__ void ring_shift(double* X, double* Y, double xn, doub
     X[W + 1] = xn;
 }
 
+// Lower bound on the cost of consuming candidate sample y whose band sees the
+// query envelope [lo, hi]; for LCSS, 1 if y can match at all.  Rounding: the
+// cells' costs are at least these values times (1 - 7u) (u = 2^-53), and the
+// LCSS / EDR comparisons are widened by a relative 1e-12, far above the
+// rounding of |x - y| against e.
+template <int METRIC>
+__device__ inline double consume_lb(double y, double lo, double hi, double arg) {
+    const double e = fmax(fmax(lo - y, y - hi), 0.0);
+    if (METRIC == DTW) return e * e;
+    if (METRIC == ERP) return fmin(e * e, (y - arg) * (y - arg));
+    if (METRIC == EDR) return (e > arg * (1.0 + 1e-12) + 1e-300) ? 1.0 : 0.0;
+    if (METRIC == MSM) return fmin(e, arg);
+    return (e <= arg * (1.0 + 1e-12) + 1e-300) ? 1.0 : 0.0;
+}
+
+// total: sum of consume_lb over all m samples, p: the same sum, in the same
+// order, over samples 0 .. K (K = the frontier's column bound for this check).
+// fl(total - p) is within (m + 1) u total of the exact suffix sum; the path
+// beyond the frontier adds at most 2m + 2 rounded terms.
 template <int METRIC, int W>
 __device__ __forceinline__ bool ad_alive(const double* A, const double* B, int b, int M, int m,
-                                         double thr) {
+                                         double thr, double total, double p) {
     if (METRIC == LCSS) {
         double mx = A[0];
 #pragma unroll
@@ -192,21 +219,27 @@ __device__ __forceinline__ bool ad_alive(const double* A, const double* B, int b
 #pragma unroll
         for (int k = 0; k < W; k++) mx = fmax(mx, B[k]);
         // matches still possible after a cell of anti-diagonal >= 2b+W
-        const int rem = (2 * M - (2 * b + W)) / 2 + 1;
-        return !(1.0 - (mx + (double)rem) / (double)m > thr);
+        const double rem = fmin((double)((2 * M - (2 * b + W)) / 2 + 1), total - p);
+        return !(1.0 - (mx + rem) / (double)m > thr);
     }
     double mn = A[0];
 #pragma unroll
     for (int k = 1; k <= W; k++) mn = fmin(mn, A[k]);
 #pragma unroll
     for (int k = 0; k < W; k++) mn = fmin(mn, B[k]);
-    return !(mn > thr);
+    const double cb = fmax((total - p) - (double)(m + 8) * 1.2e-16 * total, 0.0);
+    return !((mn + cb) * (1.0 - (double)(2 * m + 24) * 1.2e-16) > thr);
 }
 
 template <int METRIC, int W, bool EXACT>
 __device__ double eld_ad(const double* xs, const double* xgs, const CandSrc& c, int m, int w,
-                         double arg, double thr, bool active, bool* dead_out) {
+                         double arg, double thr, const double* lqs, const double* uqs,
+                         bool active, bool* dead_out) {
     constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
+    // after iteration b the ring holds Y[k] = y[b + k]; the frontier's columns
+    // are at most b + W (grid: samples up to b + W - 1 are consumed); LCSS
+    // bounds matches by samples beyond the smallest column, b
+    constexpr int KOFF = (METRIC == LCSS) ? 0 : (GRID1 ? W - 1 : W);
     const int M = GRID1 ? m : m - 1;
     const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;
     double A[W + 1], B[W + 1];
@@ -228,20 +261,48 @@ __device__ double eld_ad(const double* xs, const double* xgs, const CandSrc& c, 
     }
     bool alive = active;
     const bool check = thr < 1e300;
+    // p starts as the prefix of total through sample b0 + KOFF (the first
+    // iteration's bound) and then follows the same summation order
+    double total = 0.0, p = 0.0;
+    if (check) {
+        constexpr int U = 8;
+        int k = 0;
+        for (; k + U <= m; k += U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = c.at(k + u);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                total += consume_lb<METRIC>(v[u], lqs[kXPad + k + u], uqs[kXPad + k + u], arg);
+                p = (k + u == b0 + KOFF) ? total : p;
+            }
+        }
+        for (; k < m; k++) {
+            total += consume_lb<METRIC>(c.at(k), lqs[kXPad + k], uqs[kXPad + k], arg);
+            p = (k == b0 + KOFF) ? total : p;
+        }
+    }
     int b = b0;
     // head
     for (; b <= bend && b < bi0; b++) {
         ad_step<METRIC, W, EXACT, true>(b, M, w, arg, A, B, X, Y);
         ring_shift<W>(X, Y, xs[kXPad + b + W + 1], c.at(b + W + 1));
+        const int K = b + KOFF;
+        if (check && b > b0 && K >= 0 && K < m)
+            p += consume_lb<METRIC>(Y[KOFF], lqs[kXPad + K], uqs[kXPad + K], arg);
     }
     // interior, early abandon every 4 iterations
     int it = 0;
     for (; b <= bi1; b++, it++) {
         ad_step<METRIC, W, EXACT, false>(b, M, w, arg, A, B, X, Y);
         ring_shift<W>(X, Y, xs[kXPad + b + W + 1], c.at(b + W + 1));
-        if (check && (it & 3) == 3) {
-            alive = alive && ad_alive<METRIC, W>(A, B, b, M, m, thr);
-            if (!__any(alive)) break;
+        if (check) {
+            const int K = b + KOFF;
+            if (K < m) p += consume_lb<METRIC>(Y[KOFF], lqs[kXPad + K], uqs[kXPad + K], arg);
+            if ((it & 3) == 3) {
+                alive = alive && ad_alive<METRIC, W>(A, B, b, M, m, thr, total, p);
+                if (!__any(alive)) break;
+            }
         }
     }
     double result = S;
@@ -320,7 +381,9 @@ __device__ double eld_rows(const double* __restrict__ q, const double* __restric
 // in[] lists window indices (target mode) or, with rows != null, row numbers
 // 0..n_direct-1 of normalised candidates.  W == 0: generic row-band path.
 template <int METRIC, int W, bool EXACT>
-__global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const int64_t* __restrict__ in,
+__global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const double* __restrict__ LQ,
+                                             const double* __restrict__ UQ,
+                                             const int64_t* __restrict__ in,
                                              const unsigned long long* __restrict__ in_count,
                                              int64_t n_direct, const double* __restrict__ rows,
                                              int keep_all, int64_t* __restrict__ out_idx,
@@ -335,17 +398,27 @@ __global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const int64_t* __rest
     const int m = a.p.m, w = a.p.w;
     const double cut = a.p.cutoff;
     double thr = 1e308;
-    if (!keep_all) {
+    // abandoning needs non-negative step costs (MSM: c >= 0)
+    if (!keep_all && !(METRIC == MSM && a.p.arg < 0.0)) {
         if (METRIC == DTW || METRIC == ERP) thr = cut * cut * (1.0 + 1e-12) + 1e-300;
         else thr = cut * (1.0 + 1e-12) + 1e-300;
     }
     const int span = m + 2 * kXPad;
+    // LDS: query, [ERP: gap costs], query envelope lower, upper (the last two
+    // only when abandoning)
+    constexpr int NX = (METRIC == ERP) ? 2 : 1;
+    double* lqs = xlds + NX * span;
+    double* uqs = lqs + span;
     if (W > 0) {
         for (int i = threadIdx.x; i < span; i += blockDim.x) {
             const int qi = i - kXPad;
             const bool in_q = qi >= 0 && qi < m;
             xlds[i] = in_q ? a.q.qn[qi] : 0.0;
             if (METRIC == ERP) xlds[span + i] = in_q ? a.q.XG[qi] : 0.0;
+            if (!keep_all) {
+                lqs[i] = in_q ? LQ[qi] : 0.0;
+                uqs[i] = in_q ? UQ[qi] : 0.0;
+            }
         }
         __syncthreads();
     }
@@ -372,7 +445,8 @@ __global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const int64_t* __rest
         bool dead = false;
         double d;
         if constexpr (W > 0) {
-            d = eld_ad<METRIC, W, EXACT>(xlds, xlds + span, c, m, w, a.p.arg, thr, have, &dead);
+            d = eld_ad<METRIC, W, EXACT>(xlds, xlds + span, c, m, w, a.p.arg, thr, lqs, uqs, have,
+                                         &dead);
         } else {
             MemArr b{scratch + tid, stride * 2};
             MemArr y{scratch + tid + stride, stride * 2};
@@ -403,6 +477,12 @@ __global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const int64_t* __rest
 
 int64_t eld_scratch_threads() { return (int64_t)512 * 256; }
 
+bool eld_uses_scratch(int metric, int m, int w) {
+    const size_t lds = sizeof(double) * (size_t)(m + 2 * kXPad) * (metric == ERP ? 4 : 3);
+    if (lds > kEldLdsCap) return true;
+    return w > 32 || (w > 25 && metric == ERP);
+}
+
 template <int METRIC>
 static hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64_t* in,
                                 const unsigned long long* in_count, int64_t n_direct,
@@ -412,18 +492,20 @@ static hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64
     const int w = a.p.w;
     const dim3 blk(256), grid(2048);
     const int ka = keep_all ? 1 : 0;
-    const size_t lds = sizeof(double) * (size_t)(a.p.m + 2 * kXPad) * (METRIC == ERP ? 2 : 1);
+    // register variants stage query, [ERP gap costs,] and envelope in LDS
+    const size_t lds = sizeof(double) * (size_t)(a.p.m + 2 * kXPad) * (METRIC == ERP ? 4 : 3);
+    const bool fits = lds <= kEldLdsCap;
 #define FSM_ELD(WW, EX) \
-    hipLaunchKernelGGL((k_eld<METRIC, WW, EX>), grid, blk, lds, st, a, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch)
-    if (w == 6) FSM_ELD(6, true);
-    else if (w == 25) FSM_ELD(25, true);
-    else if (w <= 8) FSM_ELD(8, false);
-    else if (w <= 16) FSM_ELD(16, false);
-    else if (w <= 25) FSM_ELD(25, false);
-    else if (w <= 32 && METRIC != ERP) FSM_ELD(32, false);
+    hipLaunchKernelGGL((k_eld<METRIC, WW, EX>), grid, blk, lds, st, a, a.q.LQ, a.q.UQ, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch)
+    if (fits && w == 6) FSM_ELD(6, true);
+    else if (fits && w == 25) FSM_ELD(25, true);
+    else if (fits && w <= 8) FSM_ELD(8, false);
+    else if (fits && w <= 16) FSM_ELD(16, false);
+    else if (fits && w <= 25) FSM_ELD(25, false);
+    else if (fits && w <= 32 && METRIC != ERP) FSM_ELD(32, false);
     else {
         if (!scratch) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_eld<METRIC, 0, false>), dim3((unsigned)(scratch_threads / 256)), blk, 0, st, a, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch);
+        hipLaunchKernelGGL((k_eld<METRIC, 0, false>), dim3((unsigned)(scratch_threads / 256)), blk, 0, st, a, a.q.LQ, a.q.UQ, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch);
     }
 #undef FSM_ELD
     return hipGetLastError();

@@ -1,0 +1,518 @@
+// k_glb.hip -- the GLB filter (glb_elastic, src/glb.cpp:200-373) on gfx950.
+//
+// One candidate per lane.  Only the keep decision of the reference leaves
+// this kernel (the stage entry point excepted), and that decision depends on
+// three numbers: the boundary term and the final values of the two running
+// sums (glb_keeps).  The fast pass (glb_fast) evaluates the sums without
+// division and in a branch-free loop, with a rigorous bound on their distance
+// from the reference's sums; where the decision is the same at both ends of
+// that interval it is the reference's decision.  Otherwise the candidate
+// takes the exact path (glb_one), which evaluates every expression in the
+// reference's operand order, sums and early abandon included, so its value is
+// bit-identical to glb_elastic's.
+//
+// Inputs are valid windows (finite samples, sd > 1e-12) and a finite query:
+// this file is compiled with -fno-honor-nans.
+#include "engine_internal.h"
+
+namespace fsm {
+
+__device__ inline double sqd(double x, double y) { return (x - y) * (x - y); }  // include/def.h:5
+__device__ inline double mn(double x, double y) { return (x < y) ? x : y; }    // include/def.h:9
+__device__ inline double mx(double x, double y) { return (x > y) ? x : y; }    // include/def.h:10
+__device__ inline double edr_subcost(double a, double b, double eps) {          // src/glb.cpp:8-10
+    return (fabs(a - b) <= eps) ? 0.0 : 1.0;
+}
+__device__ inline bool env_match(double v, double lo, double hi, double eps) {  // src/glb.cpp:17-22
+    if (v < lo) return lo - v <= eps;
+    if (v > hi) return v - hi <= eps;
+    return true;
+}
+// std::min / std::max of the reference: min(a,b) = (b < a) ? b : a
+__device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ inline double smin3(double a, double b, double c) { return smin(smin(a, b), c); }
+
+// Envelope of the normalised candidate, radius w, clipped to the window
+// (lower_upper_lemire on t_temp, src/glb.cpp:302-304).  Normalisation
+// x -> (x - mu) / sd is monotone for sd > 0, so min / max are taken on raw
+// samples and normalised once; interior positions come from the
+// target-wide envelope.
+struct CandEnvelope {
+    const double* t;
+    const double* Lr;
+    const double* Ur;
+    int m, w;
+    int pre_r;  // prefix covers t[0 .. pre_r]
+    double pre_lo, pre_hi;
+    __device__ void init(const double* t_, const double* Lr_, const double* Ur_, int m_, int w_) {
+        t = t_; Lr = Lr_; Ur = Ur_; m = m_; w = w_;
+        pre_r = 0;
+        pre_lo = t[0];
+        pre_hi = t[0];
+    }
+    // raw min / max for position i; calls must use non-decreasing i.
+    __device__ void at(int i, double* lo, double* hi) {
+        if (i < w) {
+            const int r = (i + w < m - 1) ? i + w : m - 1;
+            while (pre_r < r) {
+                pre_r++;
+                pre_lo = fmin(pre_lo, t[pre_r]);
+                pre_hi = fmax(pre_hi, t[pre_r]);
+            }
+            *lo = pre_lo;
+            *hi = pre_hi;
+        } else if (i > m - 1 - w) {
+            double a = t[i - w], b = t[i - w];
+            for (int x = i - w + 1; x < m; x++) {
+                a = fmin(a, t[x]);
+                b = fmax(b, t[x]);
+            }
+            *lo = a;
+            *hi = b;
+        } else {
+            *lo = Lr[i];
+            *hi = Ur[i];
+        }
+    }
+};
+
+// Normalised samples (src/fsmeld.cpp:822-826): the exact path needs the
+// correctly rounded quotient fl(fl(t - mu) / sd); a = fl(fl(t - mu) * fl(1 / sd))
+// differs from it by at most 3.01 u |a| (u = 2^-53: three roundings of
+// relative size u, none of them shared).
+struct Norm {
+    double mu, sd, rinv;
+    __device__ double approx(double x) const { return (x - mu) * rinv; }
+    __device__ double exact(double x) const { return (x - mu) / sd; }
+};
+__device__ inline double nmargin(double a) { return fabs(a) * 8.881784197001252e-16 + 1e-300; }
+
+// T strictly inside [lo, hi] for every rounding of the approximate value a
+__device__ inline bool surely_inside(double a, double lo, double hi) {
+    const double d = nmargin(a);
+    return a - d > lo && a + d < hi;
+}
+
+// Boundary term of glb_elastic from the exact normalised samples T0 = t^[0],
+// Tl = t^[n-1], T2 = t^[n-2]: src/glb.cpp:217 (DTW), :226-230 (ERP), :242
+// (EDR), :264-275 (MSM); LCSS has none.
+template <int METRIC>
+__device__ inline double glb_boundary(double T0, double Tl, double T2, const double* Q, int n,
+                                      double arg) {
+    const int b = n - 1;
+    if (METRIC == DTW) return sqd(T0, Q[0]) + sqd(Tl, Q[b]);
+    if (METRIC == ERP) return smin3(sqd(Q[b], Tl), sqd(Q[b], arg), sqd(Tl, arg));
+    if (METRIC == EDR) return mn(edr_subcost(Q[b], Tl, arg), 1.0);
+    if (METRIC == MSM) {
+        if ((T2 >= Tl && Tl >= Q[b]) || (T2 <= Tl && Tl <= Q[b]) ||
+            (Q[b - 1] <= Q[b] && Q[b] <= Tl) || (Q[b - 1] >= Q[b] && Q[b] >= Tl))
+            return fabs(Q[0] - T0) + mn(fabs(Q[b] - Tl), arg);
+        return fabs(Q[0] - T0) +
+               smin3(fabs(Q[b] - Tl), arg + fabs(Tl - T2), arg + fabs(Q[b] - Q[b - 1]));
+    }
+    return 0.0;
+}
+
+// ----------------------------------------------------------- exact path ---
+template <int METRIC>
+__device__ double glb_one(const double* __restrict__ t, double mu, double sd,
+                          const double* __restrict__ Q, const double* __restrict__ LQ,
+                          const double* __restrict__ UQ, const double* Lr, const double* Ur,
+                          int m, int w, double arg, double bsf) {
+    if (m < 2) return 0.0;
+    const int n = m;
+    const Norm N{mu, sd, 1.0 / sd};
+    double boundary = 0.0, t2q = 0.0, lcss_t = 0.0;
+    const double Tl = N.exact(t[n - 1]);
+    if (METRIC != LCSS)
+        boundary = glb_boundary<METRIC>(N.exact(t[0]), Tl, N.exact(t[n - 2]), Q, n, arg);
+    // The reference tests early abandon after every position; positions that
+    // add nothing leave the test unchanged, so it is evaluated once before the
+    // loop (its first test) and after every contributing position.
+    if (METRIC == DTW) {
+        if (n > 2 && boundary + t2q > bsf * bsf) return kGlbAbandon;
+        for (int i = 1; i < n - 1; ++i) {
+            if (surely_inside(N.approx(t[i]), LQ[i], UQ[i])) continue;
+            const double v = N.exact(t[i]);
+            if (v < LQ[i]) t2q += sqd(v, LQ[i]);
+            else if (v > UQ[i]) t2q += sqd(v, UQ[i]);
+            if (boundary + t2q > bsf * bsf) return kGlbAbandon;
+        }
+    } else if (METRIC == ERP) {
+        if (n > 2 && boundary + t2q > bsf * bsf) return kGlbAbandon;
+        for (int i = 1; i < n - 1; ++i) {
+            if (surely_inside(N.approx(t[i]), LQ[i], UQ[i])) continue;
+            const double v = N.exact(t[i]);
+            if (v > UQ[i]) t2q += mn(sqd(v, UQ[i]), sqd(v, arg));
+            else if (v < LQ[i]) t2q += mn(sqd(v, LQ[i]), sqd(v, arg));
+            if (boundary + t2q > bsf * bsf) return kGlbAbandon;
+        }
+    } else if (METRIC == EDR) {
+        if (n > 2 && boundary + t2q > bsf) return kGlbAbandon;
+        for (int i = 1; i < n - 1; ++i) {
+            if (surely_inside(N.approx(t[i]), LQ[i], UQ[i])) continue;
+            const double v = N.exact(t[i]);
+            if (v > UQ[i]) t2q += edr_subcost(v, UQ[i], arg);
+            else if (v < LQ[i]) t2q += edr_subcost(v, LQ[i], arg);
+            if (boundary + t2q > bsf) return kGlbAbandon;
+        }
+    } else if (METRIC == LCSS) {
+        const double min_len = (double)n;
+        for (int i = 0; i < n; ++i) {
+            const double a = N.approx(t[i]);
+            if (surely_inside(a, LQ[i], UQ[i])) {
+                lcss_t += 1.0;
+                continue;
+            }
+            const double far = nmargin(a) + 8.881784197001252e-16 * (fabs(LQ[i]) + fabs(UQ[i]) + arg);
+            if (a + far < LQ[i] - arg || a - far > UQ[i] + arg) continue;  // surely no match
+            if (env_match(N.exact(t[i]), LQ[i], UQ[i], arg)) lcss_t += 1.0;
+        }
+        if (smax(0.0, 1.0 - lcss_t / min_len) > bsf) return kGlbAbandon;
+    } else {  // MSM
+        if (n > 2 && boundary + t2q > bsf) return kGlbAbandon;
+        for (int i = 1; i < n - 1; ++i) {
+            if (surely_inside(N.approx(t[i]), LQ[i], UQ[i])) continue;
+            const double v = N.exact(t[i]);
+            if (v > UQ[i]) t2q += mn(fabs(v - UQ[i]), arg);
+            else if (v < LQ[i]) t2q += mn(fabs(v - LQ[i]), arg);
+            if (boundary + t2q > bsf) return kGlbAbandon;
+        }
+    }
+
+    // candidate envelope (raw), normalised only where a decision needs it;
+    // the pass's first abandon test equals pass 1's and has already passed
+    CandEnvelope E;
+    E.init(t, Lr, Ur, m, w);
+    double q2t = 0.0, qr = 0.0;
+    const int i0 = (METRIC == LCSS) ? 0 : 1;
+    const int i1 = (METRIC == LCSS) ? n : n - 1;
+    for (int i = i0; i < i1; ++i) {
+        double lo, hi;
+        E.at(i, &lo, &hi);
+        const double aL = N.approx(lo), aU = N.approx(hi);
+        const double q = Q[i];
+        const bool inside = q - nmargin(aL) > aL && q + nmargin(aU) < aU;
+        if (METRIC == LCSS) {
+            if (inside) {
+                qr += 1.0;
+                continue;
+            }
+            const double far = nmargin(aL) + nmargin(aU) +
+                               8.881784197001252e-16 * (fabs(q) + arg);
+            if (q + far < aL - arg || q - far > aU + arg) continue;
+            if (env_match(q, N.exact(lo), N.exact(hi), arg)) qr += 1.0;
+            continue;
+        }
+        if (inside) continue;
+        const double LT = N.exact(lo), UT = N.exact(hi);
+        if (METRIC == DTW) {
+            if (q < LT) q2t += sqd(q, LT);
+            else if (q > UT) q2t += sqd(q, UT);
+            if (boundary + q2t > bsf * bsf) return kGlbAbandon;
+        } else if (METRIC == ERP) {
+            if (q > UT) q2t += mn(sqd(q, UT), sqd(q, arg));
+            else if (q < LT) q2t += mn(sqd(q, LT), sqd(q, arg));
+            if (boundary + q2t > bsf * bsf) return kGlbAbandon;
+        } else if (METRIC == EDR) {
+            if (q > UT) q2t += edr_subcost(q, UT, arg);
+            else if (q < LT) q2t += edr_subcost(q, LT, arg);
+            if (boundary + q2t > bsf) return kGlbAbandon;
+        } else {  // MSM
+            if (q > UT) q2t += mn(fabs(q - UT), arg);
+            else if (q < LT) q2t += mn(fabs(q - LT), arg);
+            if (boundary + q2t > bsf) return kGlbAbandon;
+        }
+    }
+    if (METRIC == DTW || METRIC == ERP) return sqrt(smax(0.0, boundary + mx(t2q, q2t)));
+    if (METRIC == LCSS) return smax(0.0, 1.0 - mn(lcss_t, qr) / (double)n);
+    return boundary + mx(t2q, q2t);
+}
+
+// ------------------------------------------------------------ fast path ---
+// The reference's keep decision (glb_elastic then `<= cutoff`,
+// src/fsmeld.cpp:827-831) from the boundary B and the final running sums T
+// (candidate to query envelope) and Q (query to candidate envelope); for
+// LCSS T and Q are the match counts.  glb_elastic abandons (1008610086) once
+// B plus a running sum exceeds bsf (bsf^2 for DTW / ERP); running sums never
+// decrease, so the final sums decide the same.  Valid for bsf < 1008610086,
+// where an abandoned candidate is never kept.  Every operation is monotone,
+// so the decision is monotone in T and Q: non-increasing for the distance
+// sums, non-decreasing for the LCSS counts.
+template <int METRIC>
+__device__ inline bool glb_keeps(double B, double T, double Q, int n, double bsf) {
+    if (METRIC == DTW || METRIC == ERP) {
+        const double b2 = bsf * bsf;
+        if (n > 2 && (B + T > b2 || B + Q > b2)) return false;
+        return sqrt(smax(0.0, B + mx(T, Q))) <= bsf;
+    } else if (METRIC == LCSS) {
+        const double nn = (double)n;
+        if (smax(0.0, 1.0 - T / nn) > bsf) return false;
+        return smax(0.0, 1.0 - mn(T, Q) / nn) <= bsf;
+    } else {
+        if (n > 2 && (B + T > bsf || B + Q > bsf)) return false;
+        return B + mx(T, Q) <= bsf;
+    }
+}
+
+// Bound on |reference sum - fast sum| for DTW, ERP and MSM.  The fast sum S
+// adds at most n terms.  Each term is a function of a distance to an
+// envelope whose arguments moved by at most delta (normalisation): MSM's
+// min(d, c) is 1-Lipschitz; DTW's d^2 and ERP's min(d^2, (x - g)^2) move by
+// at most delta (2K + delta) where K bounds the distances involved.  Both sums
+// also carry term rounding (<= 3u relative) and summation in different orders
+// (<= (n - 1) u relative each), psi below.
+template <int METRIC>
+__device__ inline double glb_sum_err(double S, int n, double delta, double K) {
+    const double psi = (double)(n + 8) * 1.2e-16;
+    const double D = (METRIC == MSM) ? (double)n * delta : (double)n * delta * (2.0 * K + delta);
+    return (D * (1.0 + psi) + 2.0 * psi * S * (1.0 + 2.0 * psi)) * 1.01 + 1e-300;
+}
+
+constexpr int kGlbChunk = 16;  // positions between wave-level exit tests
+
+// Contribution of one position with envelope distance e >= 0 (fast path).
+// xg2 = (x - g)^2 of the moving sample (ERP); for EDR / LCSS, r = e - eps
+// decides mismatch / match unless |r| <= eta.
+template <int METRIC>
+__device__ inline void fast_term(double e, double xg2, double arg, double eta, double& s,
+                                 int& sure, int& unsure) {
+    if (METRIC == DTW) {
+        s = fma(e, e, s);
+    } else if (METRIC == ERP) {
+        s += fmin(e * e, xg2);
+    } else if (METRIC == MSM) {
+        s += fmin(e, arg);
+    } else {
+        const double r = e - arg;
+        const bool u = fabs(r) <= eta;
+        unsure += u ? 1 : 0;
+        if (METRIC == EDR) sure += (!u && r > 0.0) ? 1 : 0;  // mismatch
+        else sure += (!u && r < 0.0) ? 1 : 0;                // match
+    }
+}
+
+// Fast decision for one candidate: 1 keep, -1 prune, 0 undecided (take the
+// exact path).  Every lane of the wave calls it; loop trip counts are
+// uniform and the loops end early only when every lane has been pruned.
+// Query arrays reach the kernel as separate read-only, non-aliasing pointers
+// so that their wave-uniform reads become scalar loads.
+template <int METRIC>
+__device__ int glb_fast(const VerifyArgs& A, const double* __restrict__ Q,
+                        const double* __restrict__ LQ, const double* __restrict__ UQ,
+                        const double* __restrict__ XG, const double* __restrict__ t,
+                        const double* __restrict__ Lr, const double* __restrict__ Ur,
+                        double mu, double sd, bool active) {
+    constexpr bool COUNT = (METRIC == EDR || METRIC == LCSS);
+    const int n = A.p.m, w = A.p.w;
+    const double arg = A.p.arg, bsf = A.p.cutoff;
+    const double thr = (METRIC == DTW || METRIC == ERP) ? bsf * bsf : bsf;
+    const double QM = A.q.qabs;
+    const double r = 1.0 / sd;
+    const int i0 = (METRIC == LCSS) ? 0 : 1;
+    const int i1 = (METRIC == LCSS) ? n : n - 1;
+    const int npos = i1 - i0;
+
+    double B = 0.0;
+    if (METRIC != LCSS)
+        B = glb_boundary<METRIC>((t[0] - mu) / sd, (t[n - 1] - mu) / sd, (t[n - 2] - mu) / sd, Q,
+                                 n, arg);
+    // amax bounds |a| over every sample of the window; the candidate
+    // envelope's normalised values are such samples (normalisation is monotone)
+    double amax = fmax(fabs((t[0] - mu) * r), fabs((t[n - 1] - mu) * r));
+    const double kq = fmax(QM, fabs(arg));
+    const double ceta = 8e-16 * (QM + fabs(arg)) + 1e-300;
+    bool dead = !active;
+
+    // pass 1: candidate samples against the query envelope
+    double s1 = 0.0;
+    int sure1 = 0, uns1 = 0;
+    auto term1 = [&](int k) {
+        const double av = (t[k] - mu) * r;
+        amax = fmax(amax, fabs(av));
+        const double e = fmax(fmax(LQ[k] - av, av - UQ[k]), 0.0);
+        const double xg = av - arg;
+        fast_term<METRIC>(e, xg * xg, arg, fma(fabs(av), 8e-16, ceta), s1, sure1, uns1);
+    };
+    auto check1 = [&](int ie) {
+        if (dead) return;
+        if (METRIC == LCSS) {
+            const double tmax = (double)(sure1 + uns1 + (i1 - ie));
+            dead = smax(0.0, 1.0 - tmax / (double)n) > bsf;
+        } else if (METRIC == EDR) {
+            dead = B + (double)sure1 > bsf;
+        } else {
+            const double lo = s1 - glb_sum_err<METRIC>(s1, n, 4e-16 * amax + 1e-300, amax + kq);
+            dead = B + lo > thr;
+        }
+    };
+    int i = i0;
+    for (; i + kGlbChunk <= i1; i += kGlbChunk) {
+#pragma unroll
+        for (int u = 0; u < kGlbChunk; u++) term1(i + u);
+        check1(i + kGlbChunk);
+        if (__all(dead)) return -1;
+    }
+    for (; i < i1; i++) term1(i);
+
+    // pass 2: query against the candidate envelope (raw window min / max over
+    // positions clipped to the window, normalised like pass 1)
+    const double eta2 = 8e-16 * (QM + amax + fabs(arg)) + 1e-300;
+    double s2 = 0.0;
+    int sure2 = 0, uns2 = 0;
+    int done2 = 0;
+    auto term2 = [&](int k, double lo, double hi) {
+        const double aL = (lo - mu) * r, aU = (hi - mu) * r;
+        const double q = Q[k];
+        const double e = fmax(fmax(aL - q, q - aU), 0.0);
+        fast_term<METRIC>(e, (METRIC == ERP) ? XG[k] : 0.0, arg, eta2, s2, sure2, uns2);
+    };
+    auto check2 = [&]() {
+        if (dead) return;
+        if (METRIC == LCSS) {
+            const double qmax = (double)(sure2 + uns2 + (npos - done2));
+            dead = smax(0.0, 1.0 - mn((double)(sure1 + uns1), qmax) / (double)n) > bsf;
+        } else if (METRIC == EDR) {
+            dead = B + (double)sure2 > bsf;
+        } else {
+            const double lo = s2 - glb_sum_err<METRIC>(s2, n, 4e-16 * amax + 1e-300, amax + kq);
+            dead = B + lo > thr;
+        }
+    };
+    // head: positions i < w see the window prefix t[0 .. min(i + w, n - 1)]
+    const int h1 = min(w, i1);
+    {
+        double plo = t[0], phi = t[0];
+        int pr = 0;
+        for (int k = i0; k < h1; k++) {
+            const int rr = min(k + w, n - 1);
+            while (pr < rr) {
+                pr++;
+                plo = fmin(plo, t[pr]);
+                phi = fmax(phi, t[pr]);
+            }
+            term2(k, plo, phi);
+        }
+        done2 += max(h1 - i0, 0);
+    }
+    // interior: w <= i < n - w, the target-wide envelope
+    const int m0 = max(i0, w), m1 = min(i1, n - w);
+    int k0 = m0;
+    for (; k0 + kGlbChunk <= m1; k0 += kGlbChunk) {
+#pragma unroll
+        for (int u = 0; u < kGlbChunk; u++) term2(k0 + u, Lr[k0 + u], Ur[k0 + u]);
+        done2 += kGlbChunk;
+        check2();
+        if (__all(dead)) return -1;
+    }
+    for (; k0 < m1; k0++, done2++) term2(k0, Lr[k0], Ur[k0]);
+    // tail: positions i >= max(w, n - w) see the suffix t[i - w .. n - 1];
+    // walked from the right end so the suffix only grows
+    const int tb = max(max(i0, w), n - w);
+    {
+        double slo = t[n - 1], shi = t[n - 1];
+        int sl = n - 1;
+        for (int k = i1 - 1; k >= tb; k--) {
+            const int ll = k - w;
+            while (sl > ll) {
+                sl--;
+                slo = fmin(slo, t[sl]);
+                shi = fmax(shi, t[sl]);
+            }
+            term2(k, slo, shi);
+        }
+        done2 += max(i1 - tb, 0);
+    }
+    check2();
+    if (dead) return -1;
+
+    double Tlo, Thi, Qlo, Qhi;
+    if (COUNT) {
+        Tlo = (double)sure1;
+        Thi = (double)(sure1 + uns1);
+        Qlo = (double)sure2;
+        Qhi = (double)(sure2 + uns2);
+    } else {
+        const double delta = 4e-16 * amax + 1e-300, K = amax + kq;
+        const double e1 = glb_sum_err<METRIC>(s1, n, delta, K);
+        const double e2 = glb_sum_err<METRIC>(s2, n, delta, K);
+        Tlo = fmax(s1 - e1, 0.0);
+        Thi = s1 + e1;
+        Qlo = fmax(s2 - e2, 0.0);
+        Qhi = s2 + e2;
+    }
+    if (METRIC == LCSS) {
+        if (glb_keeps<METRIC>(B, Tlo, Qlo, n, bsf)) return 1;
+        if (!glb_keeps<METRIC>(B, Thi, Qhi, n, bsf)) return -1;
+        return 0;
+    }
+    if (glb_keeps<METRIC>(B, Thi, Qhi, n, bsf)) return 1;
+    if (!glb_keeps<METRIC>(B, Tlo, Qlo, n, bsf)) return -1;
+    return 0;
+}
+
+template <int METRIC>
+__global__ void __launch_bounds__(256) k_glb(VerifyArgs a, const double* __restrict__ Q,
+                                             const double* __restrict__ LQ,
+                                             const double* __restrict__ UQ,
+                                             const double* __restrict__ XG,
+                                             const int64_t* __restrict__ in,
+                                             const unsigned long long* __restrict__ in_count,
+                                             int64_t* __restrict__ out,
+                                             unsigned long long* out_count,
+                                             double* __restrict__ vals) {
+    __shared__ unsigned long long sh[256 / 64 + 1];
+    const int64_t n = (int64_t)*in_count;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t iters = (n + stride - 1) / stride;
+    // the fast decision needs contributions >= 0 and, for EDR / LCSS, an
+    // envelope that counts as a match (eps > 0); values (vals) need the exact path
+    const bool fast = vals == nullptr && a.p.cutoff < kGlbAbandon && a.p.m > 2 &&
+                      ((METRIC == DTW || METRIC == ERP) || (METRIC == MSM && a.p.arg >= 0.0) ||
+                       a.p.arg > 0.0);
+    int64_t ci = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t it = 0; it < iters; it++, ci += stride) {
+        const bool active = ci < n;
+        const int64_t g = active ? in[ci] : 0;
+        const int64_t si = stat_index(a.G, g);
+        const double mu = active ? a.MU[si] : 0.0;
+        const double sd = active ? a.SG[si] : 1.0;
+        int verdict = 0;
+        if (fast)
+            verdict = glb_fast<METRIC>(a, Q, LQ, UQ, XG, a.T + g, a.Lraw + g, a.Uraw + g, mu, sd,
+                                       active);
+        bool keep = verdict > 0;
+        if (active && verdict == 0) {
+            const double v = glb_one<METRIC>(a.T + g, mu, sd, Q, LQ, UQ, a.Lraw + g, a.Uraw + g,
+                                             a.p.m, a.p.w, a.p.arg, a.p.cutoff);
+            if (vals) vals[ci] = v;
+            keep = v <= a.p.cutoff;  // src/fsmeld.cpp:831
+        }
+        keep = keep && active;
+        if (out) block_append(keep, g, 0.0, out, nullptr, out_count, sh);
+    }
+}
+
+static int verify_blocks() { return 2048; }
+
+hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
+                      const unsigned long long* in_count, int64_t* out,
+                      unsigned long long* out_count, double* vals) {
+    const dim3 g(verify_blocks()), b(256);
+    const QueryTables& q = a.q;
+#define FSM_GLB(MM) \
+    hipLaunchKernelGGL(k_glb<MM>, g, b, 0, st, a, q.qn, q.LQ, q.UQ, q.XG, in, in_count, out, out_count, vals)
+    switch (a.p.metric) {
+        case DTW: FSM_GLB(DTW); break;
+        case ERP: FSM_GLB(ERP); break;
+        case EDR: FSM_GLB(EDR); break;
+        case LCSS: FSM_GLB(LCSS); break;
+        case MSM: FSM_GLB(MSM); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef FSM_GLB
+    return hipGetLastError();
+}
+
+}  // namespace fsm

@@ -121,6 +121,12 @@ int fsm_stage_lbip(fsm_target* t, const fsm_query_desc* desc, const double* quer
 int fsm_stage_glb(fsm_target* t, const fsm_query_desc* desc, const double* query_raw,
                   const int64_t* windows, int64_t n, double* out);
 
+/* GLB keep decision for each given window (keep[i] = 1 when
+ * glb_elastic(...) <= desc->cutoff, src/fsmeld.cpp:827-831), computed by the
+ * production filter kernel that search and cutoff use. */
+int fsm_stage_glb_filter(fsm_target* t, const fsm_query_desc* desc, const double* query_raw,
+                         const int64_t* windows, int64_t n, uint8_t* keep);
+
 /* Exact elastic distance for each given window
  * (exact_elastic_distance, include/fsmeld_internal.h:27). */
 int fsm_stage_eld(fsm_target* t, const fsm_query_desc* desc, const double* query_raw,

@@ -57,7 +57,8 @@ def test_window_stats_exclusion(gpu, small):
 
 
 @pytest.mark.parametrize("metric,arg", METRICS)
-@pytest.mark.parametrize("m,w", [(2, 0), (7, 1), (33, 0), (40, 6), (64, 25), (50, 33), (24, 23)])
+@pytest.mark.parametrize("m,w", [(2, 0), (7, 1), (33, 0), (40, 6), (64, 25), (50, 33), (24, 23),
+                                 (64, 30), (2100, 8)])
 def test_eld_pairs_bitexact(gpu, metric, arg, m, w):
     rng = np.random.default_rng(m * 100 + w)
     q = rng.normal(size=m)
@@ -88,6 +89,47 @@ def test_eld_and_glb_windows_bitexact(gpu, small, metric, arg):
                            for x in win])
         assert same_bits(g, want_g)
         assert np.all(want_g[want_g < 1e9] <= want_d[want_g < 1e9] + 1e-12)
+
+
+@pytest.mark.parametrize("metric,arg", METRICS)
+@pytest.mark.parametrize("m,w", [(64, 6), (50, 30), (40, 0), (3, 1)])
+def test_glb_filter_decisions(gpu, small, metric, arg, m, w):
+    """The search filter's keep decisions equal `glb <= bsf` of the oracle,
+    with bsf set exactly on, and one ulp below, some windows' GLB values so
+    that the decision interval of the fast pass straddles the threshold."""
+    t, T = small
+    q, _ = F.synth_query(11, t, m, 0.1)
+    mu, sg, inv = O.window_stats(t, m)
+    win = np.flatnonzero(~inv)[::13][:2500]
+    qn = O.normalize_query(q, m)
+    LQ, UQ = O.lemire(qn, w)
+    cand = [(t[g:g + m] - mu[g]) / sg[g] for g in win]
+    full = np.array([O.glb(metric, qn, c, arg, LQ, UQ, w, 1e30) for c in cand])
+    picks = np.sort(full)[[len(full) // 50, len(full) // 5, len(full) // 2]]
+    for bsf in list(picks) + [np.nextafter(p, -np.inf) for p in picks]:
+        keep = T.glb_keep(q, m, metric, w, arg, win, bsf)
+        want = np.array([O.glb(metric, qn, c, arg, LQ, UQ, w, bsf) for c in cand]) <= bsf
+        bad = np.flatnonzero(keep != want)
+        assert bad.size == 0, f"bsf={bsf!r}: windows {win[bad][:5]} differ"
+
+
+@pytest.mark.parametrize("metric,arg", METRICS)
+@pytest.mark.parametrize("m,w", [(64, 4), (64, 12), (100, 30)])
+def test_exact_command_abandon_is_safe(gpu, small, metric, arg, m, w):
+    """The exact command abandons windows by DP frontier + suffix bound; with
+    the cutoff set exactly on windows' distances it must return precisely the
+    windows whose distance (all valid windows, no abandoning) is <= cutoff."""
+    t, T = small
+    q, _ = F.synth_query(17, t, m, 0.1)
+    _, _, inv = O.window_stats(t, m)
+    win = np.flatnonzero(~inv)
+    d = T.eld(q, m, metric, w, arg, win)
+    ds = np.sort(d)
+    for cut in (ds[0], ds[5], ds[len(ds) // 1000], ds[len(ds) // 100], ds[len(ds) // 20]):
+        e = T.exact(q, m, cut, metric, w, arg)
+        want = win[d <= cut]
+        assert np.array_equal(e.indices, want), f"cut={cut!r}"
+        assert same_bits(e.distances, d[d <= cut])
 
 
 @pytest.mark.parametrize("metric,arg", METRICS)
