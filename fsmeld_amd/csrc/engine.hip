@@ -339,7 +339,9 @@ QueryParams params_of(const fsm_query_desc* d) {
     return P;
 }
 
-// counters: [0] compacted, [1] valid, [2] after GLB, [3] ELD kept
+// counters: [0] compacted, [1] valid, [2] after GLB, [3] ELD kept, [8] ELD work
+// queue.  The LB buffer is free once compaction has read it; GLB writes each
+// survivor's consumption bound there for the ELD abandon test.
 int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
                       const QueryParams& P, bool use_lb, bool use_glb) {
     unsigned long long* C = t->counters.as<unsigned long long>();
@@ -355,17 +357,20 @@ int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
     VerifyArgs v = verify_args(t, G, Q, P);
     const int64_t* eld_in = t->list_a.as<int64_t>();
     const unsigned long long* eld_cnt = C;
+    const double* eld_total = nullptr;
     if (use_glb) {
+        HIP_TRY(t->lb.ensure(sizeof(double) * (size_t)G.ts_n));
         HIP_TRY(launch_glb(t->stream, v, t->list_a.as<int64_t>(), C, t->list_b.as<int64_t>(),
-                           C + 2, nullptr));
+                           C + 2, t->lb.as<double>(), nullptr));
         eld_in = t->list_b.as<int64_t>();
         eld_cnt = C + 2;
+        eld_total = t->lb.as<double>();
     }
     HIP_TRY(hipEventRecord(t->ev[4], t->stream));
     int rc = ensure_eld_scratch(t, P.metric, P.m, P.w);
     if (rc) return rc;
-    HIP_TRY(launch_eld(t->stream, v, eld_in, eld_cnt, false, t->out_idx.as<int64_t>(),
-                       t->out_dist.as<double>(), C + 3, t->scratch.as<double>(),
+    HIP_TRY(launch_eld(t->stream, v, eld_in, eld_total, eld_cnt, false, t->out_idx.as<int64_t>(),
+                       t->out_dist.as<double>(), C + 3, C + 8, t->scratch.as<double>(),
                        eld_scratch_threads()));
     HIP_TRY(hipEventRecord(t->ev[5], t->stream));
     return FSM_OK;
@@ -401,7 +406,7 @@ int fsm_target_open(const double* host, int64_t n, fsm_target** out) {
             return cleanup(fail(FSM_ERR_HIP, "hipEventCreate failed"));
     if (t->T.ensure(sizeof(double) * (size_t)n) != hipSuccess)
         return cleanup(fail(FSM_ERR_OOM, "device allocation of the target failed"));
-    if (t->counters.ensure(sizeof(unsigned long long) * 8) != hipSuccess)
+    if (t->counters.ensure(sizeof(unsigned long long) * 16) != hipSuccess)
         return cleanup(fail(FSM_ERR_OOM, "device allocation failed"));
     if (hipMemcpy(t->T.p, host, sizeof(double) * (size_t)n, hipMemcpyHostToDevice) !=
         hipSuccess)
@@ -523,9 +528,9 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
                 rc = ensure_eld_scratch(t, d->metric, d->m, d->w);
                 if (rc) return bail(rc);
                 VerifyArgs v = verify_args(t, G, Q, P);
-                HIP_TRY(launch_eld(t->stream, v, t->list_b.as<int64_t>(), Cd + 4, true, nullptr,
-                                   t->out_dist.as<double>(), nullptr, t->scratch.as<double>(),
-                                   eld_scratch_threads()));
+                HIP_TRY(launch_eld(t->stream, v, t->list_b.as<int64_t>(), nullptr, Cd + 4, true,
+                                   nullptr, t->out_dist.as<double>(), nullptr, Cd + 8,
+                                   t->scratch.as<double>(), eld_scratch_threads()));
                 std::vector<double> rd(rl.size());
                 HIP_TRY(hipMemcpyAsync(rd.data(), t->out_dist.p, sizeof(double) * rd.size(),
                                        hipMemcpyDeviceToHost, t->stream));
@@ -697,13 +702,13 @@ static int stage_verify(fsm_target* t, const fsm_query_desc* d, const double* q,
     VerifyArgs v = verify_args(t, G, Q, P);
     if (kind == STAGE_GLB_VALUES)
         HIP_TRY(launch_glb(t->stream, v, t->list_a.as<int64_t>(), Cd + 6, nullptr, nullptr,
-                           t->out_dist.as<double>()));
+                           nullptr, t->out_dist.as<double>()));
     else if (kind == STAGE_GLB_KEEP)
         HIP_TRY(launch_glb(t->stream, v, t->list_a.as<int64_t>(), Cd + 6, t->list_b.as<int64_t>(),
-                           Cd + 7, nullptr));
+                           Cd + 7, nullptr, nullptr));
     else
-        HIP_TRY(launch_eld(t->stream, v, t->list_a.as<int64_t>(), Cd + 6, true, nullptr,
-                           t->out_dist.as<double>(), nullptr, t->scratch.as<double>(),
+        HIP_TRY(launch_eld(t->stream, v, t->list_a.as<int64_t>(), nullptr, Cd + 6, true, nullptr,
+                           t->out_dist.as<double>(), nullptr, Cd + 8, t->scratch.as<double>(),
                            eld_scratch_threads()));
     HIP_TRY(hipStreamSynchronize(t->stream));
     if (kind != STAGE_GLB_KEEP) {
@@ -749,7 +754,7 @@ int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const d
         if (!std::isfinite(cands[i])) return fail(FSM_ERR_ARG, "candidate values must be finite");
     if (fsm_device_count() < 1) return fail(FSM_ERR_NODEV, "no HIP device");
     HIP_TRY(hipSetDevice(0));
-    DevBuf dq, dxg, dc, dout, dscr;
+    DevBuf dq, dxg, dc, dout, dscr, dqueue;
     const int64_t thr = eld_scratch_threads();
     auto done = [&](int rc) {
         dq.release();
@@ -757,11 +762,12 @@ int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const d
         dc.release();
         dout.release();
         dscr.release();
+        dqueue.release();
         return rc;
     };
     if (dq.ensure(sizeof(double) * m) || dxg.ensure(sizeof(double) * m) ||
         dc.ensure(sizeof(double) * (size_t)m * n) ||
-        dout.ensure(sizeof(double) * n))
+        dout.ensure(sizeof(double) * n) || dqueue.ensure(sizeof(unsigned long long)))
         return done(fail(FSM_ERR_OOM, "device allocation failed"));
     if (eld_uses_scratch(metric, m, w) &&
         dscr.ensure(sizeof(double) * 2 * (size_t)(2 * w + 2) * (size_t)thr))
@@ -779,8 +785,8 @@ int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const d
     va.p.m = m;
     va.p.w = w;
     va.p.arg = arg;
-    if (launch_eld_pairs(nullptr, va, dc.as<double>(), n, dout.as<double>(), dscr.as<double>(),
-                         thr) != hipSuccess)
+    if (launch_eld_pairs(nullptr, va, dc.as<double>(), n, dout.as<double>(),
+                         dqueue.as<unsigned long long>(), dscr.as<double>(), thr) != hipSuccess)
         return done(fail(FSM_ERR_HIP, "ELD launch failed"));
     if (hipDeviceSynchronize() != hipSuccess ||
         hipMemcpy(out, dout.p, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)

@@ -145,21 +145,26 @@ struct VerifyArgs {
     QueryParams p;
     Geometry G;
 };
-// GLB filter over in[0 .. *in_count): survivors appended to out (out_count);
-// values written to vals[i] when vals != null (stage-level entry point).
+// GLB filter over in[0 .. *in_count): survivors appended to out (out_count),
+// with, when out_total != null, each survivor's certified consumption bound
+// for the ELD abandon test beside it (out_total[k] belongs to out[k]); values
+// written to vals[i] when vals != null (stage-level entry point).
 hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
                       const unsigned long long* in_count, int64_t* out,
-                      unsigned long long* out_count, double* vals);
-// Exact distances over in[0 .. *in_count).  keep_all: write dist[i] for every
-// input; else append (index, distance) for distances <= P.cutoff.
+                      unsigned long long* out_count, double* out_total, double* vals);
+// Exact distances over in[0 .. *in_count) (in_total: the GLB bounds of in[],
+// or null).  keep_all: write dist[i] for every input; else append (index,
+// distance) for distances <= P.cutoff.  queue: one device counter the launch
+// resets and uses to hand out candidates.
 hipError_t launch_eld(hipStream_t st, const VerifyArgs& a, const int64_t* in,
-                      const unsigned long long* in_count, bool keep_all, int64_t* out_idx,
-                      double* out_dist, unsigned long long* out_count, double* scratch,
-                      int64_t scratch_threads);
+                      const double* in_total, const unsigned long long* in_count, bool keep_all,
+                      int64_t* out_idx, double* out_dist, unsigned long long* out_count,
+                      unsigned long long* queue, double* scratch, int64_t scratch_threads);
 // Exact distances of normalised rows cands[n][m] (no target); a.q.qn / a.q.XG
 // and a.p.{metric, m, w, arg} describe the query.
 hipError_t launch_eld_pairs(hipStream_t st, const VerifyArgs& a, const double* cands,
-                            int64_t n, double* out, double* scratch, int64_t scratch_threads);
+                            int64_t n, double* out, unsigned long long* queue, double* scratch,
+                            int64_t scratch_threads);
 int64_t eld_scratch_threads();
 // true when (metric, m, w) runs the row-band path, which needs the scratch
 // buffer of 2 (2w + 2) doubles per thread of eld_scratch_threads()

@@ -10,8 +10,11 @@
 // of one parity from the other parity (anti-diagonal d-1: "up" (i-1, j) and
 // "left" (i, j-1)) and from their own old value (d-2: "diag" (i-1, j-1)), so
 // the cells of one anti-diagonal are independent: a lane has W+1 independent
-// cells per step.  Row / column indices are wave-uniform, so boundary cases
-// are scalar branches.  Query samples are wave-uniform; candidate samples are
+// cells per step.  The matrix boundary is padding (see ad_step), so every
+// step runs the same branch-free code and each lane advances through its own
+// candidate at its own iteration: a lane whose candidate ends takes the next
+// one (lane refill), and a wave no longer waits for its slowest lane before
+// starting new work.  Query samples come from LDS; candidate samples are
 // normalised once each ((t - mu) / sigma, src/fsmeld.cpp:822-826) into a
 // register ring.
 //
@@ -20,7 +23,8 @@
 // to consume every candidate sample beyond the cell's column.  Consuming
 // sample y_k costs at least its distance to the query envelope over the band
 // (GLB's per-position term: d^2 for DTW, min(d^2, (y-g)^2) for ERP, [d > eps]
-// for EDR, min(d, c) for MSM).  So the final cost is at least
+// for EDR, min(d, c) for MSM), whose sum over the window the GLB pass hands
+// over with each surviving candidate.  So the final cost is at least
 //   min over the two anti-diagonals of the cell value + sum of those terms
 //   over the samples beyond the largest column of the two anti-diagonals,
 // and once that exceeds the cutoff (squared for DTW/ERP, relative margin
@@ -125,56 +129,62 @@ __device__ inline double cell(int i, int j, double up, double left, double diag,
     }
 }
 
-// Query staging in LDS: xs[kXPad + i] = x[i] for i in [0, m), 0 elsewhere in
-// [-kXPad, m + kXPad); for ERP xg holds (x[i] - g)^2 likewise.
+// ------------------------------------------------------ register variants ---
+// Query staging in LDS: xs[i] = x[i] for i in [0, m) and kPad for the kXPad
+// entries on either side; with abandoning, the query envelope LQ / UQ follows.
 constexpr int kXPad = 40;
 constexpr size_t kEldLdsCap = 64 * 1024;
+constexpr double kPad = 1e300;
+constexpr int kRefillMin = 8;  // idle lanes that trigger a wave's refill
 
-// Anti-diagonal DP for one lane, compile-time band half-width W >= w
-// (EXACT: w == W, so every slot is in band).  Iteration b handles
-// anti-diagonals dA = 2b + W (even slots 2r, cells (b + r, b + W - r)) and
-// dB = dA + 1 (odd slots 2r+1, cells (b + 1 + r, b + W - r)).  The candidate
-// ring holds Y[k] = y[b - 1 + k], k in [0, W+1]; a cell reads y[j] =
-// Y[j - b + 1] and y[j-1] = Y[j - b]; query samples come from LDS at
-// wave-uniform addresses.
-// One iteration (two anti-diagonals) of the band update; EDGE selects the
-// boundary-checked form.
-template <int METRIC, int W, bool EXACT, bool EDGE>
-__device__ __forceinline__ void ad_step(int b, int M, int w, double arg, double* A, double* B,
-                                        const double* X, const double* Y) {
-    const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;
+// Padded boundary.  Query and candidate samples outside [0, m) read as kPad;
+// the two anti-diagonals before the first iteration start as the
+// out-of-range sentinel S (0 for LCSS) except the virtual cell (-1, -1) = 0,
+// which seeds (0, 0) through its diagonal.  Then the interior recurrence
+// gives every in-range cell the reference's value: a cell with a negative
+// index costs a padded sample against a real one (>= 1e300: its square is
+// infinite, an LCSS match impossible, an MSM split / merge between two pads
+// costs c + ~1e300), so such cells stay at S or above and min / max always
+// pick the reference's own operand; the row / column 0 formulas of
+// src/ElasticMeasures.cpp fall out exactly (x + min(S, v, S) = x + v).  With
+// odd W the first iteration recomputes anti-diagonal -1, whose cells see the
+// virtual cell as up / left, each through a cost of at least 1e300 (EDR: its
+// unit costs leave row / column -1 at j + 1 / i + 1, never below row /
+// column 0's values).  For ERP / EDR cell (0, 0) matches pad against pad at
+// cost 0 (EDR with eps < 0 costs 1: the virtual cell is -1 there).  Cells
+// past row or column M never feed (M, M).
+
+// One iteration (two anti-diagonals) of the band update.  Iteration b
+// handles anti-diagonals dA = 2b + W (even slots 2r, cells (b + r, b + W - r))
+// and dB = dA + 1 (odd slots 2r+1, cells (b + 1 + r, b + W - r)); the rings
+// hold X[k] = x[b - 1 + k], Y[k] = y[b - 1 + k].  A cell reads x[i] = X[r+1]
+// (x[i-1] = X[r]) and y[j] = Y[j - b + 1] (y[j-1] = Y[j - b]).
+template <int METRIC, int W, bool EXACT>
+__device__ __forceinline__ void ad_step(int w, double arg, double* A, double* B, const double* X,
+                                        const double* Y) {
+    const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;  // out-of-band neighbour
 #pragma unroll
     for (int r = 0; r <= W; r++) {
-        const int i = b + r, j = b + W - r;
         const int delta = 2 * r - W;
-        const bool inb = EXACT || (delta <= w && delta >= -w);
-        const bool inr = !EDGE || (i >= 0 && j >= 0 && i <= M && j <= M);
-        if (inb && inr) {
+        if (EXACT || (delta <= w && delta >= -w)) {
             const double up = (r >= 1) ? B[r - 1] : S;
             const double left = (r <= W - 1) ? B[r] : S;
-            A[r] = cell<METRIC, EDGE>(i, j, up, left, A[r], X[r + 1], X[r], Y[W - r + 1],
-                                      Y[W - r],
-                                      METRIC == ERP ? (X[r] - arg) * (X[r] - arg) : 0.0,
-                                      METRIC == ERP ? (Y[W - r] - arg) * (Y[W - r] - arg) : 0.0,
-                                      arg);
-        } else if (EDGE) {
-            A[r] = S;
+            A[r] = cell<METRIC, false>(0, 0, up, left, A[r], X[r + 1], X[r], Y[W - r + 1],
+                                       Y[W - r],
+                                       METRIC == ERP ? (X[r] - arg) * (X[r] - arg) : 0.0,
+                                       METRIC == ERP ? (Y[W - r] - arg) * (Y[W - r] - arg) : 0.0,
+                                       arg);
         }
     }
 #pragma unroll
     for (int r = 0; r < W; r++) {
-        const int i = b + 1 + r, j = b + W - r;
         const int delta = 2 * r + 1 - W;
-        const bool inb = EXACT || (delta <= w && delta >= -w);
-        const bool inr = !EDGE || (i >= 0 && j >= 0 && i <= M && j <= M);
-        if (inb && inr) {
-            B[r] = cell<METRIC, EDGE>(i, j, A[r], A[r + 1], B[r], X[r + 2], X[r + 1],
-                                      Y[W - r + 1], Y[W - r],
-                                      METRIC == ERP ? (X[r + 1] - arg) * (X[r + 1] - arg) : 0.0,
-                                      METRIC == ERP ? (Y[W - r] - arg) * (Y[W - r] - arg) : 0.0,
-                                      arg);
-        } else if (EDGE) {
-            B[r] = S;
+        if (EXACT || (delta <= w && delta >= -w)) {
+            B[r] = cell<METRIC, false>(0, 0, A[r], A[r + 1], B[r], X[r + 2], X[r + 1],
+                                       Y[W - r + 1], Y[W - r],
+                                       METRIC == ERP ? (X[r + 1] - arg) * (X[r + 1] - arg) : 0.0,
+                                       METRIC == ERP ? (Y[W - r] - arg) * (Y[W - r] - arg) : 0.0,
+                                       arg);
         }
     }
 }
@@ -190,25 +200,30 @@ __device__ __forceinline__ void ring_shift(double* X, double* Y, double xn, doub
     X[W + 1] = xn;
 }
 
-// Lower bound on the cost of consuming candidate sample y whose band sees the
-// query envelope [lo, hi]; for LCSS, 1 if y can match at all.  Rounding: the
-// cells' costs are at least these values times (1 - 7u) (u = 2^-53), and the
-// LCSS / EDR comparisons are widened by a relative 1e-12, far above the
-// rounding of |x - y| against e.
+// Per-sample lower bound on the cost of consuming candidate sample y whose
+// band sees the query envelope [lo, hi] (DTW, ERP, MSM); for EDR 1 if y is
+// more than eps from the envelope, for LCSS 1 if it is within eps of it.
+// The EDR / LCSS tests are tightened by a relative 1e-12 so that they select
+// a superset (EDR) / subset (LCSS) of the positions the GLB pass counted.
 template <int METRIC>
 __device__ inline double consume_lb(double y, double lo, double hi, double arg) {
     const double e = fmax(fmax(lo - y, y - hi), 0.0);
     if (METRIC == DTW) return e * e;
     if (METRIC == ERP) return fmin(e * e, (y - arg) * (y - arg));
-    if (METRIC == EDR) return (e > arg * (1.0 + 1e-12) + 1e-300) ? 1.0 : 0.0;
+    if (METRIC == EDR) return (e > arg * (1.0 - 1e-12)) ? 1.0 : 0.0;
     if (METRIC == MSM) return fmin(e, arg);
-    return (e <= arg * (1.0 + 1e-12) + 1e-300) ? 1.0 : 0.0;
+    return (e <= arg * (1.0 - 1e-12)) ? 1.0 : 0.0;
 }
 
-// total: sum of consume_lb over all m samples, p: the same sum, in the same
-// order, over samples 0 .. K (K = the frontier's column bound for this check).
-// fl(total - p) is within (m + 1) u total of the exact suffix sum; the path
-// beyond the frontier adds at most 2m + 2 rounded terms.
+// Abandon test after iteration b (the two anti-diagonals in A and B).  total
+// is the GLB pass's certified bound over the candidate's samples: a lower
+// bound on the sum of their consumption costs (DTW / ERP / MSM, EDR counts
+// of certain mismatches) or an upper bound on the samples that can match at
+// all (LCSS); p sums consume_lb over samples 0 .. b + KOFF, an upper bound on
+// the same quantity over them (rounding: the factor (m + 8) 1.2e-16).  The
+// remaining path still consumes every sample beyond the frontier's largest
+// column (LCSS: matches beyond the smallest), and its rounded sum loses at
+// most (2m + 24) 1.2e-16 relative.
 template <int METRIC, int W>
 __device__ __forceinline__ bool ad_alive(const double* A, const double* B, int b, int M, int m,
                                          double thr, double total, double p) {
@@ -227,100 +242,182 @@ __device__ __forceinline__ bool ad_alive(const double* A, const double* B, int b
     for (int k = 1; k <= W; k++) mn = fmin(mn, A[k]);
 #pragma unroll
     for (int k = 0; k < W; k++) mn = fmin(mn, B[k]);
-    const double cb = fmax((total - p) - (double)(m + 8) * 1.2e-16 * total, 0.0);
+    const double cb = fmax(total - p * (1.0 + (double)(m + 8) * 1.2e-16), 0.0);
     return !((mn + cb) * (1.0 - (double)(2 * m + 24) * 1.2e-16) > thr);
 }
 
+// src/ElasticMeasures.cpp: sqrt for DTW / ERP (1e18 when not finite), 1 - L/m
+// for LCSS, the accumulated cost otherwise.
+template <int METRIC>
+__device__ inline double eld_result(double r, int m) {
+    if (METRIC == DTW || METRIC == ERP) {
+        if (!__builtin_isfinite(r)) return kEldSentinel;
+        return sqrt(smax(0.0, r));
+    }
+    if (METRIC == LCSS) return 1 - r / (double)m;
+    return r;
+}
+
+// Append (g, d) for the lanes with keep set: one atomic per wave.
+__device__ inline void wave_append(bool keep, int64_t g, double d, int64_t* out_idx,
+                                   double* out_dist, unsigned long long* out_count) {
+    const uint64_t mask = __ballot(keep);
+    if (!mask) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __builtin_ctzll(mask);
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(out_count, (unsigned long long)__builtin_popcountll(mask));
+    base = __shfl(base, leader, 64);
+    if (keep) {
+        const uint64_t below = lane ? (mask & ((1ull << lane) - 1)) : 0ull;
+        const unsigned long long pos = base + __builtin_popcountll(below);
+        out_idx[pos] = g;
+        out_dist[pos] = d;
+    }
+}
+
+// Persistent register-band kernel.  Each lane runs one candidate at a time
+// through iterations BS .. bend at its own pace; a lane whose candidate ends
+// (finished or abandoned) idles until kRefillMin lanes of its wave are idle,
+// and then they take the next candidates from the shared queue together.
+// in[] lists window indices (target mode, in_total[] the GLB pass's bounds or
+// null) or, with rows != null, rows 0 .. n_direct-1 of normalised samples.
 template <int METRIC, int W, bool EXACT>
-__device__ double eld_ad(const double* xs, const double* xgs, const CandSrc& c, int m, int w,
-                         double arg, double thr, const double* lqs, const double* uqs,
-                         bool active, bool* dead_out) {
+__global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const double* __restrict__ LQ,
+                                             const double* __restrict__ UQ,
+                                             const int64_t* __restrict__ in,
+                                             const double* __restrict__ in_total,
+                                             const unsigned long long* __restrict__ in_count,
+                                             int64_t n_direct, const double* __restrict__ rows,
+                                             int keep_all, int64_t* __restrict__ out_idx,
+                                             double* __restrict__ out_dist,
+                                             unsigned long long* out_count,
+                                             unsigned long long* queue) {
+    extern __shared__ double xlds[];
     constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
-    // after iteration b the ring holds Y[k] = y[b + k]; the frontier's columns
-    // are at most b + W (grid: samples up to b + W - 1 are consumed); LCSS
-    // bounds matches by samples beyond the smallest column, b
+    // samples 0 .. b + KOFF lie at or before the frontier's largest column
+    // (ERP / EDR cells consume y[j - 1]); LCSS: its smallest column
     constexpr int KOFF = (METRIC == LCSS) ? 0 : (GRID1 ? W - 1 : W);
+    // first iteration: anti-diagonals 0 and 1 (W even) or -1 and 0 (W odd)
+    constexpr int BS = -((W + 1) / 2);
+    const int64_t n = in_count ? (int64_t)*in_count : n_direct;
+    const int m = a.p.m, w = a.p.w;
     const int M = GRID1 ? m : m - 1;
-    const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;
-    double A[W + 1], B[W + 1];
-    double X[W + 2], Y[W + 2];
-    const int b0 = -((W + 1) / 2) - 1;
     const int bend = (2 * M - W) >= 0 ? (2 * M - W) / 2 : -((W - 2 * M + 1) / 2);
-    // interior iterations [1, M - W]: every cell in range with i, j >= 1
-    const int bi0 = 1;
-    const int bi1 = M - W;
-#pragma unroll
-    for (int k = 0; k <= W; k++) {
-        A[k] = S;
-        B[k] = S;
+    const double arg = a.p.arg, cut = a.p.cutoff;
+    double thr = 1e308;
+    // abandoning needs non-negative step costs (MSM: c >= 0)
+    if (!keep_all && !(METRIC == MSM && arg < 0.0)) {
+        if (METRIC == DTW || METRIC == ERP) thr = cut * cut * (1.0 + 1e-12) + 1e-300;
+        else thr = cut * (1.0 + 1e-12) + 1e-300;
     }
-#pragma unroll
-    for (int k = 0; k < W + 2; k++) {
-        Y[k] = c.at(b0 - 1 + k);
-        X[k] = xs[kXPad + b0 - 1 + k];
-    }
-    bool alive = active;
     const bool check = thr < 1e300;
-    // p starts as the prefix of total through sample b0 + KOFF (the first
-    // iteration's bound) and then follows the same summation order
-    double total = 0.0, p = 0.0;
-    if (check) {
-        constexpr int U = 8;
-        int k = 0;
-        for (; k + U <= m; k += U) {
-            double v[U];
+    const bool raw = rows == nullptr;
+    const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;
+    const double V = (METRIC == EDR && !(arg >= 0.0)) ? -1.0 : 0.0;
+
+    const int span = m + 2 * kXPad;
+    double* xs = xlds + kXPad;
+    double* lqs = xlds + span + kXPad;
+    double* uqs = lqs + span;
+    for (int i = threadIdx.x; i < span; i += blockDim.x) {
+        const int qi = i - kXPad;
+        const bool in_q = qi >= 0 && qi < m;
+        xlds[i] = in_q ? a.q.qn[qi] : kPad;
+        if (check) {
+            xlds[span + i] = in_q ? LQ[qi] : 0.0;
+            xlds[2 * span + i] = in_q ? UQ[qi] : 0.0;
+        }
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    double A[W + 1], B[W + 1], X[W + 2], Y[W + 2];
 #pragma unroll
-            for (int u = 0; u < U; u++) v[u] = c.at(k + u);
+    for (int k = 0; k <= W; k++) A[k] = B[k] = S;
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                total += consume_lb<METRIC>(v[u], lqs[kXPad + k + u], uqs[kXPad + k + u], arg);
-                p = (k + u == b0 + KOFF) ? total : p;
+    for (int k = 0; k < W + 2; k++) X[k] = Y[k] = 0.0;
+    const double* tp = raw ? a.T : rows;
+    double mu = 0.0, sd = 1.0, total = 0.0, p = 0.0;
+    int64_t ci = 0, g = 0;
+    int b = 0, it = 0;
+    bool busy = false, more = true;
+    while (true) {
+        const uint64_t idle = __ballot(!busy);
+        const int nidle = __builtin_popcountll(idle);
+        if (more && (nidle >= kRefillMin || nidle == 64)) {
+            const int leader = __builtin_ctzll(idle);
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(queue, (unsigned long long)nidle);
+            base = __shfl(base, leader, 64);
+            if (base + (unsigned long long)nidle >= (unsigned long long)n) more = false;
+            if (!busy) {
+                const int rank = __builtin_popcountll(idle & ((1ull << lane) - 1));
+                const int64_t c = (int64_t)base + rank;
+                if (c < n) {
+                    ci = c;
+                    if (raw) {
+                        g = in[c];
+                        const int64_t si = stat_index(a.G, g);
+                        tp = a.T + g;
+                        mu = a.MU[si];
+                        sd = a.SG[si];
+                    } else {
+                        g = c;
+                        tp = rows + c * (int64_t)m;
+                    }
+                    total = in_total ? in_total[c] : ((METRIC == LCSS) ? (double)m : 0.0);
+                    b = BS;
+#pragma unroll
+                    for (int k = 0; k <= W; k++) A[k] = B[k] = S;
+                    if (W & 1) B[(W - 1) / 2] = V;
+                    else A[W / 2] = V;
+#pragma unroll
+                    for (int k = 0; k < W + 2; k++) {
+                        const int j = BS - 1 + k;
+                        double v = kPad;
+                        if (j >= 0 && j < m) v = raw ? (tp[j] - mu) / sd : tp[j];
+                        Y[k] = v;
+                        X[k] = xs[j];
+                    }
+                    p = 0.0;
+                    if (check) {
+#pragma unroll
+                        for (int s = 0; s <= BS - 1 + KOFF; s++)
+                            if (s < m) p += consume_lb<METRIC>(Y[s - BS + 1], lqs[s], uqs[s], arg);
+                    }
+                    busy = true;
+                }
             }
         }
-        for (; k < m; k++) {
-            total += consume_lb<METRIC>(c.at(k), lqs[kXPad + k], uqs[kXPad + k], arg);
-            p = (k == b0 + KOFF) ? total : p;
-        }
-    }
-    int b = b0;
-    // head
-    for (; b <= bend && b < bi0; b++) {
-        ad_step<METRIC, W, EXACT, true>(b, M, w, arg, A, B, X, Y);
-        ring_shift<W>(X, Y, xs[kXPad + b + W + 1], c.at(b + W + 1));
-        const int K = b + KOFF;
-        if (check && b > b0 && K >= 0 && K < m)
-            p += consume_lb<METRIC>(Y[KOFF], lqs[kXPad + K], uqs[kXPad + K], arg);
-    }
-    // interior, early abandon every 4 iterations
-    int it = 0;
-    for (; b <= bi1; b++, it++) {
-        ad_step<METRIC, W, EXACT, false>(b, M, w, arg, A, B, X, Y);
-        ring_shift<W>(X, Y, xs[kXPad + b + W + 1], c.at(b + W + 1));
+        if (!__any(busy)) break;
+
+        ad_step<METRIC, W, EXACT>(w, arg, A, B, X, Y);
+        const int jn = b + W + 1;
+        const int jj = min(max(jn, 0), m - 1);
+        double yv = tp[jj];
+        if (raw) yv = (yv - mu) / sd;
+        ring_shift<W>(X, Y, xs[jn], (jn == jj) ? yv : kPad);
         if (check) {
             const int K = b + KOFF;
-            if (K < m) p += consume_lb<METRIC>(Y[KOFF], lqs[kXPad + K], uqs[kXPad + K], arg);
-            if ((it & 3) == 3) {
-                alive = alive && ad_alive<METRIC, W>(A, B, b, M, m, thr, total, p);
-                if (!__any(alive)) break;
+            if (K >= 0 && K < m) p += consume_lb<METRIC>(Y[KOFF], lqs[K], uqs[K], arg);
+        }
+        const bool fin = busy && b == bend;
+        if (busy) b++;
+        if (__any(fin)) {
+            const double d = eld_result<METRIC>((W & 1) ? B[(W - 1) / 2] : A[W / 2], m);
+            if (keep_all) {
+                if (fin) out_dist[ci] = d;
+            } else {
+                // src/fsmeld.cpp:846-847
+                wave_append(fin && __builtin_isfinite(d) && d <= cut, g, d, out_idx, out_dist,
+                            out_count);
             }
+            busy = busy && !fin;
         }
+        if (check && ((++it) & 3) == 0)
+            busy = busy && ad_alive<METRIC, W>(A, B, b - 1, M, m, thr, total, p);
     }
-    double result = S;
-    if (__any(alive) || !check) {
-        // tail (the final cell (M, M) sits in slot W of iteration bend)
-        for (b = (b > bi1 ? b : bi1 + 1); b <= bend; b++) {
-            ad_step<METRIC, W, EXACT, true>(b, M, w, arg, A, B, X, Y);
-            ring_shift<W>(X, Y, xs[kXPad + b + W + 1], c.at(b + W + 1));
-        }
-        result = (W & 1) ? B[(W - 1) / 2] : A[W / 2];
-    }
-    *dead_out = !alive;
-    if (METRIC == DTW || METRIC == ERP) {
-        if (!__builtin_isfinite(result)) return S;
-        return sqrt(smax(0.0, result));
-    }
-    if (METRIC == LCSS) return 1 - result / (double)m;
-    return result;
 }
 
 // ---------------------------------------------------------------- generic ---
@@ -377,51 +474,23 @@ __device__ double eld_rows(const double* __restrict__ q, const double* __restric
     return res;
 }
 
-// ----------------------------------------------------------------- kernel ---
-// in[] lists window indices (target mode) or, with rows != null, row numbers
-// 0..n_direct-1 of normalised candidates.  W == 0: generic row-band path.
-template <int METRIC, int W, bool EXACT>
-__global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const double* __restrict__ LQ,
-                                             const double* __restrict__ UQ,
-                                             const int64_t* __restrict__ in,
-                                             const unsigned long long* __restrict__ in_count,
-                                             int64_t n_direct, const double* __restrict__ rows,
-                                             int keep_all, int64_t* __restrict__ out_idx,
-                                             double* __restrict__ out_dist,
-                                             unsigned long long* out_count,
-                                             double* __restrict__ scratch) {
-    extern __shared__ double xlds[];
+// Grid-stride kernel of the row-band path (radii above the register
+// variants, or queries too long for their LDS staging).
+template <int METRIC>
+__global__ void __launch_bounds__(256) k_eld_rows(VerifyArgs a, const int64_t* __restrict__ in,
+                                                  const unsigned long long* __restrict__ in_count,
+                                                  int64_t n_direct,
+                                                  const double* __restrict__ rows, int keep_all,
+                                                  int64_t* __restrict__ out_idx,
+                                                  double* __restrict__ out_dist,
+                                                  unsigned long long* out_count,
+                                                  double* __restrict__ scratch) {
     const int64_t n = in_count ? (int64_t)*in_count : n_direct;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t iters = (n + stride - 1) / stride;
     const int m = a.p.m, w = a.p.w;
     const double cut = a.p.cutoff;
-    double thr = 1e308;
-    // abandoning needs non-negative step costs (MSM: c >= 0)
-    if (!keep_all && !(METRIC == MSM && a.p.arg < 0.0)) {
-        if (METRIC == DTW || METRIC == ERP) thr = cut * cut * (1.0 + 1e-12) + 1e-300;
-        else thr = cut * (1.0 + 1e-12) + 1e-300;
-    }
-    const int span = m + 2 * kXPad;
-    // LDS: query, [ERP: gap costs], query envelope lower, upper (the last two
-    // only when abandoning)
-    constexpr int NX = (METRIC == ERP) ? 2 : 1;
-    double* lqs = xlds + NX * span;
-    double* uqs = lqs + span;
-    if (W > 0) {
-        for (int i = threadIdx.x; i < span; i += blockDim.x) {
-            const int qi = i - kXPad;
-            const bool in_q = qi >= 0 && qi < m;
-            xlds[i] = in_q ? a.q.qn[qi] : 0.0;
-            if (METRIC == ERP) xlds[span + i] = in_q ? a.q.XG[qi] : 0.0;
-            if (!keep_all) {
-                lqs[i] = in_q ? LQ[qi] : 0.0;
-                uqs[i] = in_q ? UQ[qi] : 0.0;
-            }
-        }
-        __syncthreads();
-    }
     int64_t ci = tid;
     for (int64_t itx = 0; itx < iters; itx++, ci += stride) {
         const bool have = ci < n;
@@ -442,102 +511,95 @@ __global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const double* __restr
             c.sd = a.SG[si];
             c.raw = true;
         }
-        bool dead = false;
-        double d;
-        if constexpr (W > 0) {
-            d = eld_ad<METRIC, W, EXACT>(xlds, xlds + span, c, m, w, a.p.arg, thr, lqs, uqs, have,
-                                         &dead);
-        } else {
-            MemArr b{scratch + tid, stride * 2};
-            MemArr y{scratch + tid + stride, stride * 2};
-            d = eld_rows<METRIC>(a.q.qn, a.q.XG, c, m, w, a.p.arg, b, y);
-        }
+        MemArr bb{scratch + tid, stride * 2};
+        MemArr yy{scratch + tid + stride, stride * 2};
+        const double d = eld_rows<METRIC>(a.q.qn, a.q.XG, c, m, w, a.p.arg, bb, yy);
         if (keep_all) {
             if (have) out_dist[ci] = d;
             continue;
         }
-        const bool keep = have && !dead && __builtin_isfinite(d) && d <= cut;  // src/fsmeld.cpp:846-847
-        const uint64_t mask = __ballot(keep);
-        if (mask) {
-            const int lane = threadIdx.x & 63;
-            const int leader = __builtin_ctzll(mask);
-            unsigned long long base = 0;
-            if (lane == leader)
-                base = atomicAdd(out_count, (unsigned long long)__builtin_popcountll(mask));
-            base = __shfl(base, leader, 64);
-            if (keep) {
-                const uint64_t below = lane ? (mask & ((1ull << lane) - 1)) : 0ull;
-                const unsigned long long pos = base + __builtin_popcountll(below);
-                out_idx[pos] = g;
-                out_dist[pos] = d;
-            }
-        }
+        wave_append(have && __builtin_isfinite(d) && d <= cut, g, d, out_idx, out_dist, out_count);
     }
 }
 
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
 int64_t eld_scratch_threads() { return (int64_t)512 * 256; }
 
+static size_t reg_lds_bytes(int m) { return sizeof(double) * (size_t)(m + 2 * kXPad) * 3; }
+
 bool eld_uses_scratch(int metric, int m, int w) {
-    const size_t lds = sizeof(double) * (size_t)(m + 2 * kXPad) * (metric == ERP ? 4 : 3);
-    if (lds > kEldLdsCap) return true;
+    if (reg_lds_bytes(m) > kEldLdsCap) return true;
     return w > 32 || (w > 25 && metric == ERP);
 }
 
 template <int METRIC>
 static hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64_t* in,
-                                const unsigned long long* in_count, int64_t n_direct,
-                                const double* rows, bool keep_all, int64_t* out_idx,
-                                double* out_dist, unsigned long long* out_count,
+                                const double* in_total, const unsigned long long* in_count,
+                                int64_t n_direct, const double* rows, bool keep_all,
+                                int64_t* out_idx, double* out_dist,
+                                unsigned long long* out_count, unsigned long long* queue,
                                 double* scratch, int64_t scratch_threads) {
     const int w = a.p.w;
-    const dim3 blk(256), grid(2048);
+    const dim3 blk(256);
     const int ka = keep_all ? 1 : 0;
-    // register variants stage query, [ERP gap costs,] and envelope in LDS
-    const size_t lds = sizeof(double) * (size_t)(a.p.m + 2 * kXPad) * (METRIC == ERP ? 4 : 3);
-    const bool fits = lds <= kEldLdsCap;
-#define FSM_ELD(WW, EX) \
-    hipLaunchKernelGGL((k_eld<METRIC, WW, EX>), grid, blk, lds, st, a, a.q.LQ, a.q.UQ, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch)
-    if (fits && w == 6) FSM_ELD(6, true);
-    else if (fits && w == 25) FSM_ELD(25, true);
-    else if (fits && w <= 8) FSM_ELD(8, false);
-    else if (fits && w <= 16) FSM_ELD(16, false);
-    else if (fits && w <= 25) FSM_ELD(25, false);
-    else if (fits && w <= 32 && METRIC != ERP) FSM_ELD(32, false);
-    else {
+    if (eld_uses_scratch(METRIC, a.p.m, w)) {
         if (!scratch) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_eld<METRIC, 0, false>), dim3((unsigned)(scratch_threads / 256)), blk, 0, st, a, a.q.LQ, a.q.UQ, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, scratch);
+        hipLaunchKernelGGL((k_eld_rows<METRIC>), dim3((unsigned)(scratch_threads / 256)), blk, 0, st,
+                           a, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count,
+                           scratch);
+        return hipGetLastError();
     }
+    if (!queue) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, queue);
+    // persistent: enough waves to fill every SIMD; later blocks find the
+    // queue drained and exit
+    const dim3 grid(1024);
+    const size_t lds = reg_lds_bytes(a.p.m);
+#define FSM_ELD(WW, EX) \
+    hipLaunchKernelGGL((k_eld<METRIC, WW, EX>), grid, blk, lds, st, a, a.q.LQ, a.q.UQ, in, in_total, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, queue)
+    if (w == 6) FSM_ELD(6, true);
+    else if (w == 25) FSM_ELD(25, true);
+    else if (w <= 8) FSM_ELD(8, false);
+    else if (w <= 16) FSM_ELD(16, false);
+    else if (w <= 25) FSM_ELD(25, false);
+    else FSM_ELD(32, false);
 #undef FSM_ELD
     return hipGetLastError();
 }
 
 static hipError_t dispatch_eld(hipStream_t st, const VerifyArgs& a, const int64_t* in,
-                               const unsigned long long* in_count, int64_t n_direct,
-                               const double* rows, bool keep_all, int64_t* out_idx,
-                               double* out_dist, unsigned long long* out_count,
-                               double* scratch, int64_t scratch_threads) {
+                               const double* in_total, const unsigned long long* in_count,
+                               int64_t n_direct, const double* rows, bool keep_all,
+                               int64_t* out_idx, double* out_dist, unsigned long long* out_count,
+                               unsigned long long* queue, double* scratch,
+                               int64_t scratch_threads) {
+#define FSM_M(MM) \
+    launch_metric<MM>(st, a, in, in_total, in_count, n_direct, rows, keep_all, out_idx, out_dist, out_count, queue, scratch, scratch_threads)
     switch (a.p.metric) {
-        case DTW: return launch_metric<DTW>(st, a, in, in_count, n_direct, rows, keep_all, out_idx, out_dist, out_count, scratch, scratch_threads);
-        case ERP: return launch_metric<ERP>(st, a, in, in_count, n_direct, rows, keep_all, out_idx, out_dist, out_count, scratch, scratch_threads);
-        case EDR: return launch_metric<EDR>(st, a, in, in_count, n_direct, rows, keep_all, out_idx, out_dist, out_count, scratch, scratch_threads);
-        case LCSS: return launch_metric<LCSS>(st, a, in, in_count, n_direct, rows, keep_all, out_idx, out_dist, out_count, scratch, scratch_threads);
-        case MSM: return launch_metric<MSM>(st, a, in, in_count, n_direct, rows, keep_all, out_idx, out_dist, out_count, scratch, scratch_threads);
+        case DTW: return FSM_M(DTW);
+        case ERP: return FSM_M(ERP);
+        case EDR: return FSM_M(EDR);
+        case LCSS: return FSM_M(LCSS);
+        case MSM: return FSM_M(MSM);
     }
+#undef FSM_M
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_eld(hipStream_t st, const VerifyArgs& a, const int64_t* in,
-                      const unsigned long long* in_count, bool keep_all, int64_t* out_idx,
-                      double* out_dist, unsigned long long* out_count, double* scratch,
-                      int64_t scratch_threads) {
-    return dispatch_eld(st, a, in, in_count, 0, nullptr, keep_all, out_idx, out_dist,
-                        out_count, scratch, scratch_threads);
+                      const double* in_total, const unsigned long long* in_count, bool keep_all,
+                      int64_t* out_idx, double* out_dist, unsigned long long* out_count,
+                      unsigned long long* queue, double* scratch, int64_t scratch_threads) {
+    return dispatch_eld(st, a, in, in_total, in_count, 0, nullptr, keep_all, out_idx, out_dist,
+                        out_count, queue, scratch, scratch_threads);
 }
 
 hipError_t launch_eld_pairs(hipStream_t st, const VerifyArgs& a, const double* cands, int64_t n,
-                            double* out, double* scratch, int64_t scratch_threads) {
-    return dispatch_eld(st, a, nullptr, nullptr, n, cands, true, nullptr, out, nullptr, scratch,
-                        scratch_threads);
+                            double* out, unsigned long long* queue, double* scratch,
+                            int64_t scratch_threads) {
+    return dispatch_eld(st, a, nullptr, nullptr, nullptr, n, cands, true, nullptr, out, nullptr,
+                        queue, scratch, scratch_threads);
 }
 
 }  // namespace fsm

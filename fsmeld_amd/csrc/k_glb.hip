@@ -298,12 +298,16 @@ __device__ inline void fast_term(double e, double xg2, double arg, double eta, d
 // uniform and the loops end early only when every lane has been pruned.
 // Query arrays reach the kernel as separate read-only, non-aliasing pointers
 // so that their wave-uniform reads become scalar loads.
+// On keep, *total receives the pass-1 bound the ELD abandon test uses: a
+// lower bound on the real-valued sum over samples 1 .. n-2 of the per-sample
+// envelope terms (DTW / ERP / MSM), the count of certain mismatches (EDR),
+// or an upper bound on the samples that can match at all (LCSS).
 template <int METRIC>
 __device__ int glb_fast(const VerifyArgs& A, const double* __restrict__ Q,
                         const double* __restrict__ LQ, const double* __restrict__ UQ,
                         const double* __restrict__ XG, const double* __restrict__ t,
                         const double* __restrict__ Lr, const double* __restrict__ Ur,
-                        double mu, double sd, bool active) {
+                        double mu, double sd, bool active, double* total) {
     constexpr bool COUNT = (METRIC == EDR || METRIC == LCSS);
     const int n = A.p.m, w = A.p.w;
     const double arg = A.p.arg, bsf = A.p.cutoff;
@@ -442,6 +446,11 @@ __device__ int glb_fast(const VerifyArgs& A, const double* __restrict__ Q,
         Qlo = fmax(s2 - e2, 0.0);
         Qhi = s2 + e2;
     }
+    // pass 1's terms are the reference's rounded terms; the real-valued sum
+    // is lower by at most psi relative
+    if (METRIC == LCSS) *total = Thi;
+    else if (METRIC == EDR) *total = Tlo;
+    else *total = Tlo * (1.0 - (double)(n + 8) * 1.2e-16);
     if (METRIC == LCSS) {
         if (glb_keeps<METRIC>(B, Tlo, Qlo, n, bsf)) return 1;
         if (!glb_keeps<METRIC>(B, Thi, Qhi, n, bsf)) return -1;
@@ -461,6 +470,7 @@ __global__ void __launch_bounds__(256) k_glb(VerifyArgs a, const double* __restr
                                              const unsigned long long* __restrict__ in_count,
                                              int64_t* __restrict__ out,
                                              unsigned long long* out_count,
+                                             double* __restrict__ out_total,
                                              double* __restrict__ vals) {
     __shared__ unsigned long long sh[256 / 64 + 1];
     const int64_t n = (int64_t)*in_count;
@@ -479,9 +489,11 @@ __global__ void __launch_bounds__(256) k_glb(VerifyArgs a, const double* __restr
         const double mu = active ? a.MU[si] : 0.0;
         const double sd = active ? a.SG[si] : 1.0;
         int verdict = 0;
+        // without the fast pass's bound: nothing (distances), every sample (LCSS)
+        double total = (METRIC == LCSS) ? (double)a.p.m : 0.0;
         if (fast)
             verdict = glb_fast<METRIC>(a, Q, LQ, UQ, XG, a.T + g, a.Lraw + g, a.Uraw + g, mu, sd,
-                                       active);
+                                       active, &total);
         bool keep = verdict > 0;
         if (active && verdict == 0) {
             const double v = glb_one<METRIC>(a.T + g, mu, sd, Q, LQ, UQ, a.Lraw + g, a.Uraw + g,
@@ -490,7 +502,7 @@ __global__ void __launch_bounds__(256) k_glb(VerifyArgs a, const double* __restr
             keep = v <= a.p.cutoff;  // src/fsmeld.cpp:831
         }
         keep = keep && active;
-        if (out) block_append(keep, g, 0.0, out, nullptr, out_count, sh);
+        if (out) block_append(keep, g, total, out, out_total, out_count, sh);
     }
 }
 
@@ -498,11 +510,11 @@ static int verify_blocks() { return 2048; }
 
 hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
                       const unsigned long long* in_count, int64_t* out,
-                      unsigned long long* out_count, double* vals) {
+                      unsigned long long* out_count, double* out_total, double* vals) {
     const dim3 g(verify_blocks()), b(256);
     const QueryTables& q = a.q;
 #define FSM_GLB(MM) \
-    hipLaunchKernelGGL(k_glb<MM>, g, b, 0, st, a, q.qn, q.LQ, q.UQ, q.XG, in, in_count, out, out_count, vals)
+    hipLaunchKernelGGL(k_glb<MM>, g, b, 0, st, a, q.qn, q.LQ, q.UQ, q.XG, in, in_count, out, out_count, out_total, vals)
     switch (a.p.metric) {
         case DTW: FSM_GLB(DTW); break;
         case ERP: FSM_GLB(ERP); break;
