@@ -56,6 +56,23 @@ __device__ inline double msm_cost(double np, double x, double y, double c) {
     return c + fmax(fmax(np - hi, lo - np), 0.0);
 }
 
+// MSM cell of the register band from e = x_i - y_j, dx = x_i - x_{i-1} and
+// dy = y_j - y_{j-1}.  Rounded subtraction is monotone and odd, so
+// fl(np - max(a, b)) = min(fl(np - a), fl(np - b)) and
+// fl(min(a, b) - np) = -max(fl(np - a), fl(np - b)): msm_cost(x_i, x_{i-1}, y_j)
+// = c + max(min(dx, e), -max(dx, e), 0) and msm_cost(y_j, x_i, y_{j-1})
+// = c + max(min(-e, dy), -max(-e, dy), 0), bit for bit, sharing e with the
+// match cost |x_i - y_j|.
+__device__ inline double msm_cell(double up, double left, double diag, double e, double dx,
+                                  double dy, double c) {
+    const double c1 = diag + fabs(e);
+    const double s2 = fmax(fmax(fmin(dx, e), -fmax(dx, e)), 0.0);
+    const double s3 = fmax(fmax(fmin(-e, dy), -fmax(-e, dy)), 0.0);
+    const double c2 = up + (c + s2);
+    const double c3 = left + (c + s3);
+    return dmin3(c1, c2, c3);
+}
+
 // Candidate sample source: raw target samples normalised on the fly, or rows
 // that are already normalised.
 struct CandSrc {
@@ -159,33 +176,54 @@ constexpr int kRefillMin = 8;  // idle lanes that trigger a wave's refill
 // and dB = dA + 1 (odd slots 2r+1, cells (b + 1 + r, b + W - r)); the rings
 // hold X[k] = x[b - 1 + k], Y[k] = y[b - 1 + k].  A cell reads x[i] = X[r+1]
 // (x[i-1] = X[r]) and y[j] = Y[j - b + 1] (y[j-1] = Y[j - b]).
+// Slot update of the even anti-diagonal (A[r], cell (b + r, b + W - r)).
+template <int METRIC, int W, bool EXACT>
+__device__ __forceinline__ void step_a(int r, int w, double arg, double* A, const double* B,
+                                       const double* X, const double* Y) {
+    const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;  // out-of-band neighbour
+    const int delta = 2 * r - W;
+    if (!(EXACT || (delta <= w && delta >= -w))) return;
+    const double up = (r >= 1) ? B[r - 1] : S;
+    const double left = (r <= W - 1) ? B[r] : S;
+    if (METRIC == MSM) {
+        A[r] = msm_cell(up, left, A[r], X[r + 1] - Y[W - r + 1], X[r + 1] - X[r],
+                        Y[W - r + 1] - Y[W - r], arg);
+        return;
+    }
+    A[r] = cell<METRIC, false>(0, 0, up, left, A[r], X[r + 1], X[r], Y[W - r + 1], Y[W - r],
+                               METRIC == ERP ? (X[r] - arg) * (X[r] - arg) : 0.0,
+                               METRIC == ERP ? (Y[W - r] - arg) * (Y[W - r] - arg) : 0.0, arg);
+}
+
+// Slot update of the odd anti-diagonal (B[r], cell (b + 1 + r, b + W - r)).
+template <int METRIC, int W, bool EXACT>
+__device__ __forceinline__ void step_b(int r, int w, double arg, const double* A, double* B,
+                                       const double* X, const double* Y) {
+    const int delta = 2 * r + 1 - W;
+    if (!(EXACT || (delta <= w && delta >= -w))) return;
+    if (METRIC == MSM) {
+        B[r] = msm_cell(A[r], A[r + 1], B[r], X[r + 2] - Y[W - r + 1], X[r + 2] - X[r + 1],
+                        Y[W - r + 1] - Y[W - r], arg);
+        return;
+    }
+    B[r] = cell<METRIC, false>(0, 0, A[r], A[r + 1], B[r], X[r + 2], X[r + 1], Y[W - r + 1],
+                               Y[W - r],
+                               METRIC == ERP ? (X[r + 1] - arg) * (X[r + 1] - arg) : 0.0,
+                               METRIC == ERP ? (Y[W - r] - arg) * (Y[W - r] - arg) : 0.0, arg);
+}
+
+// One iteration (two anti-diagonals).  B[r] needs the new A[r] and A[r + 1]
+// and A[r + 1] needs the old B[r], so the halves interleave slot by slot,
+// which keeps the values the two cells share (row / column differences)
+// live only briefly.
 template <int METRIC, int W, bool EXACT>
 __device__ __forceinline__ void ad_step(int w, double arg, double* A, double* B, const double* X,
                                         const double* Y) {
-    const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;  // out-of-band neighbour
+    step_a<METRIC, W, EXACT>(0, w, arg, A, B, X, Y);
 #pragma unroll
-    for (int r = 0; r <= W; r++) {
-        const int delta = 2 * r - W;
-        if (EXACT || (delta <= w && delta >= -w)) {
-            const double up = (r >= 1) ? B[r - 1] : S;
-            const double left = (r <= W - 1) ? B[r] : S;
-            A[r] = cell<METRIC, false>(0, 0, up, left, A[r], X[r + 1], X[r], Y[W - r + 1],
-                                       Y[W - r],
-                                       METRIC == ERP ? (X[r] - arg) * (X[r] - arg) : 0.0,
-                                       METRIC == ERP ? (Y[W - r] - arg) * (Y[W - r] - arg) : 0.0,
-                                       arg);
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < W; r++) {
-        const int delta = 2 * r + 1 - W;
-        if (EXACT || (delta <= w && delta >= -w)) {
-            B[r] = cell<METRIC, false>(0, 0, A[r], A[r + 1], B[r], X[r + 2], X[r + 1],
-                                       Y[W - r + 1], Y[W - r],
-                                       METRIC == ERP ? (X[r + 1] - arg) * (X[r + 1] - arg) : 0.0,
-                                       METRIC == ERP ? (Y[W - r] - arg) * (Y[W - r] - arg) : 0.0,
-                                       arg);
-        }
+    for (int r = 1; r <= W; r++) {
+        step_a<METRIC, W, EXACT>(r, w, arg, A, B, X, Y);
+        step_b<METRIC, W, EXACT>(r - 1, w, arg, A, B, X, Y);
     }
 }
 
