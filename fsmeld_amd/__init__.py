@@ -62,7 +62,8 @@ class _Stats(ctypes.Structure):
                 ("k_m_l", ctypes.c_int64), ("forsize", ctypes.c_int64), ("tail", ctypes.c_int64),
                 ("m_k", ctypes.c_int), ("ms_stats", ctypes.c_double),
                 ("ms_lbip", ctypes.c_double), ("ms_glb", ctypes.c_double),
-                ("ms_eld", ctypes.c_double), ("ms_total", ctypes.c_double)]
+                ("ms_eld", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("eld_cells", ctypes.c_uint64)]
 
 
 _D = ctypes.POINTER(ctypes.c_double)

@@ -340,12 +340,12 @@ QueryParams params_of(const fsm_query_desc* d) {
 }
 
 // counters: [0] compacted, [1] valid, [2] after GLB, [3] ELD kept, [8] ELD work
-// queue.  The LB buffer is free once compaction has read it; GLB writes each
+// queue, [9] ELD lane-iterations on live candidates.  The LB buffer is free once compaction has read it; GLB writes each
 // survivor's consumption bound there for the ELD abandon test.
 int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
                       const QueryParams& P, bool use_lb, bool use_glb) {
     unsigned long long* C = t->counters.as<unsigned long long>();
-    HIP_TRY(hipMemsetAsync(C, 0, sizeof(unsigned long long) * 8, t->stream));
+    HIP_TRY(hipMemsetAsync(C, 0, sizeof(unsigned long long) * 16, t->stream));
     HIP_TRY(t->list_a.ensure(sizeof(int64_t) * (size_t)G.ts_n));
     HIP_TRY(t->list_b.ensure(sizeof(int64_t) * (size_t)G.ts_n));
     HIP_TRY(t->out_idx.ensure(sizeof(int64_t) * (size_t)G.ts_n));
@@ -370,7 +370,7 @@ int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
     int rc = ensure_eld_scratch(t, P.metric, P.m, P.w);
     if (rc) return rc;
     HIP_TRY(launch_eld(t->stream, v, eld_in, eld_total, eld_cnt, false, t->out_idx.as<int64_t>(),
-                       t->out_dist.as<double>(), C + 3, C + 8, t->scratch.as<double>(),
+                       t->out_dist.as<double>(), C + 3, C + 8, C + 9, t->scratch.as<double>(),
                        eld_scratch_threads()));
     HIP_TRY(hipEventRecord(t->ev[5], t->stream));
     return FSM_OK;
@@ -469,7 +469,7 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
     st.tail = G.tail;
     st.m_k = G.m_k;
     auto bail = [&](int code) { return code; };
-    unsigned long long C[8] = {};
+    unsigned long long C[16] = {};
 
     HIP_TRY(hipEventRecord(t->ev[0], t->stream));
     bool stats_new = false;
@@ -530,7 +530,7 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
                 VerifyArgs v = verify_args(t, G, Q, P);
                 HIP_TRY(launch_eld(t->stream, v, t->list_b.as<int64_t>(), nullptr, Cd + 4, true,
                                    nullptr, t->out_dist.as<double>(), nullptr, Cd + 8,
-                                   t->scratch.as<double>(), eld_scratch_threads()));
+                                   nullptr, t->scratch.as<double>(), eld_scratch_threads()));
                 std::vector<double> rd(rl.size());
                 HIP_TRY(hipMemcpyAsync(rd.data(), t->out_dist.p, sizeof(double) * rd.size(),
                                        hipMemcpyDeviceToHost, t->stream));
@@ -549,7 +549,7 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
         rc = run_filter_verify(t, G, Q, P, true, true);
         if (rc) return bail(rc);
         HIP_TRY(hipStreamSynchronize(t->stream));
-        unsigned long long C2[8] = {};
+        unsigned long long C2[16] = {};
         HIP_TRY(hipMemcpy(C2, t->counters.p, sizeof(C2), hipMemcpyDeviceToHost));
         std::memcpy(C, C2, sizeof(C));
         st.valid = C[1];
@@ -595,6 +595,7 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
         st.threshold = R->dist.back();
     }
     st.matches = R->idx.size();
+    st.eld_cells = (uint64_t)C[9] * (uint64_t)(2 * d->w + 1);
     st.ms_stats = stats_new ? ms_between(t->ev[0], t->ev[1]) : 0.0;
     st.ms_lbip = d->mode == FSM_MODE_EXACT ? 0.0 : ms_between(t->ev[1], t->ev[2]);
     if (d->mode == FSM_MODE_EXACT) {
@@ -708,8 +709,8 @@ static int stage_verify(fsm_target* t, const fsm_query_desc* d, const double* q,
                            Cd + 7, nullptr, nullptr));
     else
         HIP_TRY(launch_eld(t->stream, v, t->list_a.as<int64_t>(), nullptr, Cd + 6, true, nullptr,
-                           t->out_dist.as<double>(), nullptr, Cd + 8, t->scratch.as<double>(),
-                           eld_scratch_threads()));
+                           t->out_dist.as<double>(), nullptr, Cd + 8, nullptr,
+                           t->scratch.as<double>(), eld_scratch_threads()));
     HIP_TRY(hipStreamSynchronize(t->stream));
     if (kind != STAGE_GLB_KEEP) {
         HIP_TRY(hipMemcpy(out, t->out_dist.p, sizeof(double) * n, hipMemcpyDeviceToHost));

@@ -156,10 +156,13 @@ hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
 // or null).  keep_all: write dist[i] for every input; else append (index,
 // distance) for distances <= P.cutoff.  queue: one device counter the launch
 // resets and uses to hand out candidates.
+// work (or null) += lane-iterations spent on live candidates by the
+// register-band kernel.
 hipError_t launch_eld(hipStream_t st, const VerifyArgs& a, const int64_t* in,
                       const double* in_total, const unsigned long long* in_count, bool keep_all,
                       int64_t* out_idx, double* out_dist, unsigned long long* out_count,
-                      unsigned long long* queue, double* scratch, int64_t scratch_threads);
+                      unsigned long long* queue, unsigned long long* work, double* scratch,
+                      int64_t scratch_threads);
 // Exact distances of normalised rows cands[n][m] (no target); a.q.qn / a.q.XG
 // and a.p.{metric, m, w, arg} describe the query.
 hipError_t launch_eld_pairs(hipStream_t st, const VerifyArgs& a, const double* cands,

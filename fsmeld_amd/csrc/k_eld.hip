@@ -330,7 +330,8 @@ __global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const double* __restr
                                              int keep_all, int64_t* __restrict__ out_idx,
                                              double* __restrict__ out_dist,
                                              unsigned long long* out_count,
-                                             unsigned long long* queue) {
+                                             unsigned long long* queue,
+                                             unsigned long long* work) {
     extern __shared__ double xlds[];
     constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
     // samples 0 .. b + KOFF lie at or before the frontier's largest column
@@ -379,6 +380,7 @@ __global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const double* __restr
     double mu = 0.0, sd = 1.0, total = 0.0, p = 0.0;
     int64_t ci = 0, g = 0;
     int b = 0, it = 0;
+    unsigned live_iters = 0;
     bool busy = false, more = true;
     while (true) {
         const uint64_t idle = __ballot(!busy);
@@ -441,6 +443,7 @@ __global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const double* __restr
             if (K >= 0 && K < m) p += consume_lb<METRIC>(Y[KOFF], lqs[K], uqs[K], arg);
         }
         const bool fin = busy && b == bend;
+        live_iters += busy ? 1u : 0u;
         if (busy) b++;
         if (__any(fin)) {
             const double d = eld_result<METRIC>((W & 1) ? B[(W - 1) / 2] : A[W / 2], m);
@@ -455,6 +458,11 @@ __global__ void __launch_bounds__(256) k_eld(VerifyArgs a, const double* __restr
         }
         if (check && ((++it) & 3) == 0)
             busy = busy && ad_alive<METRIC, W>(A, B, b - 1, M, m, thr, total, p);
+    }
+    if (work) {
+        unsigned long long v = live_iters;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) atomicAdd(work, v);
     }
 }
 
@@ -577,7 +585,8 @@ static hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64
                                 int64_t n_direct, const double* rows, bool keep_all,
                                 int64_t* out_idx, double* out_dist,
                                 unsigned long long* out_count, unsigned long long* queue,
-                                double* scratch, int64_t scratch_threads) {
+                                unsigned long long* work, double* scratch,
+                                int64_t scratch_threads) {
     const int w = a.p.w;
     const dim3 blk(256);
     const int ka = keep_all ? 1 : 0;
@@ -595,7 +604,7 @@ static hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64
     const dim3 grid(1024);
     const size_t lds = reg_lds_bytes(a.p.m);
 #define FSM_ELD(WW, EX) \
-    hipLaunchKernelGGL((k_eld<METRIC, WW, EX>), grid, blk, lds, st, a, a.q.LQ, a.q.UQ, in, in_total, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, queue)
+    hipLaunchKernelGGL((k_eld<METRIC, WW, EX>), grid, blk, lds, st, a, a.q.LQ, a.q.UQ, in, in_total, in_count, n_direct, rows, ka, out_idx, out_dist, out_count, queue, work)
     if (w == 6) FSM_ELD(6, true);
     else if (w == 25) FSM_ELD(25, true);
     else if (w <= 8) FSM_ELD(8, false);
@@ -610,10 +619,10 @@ static hipError_t dispatch_eld(hipStream_t st, const VerifyArgs& a, const int64_
                                const double* in_total, const unsigned long long* in_count,
                                int64_t n_direct, const double* rows, bool keep_all,
                                int64_t* out_idx, double* out_dist, unsigned long long* out_count,
-                               unsigned long long* queue, double* scratch,
-                               int64_t scratch_threads) {
+                               unsigned long long* queue, unsigned long long* work,
+                               double* scratch, int64_t scratch_threads) {
 #define FSM_M(MM) \
-    launch_metric<MM>(st, a, in, in_total, in_count, n_direct, rows, keep_all, out_idx, out_dist, out_count, queue, scratch, scratch_threads)
+    launch_metric<MM>(st, a, in, in_total, in_count, n_direct, rows, keep_all, out_idx, out_dist, out_count, queue, work, scratch, scratch_threads)
     switch (a.p.metric) {
         case DTW: return FSM_M(DTW);
         case ERP: return FSM_M(ERP);
@@ -628,16 +637,17 @@ static hipError_t dispatch_eld(hipStream_t st, const VerifyArgs& a, const int64_
 hipError_t launch_eld(hipStream_t st, const VerifyArgs& a, const int64_t* in,
                       const double* in_total, const unsigned long long* in_count, bool keep_all,
                       int64_t* out_idx, double* out_dist, unsigned long long* out_count,
-                      unsigned long long* queue, double* scratch, int64_t scratch_threads) {
+                      unsigned long long* queue, unsigned long long* work, double* scratch,
+                      int64_t scratch_threads) {
     return dispatch_eld(st, a, in, in_total, in_count, 0, nullptr, keep_all, out_idx, out_dist,
-                        out_count, queue, scratch, scratch_threads);
+                        out_count, queue, work, scratch, scratch_threads);
 }
 
 hipError_t launch_eld_pairs(hipStream_t st, const VerifyArgs& a, const double* cands, int64_t n,
                             double* out, unsigned long long* queue, double* scratch,
                             int64_t scratch_threads) {
     return dispatch_eld(st, a, nullptr, nullptr, nullptr, n, cands, true, nullptr, out, nullptr,
-                        queue, scratch, scratch_threads);
+                        queue, nullptr, scratch, scratch_threads);
 }
 
 }  // namespace fsm

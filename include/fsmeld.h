@@ -72,6 +72,9 @@ typedef struct {
     /* device time per stage from HIP events on the engine's stream, ms */
     double ms_stats, ms_lbip, ms_glb, ms_eld;
     double ms_total; /* host wall time of the call, ms */
+    /* band cells the ELD kernel evaluated for live candidates (iterations of
+     * the register-band kernel x (2w + 1)); 0 when the row-band kernel ran */
+    uint64_t eld_cells;
 } fsm_stats;
 
 /* ---- device and errors ---- */
