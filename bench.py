@@ -32,6 +32,15 @@ an FMA counting 2).
 
 --cutoff-cache PATH stores the cutoffs (and reads them on later runs), so a
 profiler run can launch the kernels in the timed configuration only.
+
+--workload batch: 2^24 random walk, 256 queries (m log-uniform in [64, 2048],
+w = 5% of m, metric uniform over the five, cutoffs from k=10): the plain path
+(one fsm_query at a time) against fsm_query_batch at several concurrencies;
+every batched result is compared with the plain one.
+--workload serve: the 2^26 target, Poisson arrivals at --rates queries/s of
+queries with m log-uniform in [64, 1024]: latency from submission to result
+through the serving loop (fsm_server_*) against the same arrivals served one
+query at a time (one worker).
 --copy-calibration runs a plain 1 GiB device copy (the scale for HBM
 counters) and exits.
 """
@@ -88,6 +97,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cutoff-cache", default=None)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--workload", default="plain", choices=["plain", "batch", "serve"])
+    ap.add_argument("--concurrency", default="4,8")
+    ap.add_argument("--rates", default="50,200,1000")
+    ap.add_argument("--requests", type=int, default=60, help="arrivals per rate (serve)")
+    ap.add_argument("--workers", type=int, default=8, help="server workers (serve)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -96,6 +110,12 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.steps)
+        return
+    if args.workload == "batch":
+        run_batch(args)
+        return
+    if args.workload == "serve":
+        run_serve(args)
         return
 
     import fsmeld_amd as F
@@ -267,6 +287,147 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def _specs_with_cutoffs(T, queries, k, cache, cache_key):
+    """Search specs whose cutoffs come from k-NN queries (batched, untimed)."""
+    cut = load_cutoffs(cache, cache_key)
+    if cut is None:
+        cspecs = [dict(mode="cutoff", query=q["query"], m=q["m"], metric=q["metric"], w=q["w"],
+                       arg=q["arg"], k=k) for q in queries]
+        t0 = time.perf_counter()
+        cut = []
+        for i in range(0, len(cspecs), 32):
+            res = T.batch(cspecs[i:i + 32], concurrency=8)
+            cut += [r.threshold if not isinstance(r, Exception) else None for r in res]
+            log(f"cutoffs {len(cut)}/{len(cspecs)} (k={k}) after {time.perf_counter() - t0:.1f}s")
+        if cache:
+            with open(cache, "w") as f:
+                json.dump({"key": cache_key, "cutoffs": cut}, f)
+    return [dict(mode="search", query=q["query"], m=q["m"], metric=q["metric"], w=q["w"],
+                 arg=q["arg"], cutoff=c) for q, c in zip(queries, cut) if c is not None]
+
+
+def _pct(v, p):
+    return round(float(np.percentile(v, p)), 3) if len(v) else None
+
+
+def run_batch(args):
+    import fsmeld_amd as F
+    from fsmeld_amd import workloads
+    cfg = workloads.batching()
+    T = F.Target(cfg["target"])
+    n = len(cfg["target"])
+    specs = _specs_with_cutoffs(T, cfg["queries"], cfg["k"], args.cutoff_cache,
+                                f"batching n={n} q={len(cfg['queries'])} k={cfg['k']}")
+    # warm every window length's statistics path once (not timed)
+    T.batch(specs[:16], concurrency=4)
+    t0 = time.perf_counter()
+    plain = []
+    for i, sp in enumerate(specs):
+        plain.append(T.query("search", sp["query"], sp["m"], sp["metric"], sp["w"], sp["arg"],
+                             cutoff_value=sp["cutoff"]))
+        if i % 32 == 31:
+            log(f"plain {i + 1}/{len(specs)} after {time.perf_counter() - t0:.1f}s")
+    plain_s = time.perf_counter() - t0
+    plain_lat = [r.stats["ms_total"] for r in plain]
+    log(f"plain: {len(specs)} queries in {plain_s:.2f}s")
+    modes = []
+    identical = True
+    for c in [int(x) for x in args.concurrency.split(",")]:
+        t0 = time.perf_counter()
+        got = T.batch(specs, concurrency=c)
+        el = time.perf_counter() - t0
+        for g, p in zip(got, plain):
+            if isinstance(g, Exception) or not np.array_equal(g.indices, p.indices) or \
+                    not np.array_equal(g.distances.view(np.uint64), p.distances.view(np.uint64)):
+                identical = False
+        lat = [g.stats["ms_total"] for g in got if not isinstance(g, Exception)]
+        modes.append(dict(concurrency=c, queries_per_s=round(len(specs) / el, 3),
+                          speedup=round(plain_s / el, 3), p50_ms=_pct(lat, 50),
+                          p99_ms=_pct(lat, 99)))
+        log(f"batch concurrency {c}: {el:.2f}s")
+    best = max(modes, key=lambda x: x["queries_per_s"])
+    print(json.dumps({
+        "metric": "queries/sec, batched vs plain one-at-a-time (config 4); p50/p99 latency",
+        "value": best["queries_per_s"], "unit": "queries/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batching: 2^24 random walk, {len(specs)} searches, m log-uniform "
+                               f"[64, 2048], w = 5% m, metric uniform over five, cutoffs k=10"},
+        "plain": dict(queries_per_s=round(len(specs) / plain_s, 3), p50_ms=_pct(plain_lat, 50),
+                      p99_ms=_pct(plain_lat, 99)),
+        "batched": modes, "results_identical_to_plain": identical}), flush=True)
+
+
+def run_serve(args):
+    import threading
+    import fsmeld_amd as F
+    from fsmeld_amd import workloads
+    cfg = workloads.serving()
+    T = F.Target(cfg["target"])
+    n = len(cfg["target"])
+    specs = _specs_with_cutoffs(T, cfg["queries"], cfg["k"], args.cutoff_cache,
+                                f"serving n={n} q={len(cfg['queries'])} k={cfg['k']}")
+    budget = 96 << 30
+    rng = np.random.default_rng(7)
+
+    def serve(rate, workers):
+        gaps = rng.exponential(1.0 / rate, args.requests)
+        order = rng.integers(0, len(specs), args.requests)
+        lat, done_t = [], []
+        with F.Server(T, mem_budget=budget, max_workers=workers) as S:
+            tickets = []
+            lock = threading.Lock()
+
+            def collect():
+                i = 0
+                while True:
+                    with lock:
+                        if i >= len(tickets) and len(tickets) == args.requests:
+                            return
+                        tk = tickets[i] if i < len(tickets) else None
+                    if tk is None:
+                        time.sleep(0.0005)
+                        continue
+                    state, res, l = S.poll(tk, wait_ms=-1)
+                    lat.append(l)
+                    done_t.append(time.perf_counter())
+                    i += 1
+
+            th = threading.Thread(target=collect)
+            th.start()
+            t0 = time.perf_counter()
+            due = t0
+            for g, j in zip(gaps, order):
+                due += g
+                d = due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tk = S.submit(specs[int(j)])
+                with lock:
+                    tickets.append(tk)
+            th.join()
+            span = max(done_t) - t0
+        return dict(offered_rate=rate, completed=len(lat), queries_per_s=round(len(lat) / span, 3),
+                    p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99))
+
+    serve(50, args.workers)  # warm every handle (not reported)
+    rows = []
+    for rate in [float(x) for x in args.rates.split(",")]:
+        a = serve(rate, args.workers)
+        b = serve(rate, 1)
+        rows.append(dict(rate=rate, served=a, one_at_a_time=b))
+        log(f"rate {rate}: served {a['queries_per_s']} q/s p99 {a['p99_ms']} ms; "
+            f"one at a time {b['queries_per_s']} q/s p99 {b['p99_ms']} ms")
+    print(json.dumps({
+        "metric": "served queries/sec and p50/p99 latency under Poisson arrivals (config 5)",
+        "value": max(r["served"]["queries_per_s"] for r in rows), "unit": "queries/s",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"serving: 2^26 target, {len(specs)} distinct searches, m "
+                               f"log-uniform [64, 1024], w = 5% m, mixed metrics, cutoffs k=10; "
+                               f"{args.requests} Poisson arrivals per rate; {args.workers} workers, "
+                               f"{budget >> 30} GiB budget"},
+        "rates": rows}), flush=True)
 
 
 if __name__ == "__main__":

@@ -101,6 +101,20 @@ _sig("fsm_stage_glb_filter", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _I64
 _sig("fsm_stage_eld", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _I64, ctypes.c_int64, _D])
 _sig("fsm_eld_pairs", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      _D, _D, ctypes.c_int64, _D])
+_sig("fsm_target_share", ctypes.c_int, [_VP, ctypes.POINTER(_VP)])
+_sig("fsm_target_trim", None, [_VP])
+_sig("fsm_query_bytes", ctypes.c_uint64, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int])
+_sig("fsm_query_batch", ctypes.c_int, [_VP, ctypes.c_int64, ctypes.POINTER(_Desc),
+                                       ctypes.POINTER(_D), _I64, ctypes.c_int,
+                                       ctypes.POINTER(_VP), ctypes.POINTER(ctypes.c_int)])
+_sig("fsm_server_create", ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_VP)])
+_sig("fsm_server_submit", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, ctypes.c_int64,
+                                         ctypes.POINTER(ctypes.c_uint64)])
+_sig("fsm_server_poll", ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_VP),
+                                       ctypes.POINTER(ctypes.c_int), _D])
+_sig("fsm_server_cancel", ctypes.c_int, [_VP, ctypes.c_uint64])
+_sig("fsm_server_info", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int), _I64, _I64])
+_sig("fsm_server_destroy", None, [_VP])
 _sig("fsm_synth_target", ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, _D])
@@ -136,6 +150,38 @@ class QueryResult:
     @property
     def threshold(self) -> float:
         return self.stats.get("threshold", float("nan"))
+
+
+FSM_PENDING = 1
+FSM_ERR_CANCELLED = -7
+
+
+def _take_result(r) -> QueryResult:
+    """Copies a library result object into a QueryResult and frees it."""
+    try:
+        st = _Stats()
+        _check(_lib.fsm_result_stats(r, ctypes.byref(st)))
+        n = int(_lib.fsm_result_count(r))
+        idx = np.ctypeslib.as_array(_lib.fsm_result_indices(r), shape=(n,)).copy() if n else \
+            np.zeros(0, np.int64)
+        dist = np.ctypeslib.as_array(_lib.fsm_result_distances(r), shape=(n,)).copy() if n \
+            else np.zeros(0, np.float64)
+        stats = {name: getattr(st, name) for name, _ in _Stats._fields_}
+    finally:
+        _lib.fsm_result_free(r)
+    return QueryResult(idx, dist, stats)
+
+
+def _spec_desc(spec) -> _Desc:
+    """Descriptor of a query spec: dict(mode, m, metric, w, arg, cutoff | k, exclude)."""
+    return _desc(spec["mode"], spec["metric"], spec["m"], spec["w"], spec.get("arg", 0.0),
+                 spec.get("cutoff", 0.0), spec.get("k", 0), spec.get("exclude"))
+
+
+def query_bytes(n, metric, m, w) -> int:
+    """Device bytes one handle holds for a query of (metric, m, w) over n samples."""
+    return int(_lib.fsm_query_bytes(int(n), METRICS[metric] if isinstance(metric, str)
+                                    else int(metric), int(m), int(w)))
 
 
 def _desc(mode, metric, m, w, arg, cutoff_value=0.0, k=0, exclude=None) -> _Desc:
@@ -191,18 +237,36 @@ class Target:
         d = _desc(mode, metric, m, w, arg, cutoff_value, k, exclude)
         r = _VP()
         _check(_lib.fsm_query(self._h, ctypes.byref(d), _ptr(q), int(q.size), ctypes.byref(r)))
-        try:
-            st = _Stats()
-            _check(_lib.fsm_result_stats(r, ctypes.byref(st)))
-            n = int(_lib.fsm_result_count(r))
-            idx = np.ctypeslib.as_array(_lib.fsm_result_indices(r), shape=(n,)).copy() if n else \
-                np.zeros(0, np.int64)
-            dist = np.ctypeslib.as_array(_lib.fsm_result_distances(r), shape=(n,)).copy() if n \
-                else np.zeros(0, np.float64)
-            stats = {name: getattr(st, name) for name, _ in _Stats._fields_}
-        finally:
-            _lib.fsm_result_free(r)
-        return QueryResult(idx, dist, stats)
+        return _take_result(r)
+
+    def batch(self, specs, concurrency=4):
+        """Runs every spec (dict(mode, query, m, metric, w, arg, cutoff | k,
+        exclude)) with up to `concurrency` queries in flight; returns one
+        QueryResult or FsmeldError per spec, in order."""
+        n = len(specs)
+        qs = [_f64(sp["query"]) for sp in specs]
+        descs = (_Desc * max(n, 1))(*[_spec_desc(sp) for sp in specs])
+        qptr = (_D * max(n, 1))(*[_ptr(q) for q in qs])
+        lens = np.array([q.size for q in qs] or [0], dtype=np.int64)
+        res = (_VP * max(n, 1))()
+        status = (ctypes.c_int * max(n, 1))()
+        _check(_lib.fsm_query_batch(self._h, n, descs, qptr, _ptr(lens, _I64), int(concurrency),
+                                    res, status))
+        out = []
+        for i in range(n):
+            if status[i] == 0:
+                out.append(_take_result(res[i]))
+            else:
+                out.append(FsmeldError(status[i], "query failed"))
+        return out
+
+    def share(self) -> "Target":
+        """Another handle to the same device samples (own stream and buffers)."""
+        t = Target.__new__(Target)
+        t._h = _VP()
+        t.values = self.values
+        _check(_lib.fsm_target_share(self._h, ctypes.byref(t._h)))
+        return t
 
     def search(self, query, m, cutoff_value, metric, w, arg, exclude=None) -> QueryResult:
         return self.query("search", query, m, metric, w, arg, cutoff_value=cutoff_value,
@@ -326,3 +390,67 @@ def synth_query(seed: int, target: np.ndarray, m: int, noise_sd=0.1):
     _check(_lib.fsm_synth_query(int(seed), _ptr(t), int(t.size), int(m), float(noise_sd),
                                 _ptr(q), ctypes.byref(off)))
     return q, int(off.value)
+
+
+class Server:
+    """Serving loop over a Target: submit / poll / cancel with memory-based
+    admission and concurrent execution (fsm_server_* of include/fsmeld.h)."""
+
+    def __init__(self, target: Target, mem_budget: int, max_workers: int = 4):
+        self._target = target
+        self._h = _VP()
+        _check(_lib.fsm_server_create(target._h, int(mem_budget), int(max_workers),
+                                      ctypes.byref(self._h)))
+        self._queries = {}
+
+    def submit(self, spec) -> int:
+        q = _f64(spec["query"])
+        d = _spec_desc(spec)
+        ticket = ctypes.c_uint64()
+        _check(_lib.fsm_server_submit(self._h, ctypes.byref(d), _ptr(q), int(q.size),
+                                      ctypes.byref(ticket)))
+        return int(ticket.value)
+
+    def poll(self, ticket, wait_ms=0):
+        """(state, result, latency_ms): state "pending", "done" (result is a
+        QueryResult or FsmeldError), or "cancelled"."""
+        r = _VP()
+        status = ctypes.c_int()
+        lat = ctypes.c_double()
+        rc = _lib.fsm_server_poll(self._h, int(ticket), int(wait_ms), ctypes.byref(r),
+                                  ctypes.byref(status), ctypes.byref(lat))
+        if rc == FSM_PENDING:
+            return "pending", None, None
+        if rc == FSM_ERR_CANCELLED:
+            return "cancelled", None, None
+        _check(rc)
+        if status.value != 0:
+            return "done", FsmeldError(status.value, "query failed"), lat.value
+        return "done", _take_result(r), lat.value
+
+    def cancel(self, ticket):
+        _check(_lib.fsm_server_cancel(self._h, int(ticket)))
+
+    def info(self):
+        w = ctypes.c_int()
+        qd = np.zeros(1, np.int64)
+        rn = np.zeros(1, np.int64)
+        _check(_lib.fsm_server_info(self._h, ctypes.byref(w), _ptr(qd, _I64), _ptr(rn, _I64)))
+        return dict(workers=w.value, queued=int(qd[0]), running=int(rn[0]))
+
+    def close(self):
+        if self._h:
+            _lib.fsm_server_destroy(self._h)
+            self._h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

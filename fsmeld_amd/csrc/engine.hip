@@ -20,8 +20,11 @@
 #include <cstring>
 #include <fstream>
 #include <limits>
+#include <atomic>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fsmeld.h"
@@ -199,11 +202,14 @@ double ms_between(hipEvent_t a, hipEvent_t b) {
 
 }  // namespace
 
+// A handle owns a stream, events and every working buffer; the device copy of
+// the target's samples is shared by all handles made from one another with
+// fsm_target_share (and freed with the last of them).
 struct fsm_target {
     int device = 0;
     hipStream_t stream = nullptr;
     int64_t N = 0;
-    DevBuf T;
+    std::shared_ptr<DevBuf> T;
     int stats_m = -1;
     DevBuf MU, SG, INV;
     int env_w = -1;
@@ -211,6 +217,9 @@ struct fsm_target {
     DevBuf lb, list_a, list_b, out_idx, out_dist, slab, rep_lb, rep_idx, tu, tl, qtab,
         scratch, counters;
     hipEvent_t ev[8] = {};
+    // fsm_query_batch's handles, kept between calls
+    std::mutex pool_mu;
+    std::vector<fsm_target*> pool;
 };
 
 struct fsm_result {
@@ -256,7 +265,7 @@ int ensure_stats(fsm_target* t, const Geometry& G, bool* computed) {
     HIP_TRY(t->MU.ensure(sizeof(double) * n));
     HIP_TRY(t->SG.ensure(sizeof(double) * n));
     HIP_TRY(t->INV.ensure(n));
-    HIP_TRY(launch_window_stats(t->stream, t->T.as<double>(), G, t->MU.as<double>(),
+    HIP_TRY(launch_window_stats(t->stream, t->T->as<double>(), G, t->MU.as<double>(),
                                 t->SG.as<double>(), t->INV.as<uint8_t>()));
     t->stats_m = G.m;
     *computed = true;
@@ -267,7 +276,7 @@ int ensure_env(fsm_target* t, int w) {
     if (t->env_w == w) return FSM_OK;
     HIP_TRY(t->Lr.ensure(sizeof(double) * (size_t)t->N));
     HIP_TRY(t->Ur.ensure(sizeof(double) * (size_t)t->N));
-    HIP_TRY(launch_envelope(t->stream, t->T.as<double>(), t->N, w, t->Lr.as<double>(),
+    HIP_TRY(launch_envelope(t->stream, t->T->as<double>(), t->N, w, t->Lr.as<double>(),
                             t->Ur.as<double>()));
     t->env_w = w;
     return FSM_OK;
@@ -284,7 +293,7 @@ int run_lbip(fsm_target* t, const Geometry& G, const QueryTables& Q, const Query
     int blocks = (int)std::min<int64_t>(G.S, per > (1u << 20) ? 1024 : 2048);
     HIP_TRY(t->slab.ensure(per * (size_t)blocks));
     LbipArgs a{};
-    a.T = t->T.as<double>();
+    a.T = t->T->as<double>();
     a.MU = t->MU.as<double>();
     a.SG = t->SG.as<double>();
     a.INV = t->INV.as<uint8_t>();
@@ -308,7 +317,7 @@ int run_lbip(fsm_target* t, const Geometry& G, const QueryTables& Q, const Query
 VerifyArgs verify_args(fsm_target* t, const Geometry& G, const QueryTables& Q,
                        const QueryParams& P) {
     VerifyArgs v{};
-    v.T = t->T.as<double>();
+    v.T = t->T->as<double>();
     v.MU = t->MU.as<double>();
     v.SG = t->SG.as<double>();
     v.Lraw = t->Lr.as<double>();
@@ -388,12 +397,13 @@ int fsm_device_count(void) {
 
 const char* fsm_last_error(void) { return g_last_error.c_str(); }
 
-int fsm_target_open(const double* host, int64_t n, fsm_target** out) {
-    if (!out || !host || n <= 0) return fail(FSM_ERR_ARG, "invalid target");
-    *out = nullptr;
-    if (fsm_device_count() < 1) return fail(FSM_ERR_NODEV, "no HIP device");
+namespace {
+
+// A handle with its own stream, events and counters over device samples T.
+int new_handle(int64_t n, std::shared_ptr<DevBuf> T, fsm_target** out) {
     auto* t = new fsm_target();
     t->N = n;
+    t->T = std::move(T);
     auto cleanup = [&](int rc) {
         fsm_target_close(t);
         return rc;
@@ -404,15 +414,41 @@ int fsm_target_open(const double* host, int64_t n, fsm_target** out) {
     for (auto& e : t->ev)
         if (hipEventCreate(&e) != hipSuccess)
             return cleanup(fail(FSM_ERR_HIP, "hipEventCreate failed"));
-    if (t->T.ensure(sizeof(double) * (size_t)n) != hipSuccess)
-        return cleanup(fail(FSM_ERR_OOM, "device allocation of the target failed"));
     if (t->counters.ensure(sizeof(unsigned long long) * 16) != hipSuccess)
         return cleanup(fail(FSM_ERR_OOM, "device allocation failed"));
-    if (hipMemcpy(t->T.p, host, sizeof(double) * (size_t)n, hipMemcpyHostToDevice) !=
-        hipSuccess)
-        return cleanup(fail(FSM_ERR_HIP, "target upload failed"));
     *out = t;
     return FSM_OK;
+}
+
+}  // namespace
+
+int fsm_target_open(const double* host, int64_t n, fsm_target** out) {
+    if (!out || !host || n <= 0) return fail(FSM_ERR_ARG, "invalid target");
+    *out = nullptr;
+    if (fsm_device_count() < 1) return fail(FSM_ERR_NODEV, "no HIP device");
+    fsm_target* t = nullptr;
+    int rc = new_handle(n, std::shared_ptr<DevBuf>(new DevBuf(), [](DevBuf* b) {
+                            b->release();
+                            delete b;
+                        }), &t);
+    if (rc) return rc;
+    if (t->T->ensure(sizeof(double) * (size_t)n) != hipSuccess) {
+        fsm_target_close(t);
+        return fail(FSM_ERR_OOM, "device allocation of the target failed");
+    }
+    if (hipMemcpy(t->T->p, host, sizeof(double) * (size_t)n, hipMemcpyHostToDevice) !=
+        hipSuccess) {
+        fsm_target_close(t);
+        return fail(FSM_ERR_HIP, "target upload failed");
+    }
+    *out = t;
+    return FSM_OK;
+}
+
+int fsm_target_share(fsm_target* base, fsm_target** out) {
+    if (!base || !out) return fail(FSM_ERR_ARG, "null argument");
+    *out = nullptr;
+    return new_handle(base->N, base->T, out);
 }
 
 int fsm_target_open_file(const char* path, fsm_target** out) {
@@ -433,8 +469,10 @@ int64_t fsm_target_length(const fsm_target* t) { return t ? t->N : 0; }
 
 void fsm_target_close(fsm_target* t) {
     if (!t) return;
+    for (fsm_target* h : t->pool) fsm_target_close(h);
+    t->pool.clear();
     if (t->stream) (void)hipStreamSynchronize(t->stream);
-    DevBuf* bufs[] = {&t->T, &t->MU, &t->SG, &t->INV, &t->Lr, &t->Ur, &t->lb,
+    DevBuf* bufs[] = {&t->MU, &t->SG, &t->INV, &t->Lr, &t->Ur, &t->lb,
                       &t->list_a, &t->list_b, &t->out_idx, &t->out_dist, &t->slab,
                       &t->rep_lb, &t->rep_idx, &t->tu, &t->tl, &t->qtab, &t->scratch,
                       &t->counters};
@@ -442,6 +480,7 @@ void fsm_target_close(fsm_target* t) {
     for (auto& e : t->ev)
         if (e) (void)hipEventDestroy(e);
     if (t->stream) (void)hipStreamDestroy(t->stream);
+    t->T.reset();
     delete t;
 }
 
@@ -796,3 +835,69 @@ int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const d
 }
 
 }  // extern "C"
+
+int fsm_query_batch(fsm_target* t, int64_t n, const fsm_query_desc* descs,
+                    const double* const* queries, const int64_t* query_lens, int concurrency,
+                    fsm_result** results, int* status) {
+    if (!t || n < 0 || (n > 0 && (!descs || !queries || !query_lens || !results || !status)))
+        return fail(FSM_ERR_ARG, "null argument");
+    if (concurrency < 1) return fail(FSM_ERR_ARG, "concurrency must be at least 1");
+    for (int64_t i = 0; i < n; i++) {
+        results[i] = nullptr;
+        status[i] = FSM_ERR_ARG;
+    }
+    if (n == 0) return FSM_OK;
+    const int k = (int)std::min<int64_t>(concurrency, n);
+    std::vector<fsm_target*> hs;
+    {
+        std::lock_guard<std::mutex> lk(t->pool_mu);
+        while ((int)t->pool.size() < k) {
+            fsm_target* h = nullptr;
+            int rc = fsm_target_share(t, &h);
+            if (rc) return rc;
+            t->pool.push_back(h);
+        }
+        hs.assign(t->pool.begin(), t->pool.begin() + k);
+    }
+    // each worker thread owns one handle (stream and buffers) and takes the
+    // next query as soon as its previous one has finished
+    std::atomic<int64_t> next{0};
+    auto work = [&](fsm_target* h) {
+        (void)hipSetDevice(0);
+        for (int64_t i = next++; i < n; i = next++)
+            status[i] = fsm_query(h, &descs[i], queries[i], query_lens[i], &results[i]);
+    };
+    std::vector<std::thread> th;
+    for (int j = 1; j < k; j++) th.emplace_back(work, hs[j]);
+    work(hs[0]);
+    for (auto& x : th) x.join();
+    return FSM_OK;
+}
+
+uint64_t fsm_query_bytes(int64_t n, int metric, int m, int w) {
+    Geometry G;
+    if (m < 2 || w < 0 || w >= m || !make_geometry(n, m, &G)) return 0;
+    const uint64_t rows = (uint64_t)G.S * (uint64_t)G.kml;
+    const size_t per = lbip_slab_bytes(G);
+    const uint64_t blocks = (uint64_t)std::min<int64_t>(G.S, per > (1u << 20) ? 1024 : 2048);
+    uint64_t b = 17 * rows                          // window mean, std, invalid flag
+                 + 16 * (uint64_t)n                 // target envelope
+                 + 40 * (uint64_t)G.ts_n            // LB, two candidate lists, matches
+                 + 32 * (uint64_t)G.S               // segment thresholds, representatives
+                 + per * blocks                     // LB_IP per-block scratch
+                 + 80 * (uint64_t)(m + 1) + 128;    // query tables, counters
+    if (eld_uses_scratch(metric, m, w))
+        b += sizeof(double) * 2 * (uint64_t)(2 * w + 2) * (uint64_t)eld_scratch_threads();
+    return b;
+}
+
+void fsm_target_trim(fsm_target* t) {
+    if (!t) return;
+    (void)hipStreamSynchronize(t->stream);
+    DevBuf* bufs[] = {&t->MU, &t->SG, &t->INV, &t->Lr, &t->Ur, &t->lb, &t->list_a, &t->list_b,
+                      &t->out_idx, &t->out_dist, &t->slab, &t->rep_lb, &t->rep_idx, &t->tu,
+                      &t->tl, &t->qtab, &t->scratch};
+    for (DevBuf* b : bufs) b->release();
+    t->stats_m = -1;
+    t->env_w = -1;
+}

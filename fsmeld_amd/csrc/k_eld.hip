@@ -475,9 +475,16 @@ struct MemArr {
     __device__ double& operator[](int k) { return base[(int64_t)k * stride]; }
 };
 
+// Abandon test every 8 rows when thr < 1e300: every path leaves row i at a
+// column j <= i + w (DTW, ERP, EDR, MSM) and still consumes every later
+// sample, or enters it at a column >= i - w (LCSS) and has at most one match
+// per later matchable sample and per later row (see ad_alive); *dead is set
+// and the sentinel returned when the window cannot match.
 template <int METRIC>
 __device__ double eld_rows(const double* __restrict__ q, const double* __restrict__ qg,
-                           const CandSrc& c, int m, int w, double arg, MemArr b, MemArr y) {
+                           const CandSrc& c, int m, int w, double arg, MemArr b, MemArr y,
+                           double thr, double total, const double* __restrict__ LQ,
+                           const double* __restrict__ UQ, bool* dead) {
     constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
     const int M = GRID1 ? m : m - 1;
     const int xoff = GRID1 ? 1 : 0;
@@ -488,6 +495,10 @@ __device__ double eld_rows(const double* __restrict__ q, const double* __restric
         b[k] = S;
         y[k] = c.at(k - w - xoff);
     }
+    const bool check = thr < 1e300;
+    double p = 0.0;
+    int kp = 0;  // samples 0 .. kp-1 are in p
+    *dead = false;
     for (int i = 0; i <= M; i++) {
         const double xi = qat(q, i - xoff, m), xp = qat(q, i - 1 - xoff, m);
         const double xg = (METRIC == ERP) ? qat(qg, i - xoff, m) : 0.0;
@@ -507,6 +518,26 @@ __device__ double eld_rows(const double* __restrict__ q, const double* __restric
             }
             b[k] = v;
         }
+        if (check && (i & 7) == 7 && i < M) {
+            const int K = (METRIC == LCSS) ? i - w : (GRID1 ? i + w - 1 : i + w);
+            for (; kp <= K && kp < m; kp++) p += consume_lb<METRIC>(c.at(kp), LQ[kp], UQ[kp], arg);
+            bool alive;
+            if (METRIC == LCSS) {
+                double mx = b[0];
+                for (int k = 1; k <= w2; k++) mx = fmax(mx, b[k]);
+                const double rem = fmin((double)(M - i), total - p);
+                alive = !(1.0 - (mx + rem) / (double)m > thr);
+            } else {
+                double mn = b[0];
+                for (int k = 1; k <= w2; k++) mn = fmin(mn, b[k]);
+                const double cb = fmax(total - p * (1.0 + (double)(m + 8) * 1.2e-16), 0.0);
+                alive = !((mn + cb) * (1.0 - (double)(2 * m + 24) * 1.2e-16) > thr);
+            }
+            if (!alive) {
+                *dead = true;
+                return S;
+            }
+        }
         const double ynew = c.at(i + 1 + w - xoff);
         for (int k = 0; k < w2; k++) y[k] = y[k + 1];
         y[w2] = ynew;
@@ -523,7 +554,10 @@ __device__ double eld_rows(const double* __restrict__ q, const double* __restric
 // Grid-stride kernel of the row-band path (radii above the register
 // variants, or queries too long for their LDS staging).
 template <int METRIC>
-__global__ void __launch_bounds__(256) k_eld_rows(VerifyArgs a, const int64_t* __restrict__ in,
+__global__ void __launch_bounds__(256) k_eld_rows(VerifyArgs a, const double* __restrict__ LQ,
+                                                  const double* __restrict__ UQ,
+                                                  const int64_t* __restrict__ in,
+                                                  const double* __restrict__ in_total,
                                                   const unsigned long long* __restrict__ in_count,
                                                   int64_t n_direct,
                                                   const double* __restrict__ rows, int keep_all,
@@ -537,6 +571,11 @@ __global__ void __launch_bounds__(256) k_eld_rows(VerifyArgs a, const int64_t* _
     const int64_t iters = (n + stride - 1) / stride;
     const int m = a.p.m, w = a.p.w;
     const double cut = a.p.cutoff;
+    double thr = 1e308;
+    if (!keep_all && !(METRIC == MSM && a.p.arg < 0.0)) {
+        if (METRIC == DTW || METRIC == ERP) thr = cut * cut * (1.0 + 1e-12) + 1e-300;
+        else thr = cut * (1.0 + 1e-12) + 1e-300;
+    }
     int64_t ci = tid;
     for (int64_t itx = 0; itx < iters; itx++, ci += stride) {
         const bool have = ci < n;
@@ -559,12 +598,17 @@ __global__ void __launch_bounds__(256) k_eld_rows(VerifyArgs a, const int64_t* _
         }
         MemArr bb{scratch + tid, stride * 2};
         MemArr yy{scratch + tid + stride, stride * 2};
-        const double d = eld_rows<METRIC>(a.q.qn, a.q.XG, c, m, w, a.p.arg, bb, yy);
+        const double total =
+            (in_total && have) ? in_total[ci] : ((METRIC == LCSS) ? (double)m : 0.0);
+        bool dead = false;
+        const double d = eld_rows<METRIC>(a.q.qn, a.q.XG, c, m, w, a.p.arg, bb, yy, thr, total,
+                                          LQ, UQ, &dead);
         if (keep_all) {
             if (have) out_dist[ci] = d;
             continue;
         }
-        wave_append(have && __builtin_isfinite(d) && d <= cut, g, d, out_idx, out_dist, out_count);
+        wave_append(have && !dead && __builtin_isfinite(d) && d <= cut, g, d, out_idx, out_dist,
+                    out_count);
     }
 }
 
@@ -593,8 +637,8 @@ static hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64
     if (eld_uses_scratch(METRIC, a.p.m, w)) {
         if (!scratch) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_eld_rows<METRIC>), dim3((unsigned)(scratch_threads / 256)), blk, 0, st,
-                           a, in, in_count, n_direct, rows, ka, out_idx, out_dist, out_count,
-                           scratch);
+                           a, a.q.LQ, a.q.UQ, in, in_total, in_count, n_direct, rows, ka, out_idx,
+                           out_dist, out_count, scratch);
         return hipGetLastError();
     }
     if (!queue) return hipErrorInvalidValue;

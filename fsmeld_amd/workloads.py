@@ -44,3 +44,19 @@ def batch_queries(target: np.ndarray, count: int = 256, seed: int = 4, m_lo: int
         q, off = synth_query(seed * 100003 + i, target, m, 0.1)
         out.append(dict(query=q, offset=off, m=m, w=int(0.05 * m), metric=metric, arg=arg))
     return out
+
+
+def batching(seed: int = 4, n: int = 1 << 24, count: int = 256):
+    """2^24 random walk; `count` queries with m log-uniform in [64, 2048],
+    w = floor(0.05 m), metric uniform over the five; cutoffs from k=10."""
+    t = synth_target(seed, n, 0.0, 0.0, n_const_runs=0, const_len=0, n_nans=0)
+    return dict(target=t, queries=batch_queries(t, count, seed, 64, 2048), k=10)
+
+
+def serving(seed: int = 5, n: int = 1 << 26, distinct: int = 64):
+    """The plain large configuration's 2^26 target; `distinct` queries with m
+    log-uniform in [64, 1024], w = floor(0.05 m), mixed metrics; cutoffs from
+    k=10.  Arrivals are drawn by the caller (Poisson, rate lambda)."""
+    big = plain_large(n=n)
+    t = big["target"]
+    return dict(target=t, queries=batch_queries(t, distinct, seed, 64, 1024), k=10)

@@ -88,6 +88,12 @@ int fsm_target_open_file(const char* path, fsm_target** out);
 int64_t fsm_target_length(const fsm_target* t);
 void fsm_target_close(fsm_target* t);
 
+/* Another handle to the same device-resident samples, with its own HIP
+ * stream and working buffers: handles made this way may run queries
+ * concurrently from different host threads (the reference runs one query per
+ * process, src/main.cpp:40-110).  The samples are freed with the last handle. */
+int fsm_target_share(fsm_target* base, fsm_target** out);
+
 /* ---- query: search / cutoff / exact over the whole target ----
  * query_raw holds at least m raw (un-normalised) query values; the engine
  * z-normalises them as src/fsmeld.cpp:622-639 does.
@@ -101,6 +107,54 @@ int64_t fsm_result_count(const fsm_result* r);
 const int64_t* fsm_result_indices(const fsm_result* r);
 const double* fsm_result_distances(const fsm_result* r);
 void fsm_result_free(fsm_result* r);
+
+/* ---- many queries over one target (no reference counterpart: the
+ *      reference answers one query per process) ----
+ * Runs queries[0 .. n) (query i: descs[i], query_lens[i] raw values) with up
+ * to `concurrency` in flight, each on its own handle (kept in t between
+ * calls) driven by its own host thread; results[i] and status[i] receive what
+ * fsm_query would return for query i alone.  Returns FSM_OK once every query
+ * has run, or an error if the handles could not be set up. */
+int fsm_query_batch(fsm_target* t, int64_t n, const fsm_query_desc* descs,
+                    const double* const* queries, const int64_t* query_lens, int concurrency,
+                    fsm_result** results, int* status);
+
+/* Device bytes a handle holds while running a query of (metric, m, w) over n
+ * samples (working buffers only; the samples themselves are shared). */
+uint64_t fsm_query_bytes(int64_t n, int metric, int m, int w);
+
+/* Frees a handle's working buffers (they are re-created on its next query). */
+void fsm_target_trim(fsm_target* t);
+
+/* ---- serving loop (no reference counterpart) ----
+ * A pool of worker threads, each driving its own handle that shares t's
+ * device samples, serves submitted queries first come, first served.  A free
+ * worker starts the oldest queued query as soon as the working memory of the
+ * running and cached queries plus its own estimate (fsm_query_bytes) fits
+ * mem_budget; idle workers release cached buffers while the oldest query
+ * does not fit.  Queries that arrive while others run are admitted as
+ * workers and memory free up, and their kernels overlap on the device. */
+#define FSM_PENDING 1             /* fsm_server_poll: not finished yet */
+#define FSM_ERR_CANCELLED (-7)    /* the query was cancelled */
+typedef struct fsm_server fsm_server;
+int fsm_server_create(fsm_target* t, uint64_t mem_budget, int max_workers, fsm_server** out);
+/* Copies the query; *ticket identifies it.  FSM_ERR_OOM if its estimate alone
+ * exceeds the budget.  fsm_query's own errors (argument validation included)
+ * arrive as the *status of fsm_server_poll. */
+int fsm_server_submit(fsm_server* s, const fsm_query_desc* desc, const double* query_raw,
+                      int64_t query_len, uint64_t* ticket);
+/* Waits up to wait_ms (0: no wait, < 0: until done).  FSM_OK: *out and *status
+ * receive fsm_query's result and status, *latency_ms the time from submit to
+ * completion, and the ticket is retired; FSM_PENDING: still queued / running;
+ * FSM_ERR_CANCELLED: cancelled (ticket retired); FSM_ERR_ARG: unknown ticket. */
+int fsm_server_poll(fsm_server* s, uint64_t ticket, int wait_ms, fsm_result** out, int* status,
+                    double* latency_ms);
+/* A queued query is dropped at once; a running one finishes on the device and
+ * its result is discarded. */
+int fsm_server_cancel(fsm_server* s, uint64_t ticket);
+int fsm_server_info(fsm_server* s, int* workers, int64_t* queued, int64_t* running);
+/* Cancels what is queued, waits for running queries, frees everything. */
+void fsm_server_destroy(fsm_server* s);
 
 /* ---- stage-level entry points (parity tests; outputs in host memory) ---- */
 
