@@ -612,15 +612,196 @@ __global__ void __launch_bounds__(256) k_eld_rows(VerifyArgs a, const double* __
     }
 }
 
+// ------------------------------------------------------------ wide bands ---
+// One wavefront per candidate (radii above the register variants).  The band
+// slots k = i - j + w in [0, 2w] are spread over the lanes, 2 CPL slots per
+// lane (lane l holds k = 2 CPL l + q, q in [0, 2 CPL)); anti-diagonal d
+// updates the slots of parity (d + w) & 1 from the other parity (up k-1,
+// left k+1) and its own old value (diag), so one step needs one neighbour
+// value from the adjacent lane.  Cells use the reference's boundary forms
+// (cell<METRIC, true>).  Query and candidate samples sit in LDS; a wave takes
+// its next candidate from the work queue when the last one ends.
+template <int METRIC, int CPL>
+__device__ __forceinline__ void wave_step(int par, int d, int w, int M, int m, double arg,
+                                          const double* xs, const double* ys, double* V,
+                                          int& live_cells) {
+    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
+    constexpr int Q = 2 * CPL;
+    const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;
+    const int lane = threadIdx.x & 63;
+    // neighbour slot from the adjacent lane: the previous lane's last slot
+    // (parity 0 updates need it as "up" of q = 0), or the next lane's first
+    // (parity 1 updates need it as "left" of q = Q - 1)
+    const double nb = (par == 0) ? __shfl_up(V[Q - 1], 1, 64) : __shfl_down(V[0], 1, 64);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        if ((q & 1) != par) continue;
+        const int k = Q * lane + q;
+        const int sft = k - w;
+        const int i = (d + sft) >> 1, j = (d - sft) >> 1;
+        const bool inr = k <= 2 * w && i >= 0 && j >= 0 && i <= M && j <= M;
+        double v = S;
+        if (inr) {
+            const double up = (k == 0) ? S : (q >= 1 ? V[q - 1] : nb);
+            const double left = (k == 2 * w) ? S : (q + 1 < Q ? V[q + 1] : nb);
+            if (GRID1) {
+                const double xp = xs[i - 1], yp = ys[j - 1];
+                v = cell<METRIC, true>(i, j, up, left, V[q], 0.0, xp, 0.0, yp,
+                                       (xp - arg) * (xp - arg), (yp - arg) * (yp - arg), arg);
+            } else {
+                v = cell<METRIC, true>(i, j, up, left, V[q], xs[i], xs[i - 1], ys[j], ys[j - 1],
+                                       0.0, 0.0, arg);
+            }
+            live_cells++;
+        }
+        V[q] = v;
+    }
+}
+
+template <int METRIC, int CPL>
+__global__ void __launch_bounds__(64) k_eld_wave(VerifyArgs a, const double* __restrict__ LQ,
+                                                 const double* __restrict__ UQ,
+                                                 const int64_t* __restrict__ in,
+                                                 const double* __restrict__ in_total,
+                                                 const unsigned long long* __restrict__ in_count,
+                                                 int64_t n_direct, const double* __restrict__ rows,
+                                                 int keep_all, int64_t* __restrict__ out_idx,
+                                                 double* __restrict__ out_dist,
+                                                 unsigned long long* out_count,
+                                                 unsigned long long* queue,
+                                                 unsigned long long* work) {
+    extern __shared__ double wlds[];
+    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
+    constexpr int Q = 2 * CPL;
+    const int64_t n = in_count ? (int64_t)*in_count : n_direct;
+    const int m = a.p.m, w = a.p.w;
+    const int M = GRID1 ? m : m - 1;
+    const double arg = a.p.arg, cut = a.p.cutoff;
+    double thr = 1e308;
+    if (!keep_all && !(METRIC == MSM && arg < 0.0)) {
+        if (METRIC == DTW || METRIC == ERP) thr = cut * cut * (1.0 + 1e-12) + 1e-300;
+        else thr = cut * (1.0 + 1e-12) + 1e-300;
+    }
+    const bool check = thr < 1e300;
+    const bool raw = rows == nullptr;
+    const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;
+    const int lane = threadIdx.x & 63;
+    // xs[i], ys[j] for i, j in [-1, m]; out-of-range samples read as 0 and
+    // are only ever used by the boundary forms that ignore them
+    double* xs = wlds + 1;
+    double* ys = wlds + (m + 2) + 1;
+    for (int i = lane - 1; i <= m; i += 64) xs[i] = (i >= 0 && i < m) ? a.q.qn[i] : 0.0;
+    unsigned long long live_total = 0;
+    while (true) {
+        unsigned long long c = 0;
+        if (lane == 0) c = atomicAdd(queue, 1ull);
+        c = __shfl(c, 0, 64);
+        if (c >= (unsigned long long)n) break;
+        int64_t g;
+        const double* tp;
+        double mu = 0.0, sd = 1.0;
+        if (raw) {
+            g = in[c];
+            const int64_t si = stat_index(a.G, g);
+            tp = a.T + g;
+            mu = a.MU[si];
+            sd = a.SG[si];
+        } else {
+            g = (int64_t)c;
+            tp = rows + (int64_t)c * m;
+        }
+        __syncthreads();  // the previous candidate's reads of ys are done
+        for (int j = lane - 1; j <= m; j += 64)
+            ys[j] = (j >= 0 && j < m) ? (raw ? (tp[j] - mu) / sd : tp[j]) : 0.0;
+        __syncthreads();
+        const double total = in_total ? in_total[c] : ((METRIC == LCSS) ? (double)m : 0.0);
+        double V[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) V[q] = S;
+        double p = 0.0;
+        int kp = 0;  // samples 0 .. kp-1 are in p
+        int live_cells = 0;
+        bool alive = true;
+        for (int d = 0; d <= 2 * M; d++) {
+            const int par = (d + w) & 1;
+            if (par == 0) wave_step<METRIC, CPL>(0, d, w, M, m, arg, xs, ys, V, live_cells);
+            else wave_step<METRIC, CPL>(1, d, w, M, m, arg, xs, ys, V, live_cells);
+            if (check && (d & 7) == 7 && d < 2 * M) {
+                // frontier: anti-diagonals d - 1 and d
+                // samples at or before the frontier's largest column (LCSS:
+                // before its smallest), floor((d + w) / 2) resp. floor((d-1-w) / 2)
+                const int K = (METRIC == LCSS) ? ((d - 1 - w) >= 0 ? (d - 1 - w) / 2 : -1)
+                                               : ((d + w) >> 1) - (GRID1 ? 1 : 0);
+                const int kn = min(K, m - 1) + 1;  // new kp
+                for (int base = kp; base < kn; base += 64) {
+                    double add = 0.0;
+                    if (base + lane < kn) {
+                        const int s = base + lane;
+                        add = consume_lb<METRIC>(ys[s], LQ[s], UQ[s], arg);
+                    }
+                    for (int o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o, 64);
+                    p += add;
+                }
+                if (kn > kp) kp = kn;
+                double e = V[0];
+#pragma unroll
+                for (int q = 1; q < Q; q++)
+                    e = (METRIC == LCSS) ? fmax(e, V[q]) : fmin(e, V[q]);
+                for (int o = 32; o > 0; o >>= 1) {
+                    const double t = __shfl_xor(e, o, 64);
+                    e = (METRIC == LCSS) ? fmax(e, t) : fmin(e, t);
+                }
+                if (METRIC == LCSS) {
+                    const double rem = fmin((double)((2 * M - (d - 1)) / 2 + 1), total - p);
+                    alive = !(1.0 - (e + rem) / (double)m > thr);
+                } else {
+                    const double cb = fmax(total - p * (1.0 + (double)(m + 8) * 1.2e-16), 0.0);
+                    alive = !((e + cb) * (1.0 - (double)(2 * m + 24) * 1.2e-16) > thr);
+                }
+                if (!alive) break;
+            }
+        }
+        live_total += (unsigned long long)live_cells;
+        if (!alive) continue;
+        // cell (M, M): slot k = w
+        const double r = __shfl(V[w % Q], w / Q, 64);
+        const double dist = eld_result<METRIC>(r, m);
+        if (lane == 0) {
+            if (keep_all) {
+                out_dist[c] = dist;
+            } else if (__builtin_isfinite(dist) && dist <= cut) {  // src/fsmeld.cpp:846-847
+                const unsigned long long pos = atomicAdd(out_count, 1ull);
+                out_idx[pos] = g;
+                out_dist[pos] = dist;
+            }
+        }
+    }
+    if (work) {
+        for (int o = 32; o > 0; o >>= 1) live_total += __shfl_xor(live_total, o, 64);
+        // cells, reported as lane-iterations x (2w + 1) by the caller
+        if (lane == 0) atomicAdd(work, live_total / (unsigned long long)(2 * w + 1));
+    }
+}
+
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
 int64_t eld_scratch_threads() { return (int64_t)512 * 256; }
 
 static size_t reg_lds_bytes(int m) { return sizeof(double) * (size_t)(m + 2 * kXPad) * 3; }
 
+static size_t wave_lds_bytes(int m) { return sizeof(double) * 2 * (size_t)(m + 2); }
+
+// register band: w <= 32 (ERP: 25); one wave per candidate: w <= 127; the
+// row-band path beyond, or where the LDS staging does not fit
+static bool uses_register_band(int metric, int m, int w) {
+    return reg_lds_bytes(m) <= kEldLdsCap && (w <= 25 || (w <= 32 && metric != ERP));
+}
+static bool uses_wave_band(int metric, int m, int w) {
+    return !uses_register_band(metric, m, w) && w <= 127 && wave_lds_bytes(m) <= kEldLdsCap;
+}
+
 bool eld_uses_scratch(int metric, int m, int w) {
-    if (reg_lds_bytes(m) > kEldLdsCap) return true;
-    return w > 32 || (w > 25 && metric == ERP);
+    return !uses_register_band(metric, m, w) && !uses_wave_band(metric, m, w);
 }
 
 template <int METRIC>
@@ -643,6 +824,18 @@ static hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64
     }
     if (!queue) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, queue);
+    if (uses_wave_band(METRIC, a.p.m, w)) {
+        const size_t wl = wave_lds_bytes(a.p.m);
+        if (w <= 63)
+            hipLaunchKernelGGL((k_eld_wave<METRIC, 1>), dim3(2048), dim3(64), wl, st, a, a.q.LQ,
+                               a.q.UQ, in, in_total, in_count, n_direct, rows, ka, out_idx,
+                               out_dist, out_count, queue, work);
+        else
+            hipLaunchKernelGGL((k_eld_wave<METRIC, 2>), dim3(2048), dim3(64), wl, st, a, a.q.LQ,
+                               a.q.UQ, in, in_total, in_count, n_direct, rows, ka, out_idx,
+                               out_dist, out_count, queue, work);
+        return hipGetLastError();
+    }
     // persistent: enough waves to fill every SIMD; later blocks find the
     // queue drained and exit
     const dim3 grid(1024);

@@ -11,6 +11,8 @@ namespace fsm {
 
 // One lane per macro-segment (src/mvmean.cpp:9-46 and the invalid test of
 // src/fsmeld.cpp:110-129).  Outputs are segment-major [S][kml].
+constexpr int kStatsU = 16;
+
 __global__ void __launch_bounds__(64) k_window_stats(const double* __restrict__ T, Geometry G,
                                                      double* __restrict__ MU,
                                                      double* __restrict__ SG,
@@ -26,15 +28,26 @@ __global__ void __launch_bounds__(64) k_window_stats(const double* __restrict__ 
     const double nan = __builtin_nan("");
     double sum1 = 0.0, sum2 = 0.0;
     int bad = 0;
-    for (int i = 0; i < l; i++) {
-        const double v = a[i];
+    // The recurrences are sequential per segment (their order fixes the bits);
+    // loads are issued kStatsU at a time ahead of the chain so that each lane
+    // keeps many in flight instead of waiting on one per sample.
+    auto first = [&](double v) {
         if (__builtin_isfinite(v)) {
             sum1 += v;
             sum2 += v * v;
         } else {
             ++bad;
         }
+    };
+    int i0 = 0;
+    for (; i0 + kStatsU <= l; i0 += kStatsU) {
+        double v[kStatsU];
+#pragma unroll
+        for (int u = 0; u < kStatsU; u++) v[u] = a[i0 + u];
+#pragma unroll
+        for (int u = 0; u < kStatsU; u++) first(v[u]);
     }
+    for (; i0 < l; i0++) first(a[i0]);
     {
         const double m0 = bad ? nan : sum1 / dl;
         const double s0 = bad ? nan : sqrt(sum2 / dl - m0 * m0);
@@ -43,9 +56,7 @@ __global__ void __launch_bounds__(64) k_window_stats(const double* __restrict__ 
         inv[0] = (bad > 0) || !__builtin_isfinite(s0) || s0 <= 1e-12;
     }
     const int64_t kml = G.kml;
-    for (int64_t i = 1; i < kml; i++) {
-        const double vo = a[i - 1];
-        const double vi = a[i + l - 1];
+    auto step = [&](int64_t i, double vo, double vi) {
         const bool of = __builtin_isfinite(vo);
         const bool nf = __builtin_isfinite(vi);
         const double out = of ? vo : 0.0;
@@ -58,7 +69,19 @@ __global__ void __launch_bounds__(64) k_window_stats(const double* __restrict__ 
         mu[i] = mi;
         si[i] = sg;
         inv[i] = (bad > 0) || !__builtin_isfinite(sg) || sg <= 1e-12;
+    };
+    int64_t i = 1;
+    for (; i + kStatsU <= kml; i += kStatsU) {
+        double vo[kStatsU], vi[kStatsU];
+#pragma unroll
+        for (int u = 0; u < kStatsU; u++) {
+            vo[u] = a[i + u - 1];
+            vi[u] = a[i + u + l - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kStatsU; u++) step(i + u, vo[u], vi[u]);
     }
+    for (; i < kml; i++) step(i, a[i - 1], a[i + l - 1]);
 }
 
 hipError_t launch_window_stats(hipStream_t st, const double* T, const Geometry& G, double* MU,

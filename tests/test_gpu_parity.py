@@ -58,7 +58,8 @@ def test_window_stats_exclusion(gpu, small):
 
 @pytest.mark.parametrize("metric,arg", METRICS)
 @pytest.mark.parametrize("m,w", [(2, 0), (7, 1), (33, 0), (40, 6), (64, 25), (50, 33), (24, 23),
-                                 (64, 30), (2100, 8)])
+                                 (64, 30), (2100, 8), (3000, 8), (160, 100), (130, 70),
+                                 (4500, 10)])
 def test_eld_pairs_bitexact(gpu, metric, arg, m, w):
     rng = np.random.default_rng(m * 100 + w)
     q = rng.normal(size=m)
@@ -114,7 +115,7 @@ def test_glb_filter_decisions(gpu, small, metric, arg, m, w):
 
 
 @pytest.mark.parametrize("metric,arg", METRICS)
-@pytest.mark.parametrize("m,w", [(64, 4), (64, 12), (100, 30), (80, 36)])
+@pytest.mark.parametrize("m,w", [(64, 4), (64, 12), (100, 30), (80, 36), (160, 70)])
 def test_exact_command_abandon_is_safe(gpu, small, metric, arg, m, w):
     """The exact command abandons windows by DP frontier + suffix bound; with
     the cutoff set exactly on windows' distances it must return precisely the

@@ -109,8 +109,10 @@ def test_server_cancel_and_budget(gpu, target):
         states = [S.poll(tk, wait_ms=-1)[0] for tk in tickets]
         assert states[-1] == "cancelled"
         assert all(st == "done" for st in states[:-1])
+        big = dict(sp, m=1024, w=200)   # row-band path: its scratch exceeds the budget
+        assert F.query_bytes(len(t), "msm", 1024, 200) > need
         with pytest.raises(F.FsmeldError) as e:
-            S.submit(dict(sp, m=1024, w=51))   # larger than the whole budget
+            S.submit(big)
         assert e.value.code == -2
         # pending poll returns at once
         tk = S.submit(sp)
