@@ -215,7 +215,7 @@ struct fsm_target {
     int env_w = -1;
     DevBuf Lr, Ur;
     DevBuf lb, list_a, list_b, out_idx, out_dist, slab, rep_lb, rep_idx, tu, tl, qtab,
-        scratch, counters;
+        scratch, counters, aux;
     hipEvent_t ev[8] = {};
     // fsm_query_batch's handles, kept between calls
     std::mutex pool_mu;
@@ -348,11 +348,84 @@ QueryParams params_of(const fsm_query_desc* d) {
     return P;
 }
 
+int ensure_eld_scratch(fsm_target* t, int metric, int m, int w);
+
+// Tighter cutoff bound (see run_filter_verify): samples the GLB survivors'
+// certified bounds (lb[0 .. C[2])), selects about max(8k, 4096) survivors of
+// smallest bound (largest for LCSS, whose bound counts possible matches),
+// computes their exact distances and lowers P->cutoff to their k-th smallest
+// when that is smaller.  Uses list_a, out_dist and counters 4, 6, 7.
+int tighten_cutoff(fsm_target* t, const Geometry& G, const QueryTables& Q, QueryParams* P,
+                   int k, uint64_t* evals) {
+    unsigned long long* C = t->counters.as<unsigned long long>();
+    unsigned long long n_l = 0;
+    HIP_TRY(hipMemcpyAsync(&n_l, C + 2, sizeof(n_l), hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    const int64_t want = std::max<int64_t>(8 * (int64_t)k, 4096);
+    if ((int64_t)n_l <= 4 * want) return FSM_OK;
+    const int64_t step = std::max<int64_t>(1, (int64_t)n_l / 65536);
+    const int64_t ns = ((int64_t)n_l + step - 1) / step;
+    HIP_TRY(t->out_dist.ensure(sizeof(double) * (size_t)std::max<int64_t>(ns, want * 4)));
+    HIP_TRY(launch_gather_stride(t->stream, t->lb.as<double>(), (int64_t)n_l, step,
+                                 t->out_dist.as<double>()));
+    std::vector<double> smp((size_t)ns);
+    HIP_TRY(hipMemcpyAsync(smp.data(), t->out_dist.p, sizeof(double) * ns, hipMemcpyDeviceToHost,
+                           t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    const double sign = (P->metric == LCSS) ? -1.0 : 1.0;
+    for (double& x : smp) x *= sign;
+    const int64_t pick = std::min<int64_t>(ns - 1, std::max<int64_t>(1, want * ns / (int64_t)n_l));
+    std::nth_element(smp.begin(), smp.begin() + pick, smp.end());
+    const double tau = smp[(size_t)pick];
+    unsigned long long zero[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(C + 6, zero, sizeof(zero), hipMemcpyHostToDevice, t->stream));
+    HIP_TRY(launch_select_le(t->stream, t->list_b.as<int64_t>(), t->lb.as<double>(), C + 2, sign,
+                             tau, t->list_a.as<int64_t>(), C + 6));
+    unsigned long long nsel = 0;
+    HIP_TRY(hipMemcpyAsync(&nsel, C + 6, sizeof(nsel), hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    if ((int64_t)nsel < k || (int64_t)nsel > 64 * want) return FSM_OK;
+    HIP_TRY(t->out_dist.ensure(sizeof(double) * (size_t)std::max<unsigned long long>(nsel, 1)));
+    int rc = ensure_eld_scratch(t, P->metric, P->m, P->w);
+    if (rc) return rc;
+    VerifyArgs v = verify_args(t, G, Q, *P);
+    HIP_TRY(launch_eld(t->stream, v, t->list_a.as<int64_t>(), nullptr, C + 6, true, nullptr,
+                       t->out_dist.as<double>(), nullptr, C + 7, nullptr, t->scratch.as<double>(),
+                       eld_scratch_threads()));
+    std::vector<double> dist((size_t)nsel);
+    HIP_TRY(hipMemcpyAsync(dist.data(), t->out_dist.p, sizeof(double) * nsel,
+                           hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    if (evals) *evals = nsel;
+    std::vector<double> fin;
+    fin.reserve(dist.size());
+    for (double x : dist)
+        if (std::isfinite(x)) fin.push_back(x);
+    if ((int64_t)fin.size() < k) return FSM_OK;
+    std::nth_element(fin.begin(), fin.begin() + (k - 1), fin.end());
+    // widened by a relative 1e-9: a window whose distance equals the bound
+    // exactly must survive GLB, whose rounded sums can exceed the exact
+    // distance by an ulp where both are mathematically equal (w = 0)
+    const double xi1 = fin[(size_t)(k - 1)] * (1.0 + 1e-9) + 1e-300;
+    if (xi1 < P->cutoff) P->cutoff = xi1;
+    return FSM_OK;
+}
+
 // counters: [0] compacted, [1] valid, [2] after GLB, [3] ELD kept, [8] ELD work
 // queue, [9] ELD lane-iterations on live candidates.  The LB buffer is free once compaction has read it; GLB writes each
 // survivor's consumption bound there for the ELD abandon test.
+// cutoff mode (tighten_k > 0): when GLB leaves many candidates at the first
+// bound, the tighten_k-th smallest distance among the few thousand survivors
+// of smallest certified GLB bound is a tighter upper bound xi1 of the k-th
+// smallest distance; the survivors are filtered again by GLB at xi1 before
+// ELD.  *cutoff_used receives the bound ELD ran with, *extra_evals the
+// distances computed for it.
 int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
-                      const QueryParams& P, bool use_lb, bool use_glb) {
+                      const QueryParams& P0, bool use_lb, bool use_glb, int tighten_k = 0,
+                      double* cutoff_used = nullptr, uint64_t* extra_evals = nullptr) {
+    QueryParams P = P0;
+    if (cutoff_used) *cutoff_used = P.cutoff;
+    if (extra_evals) *extra_evals = 0;
     unsigned long long* C = t->counters.as<unsigned long long>();
     HIP_TRY(hipMemsetAsync(C, 0, sizeof(unsigned long long) * 16, t->stream));
     HIP_TRY(t->list_a.ensure(sizeof(int64_t) * (size_t)G.ts_n));
@@ -374,6 +447,22 @@ int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
         eld_in = t->list_b.as<int64_t>();
         eld_cnt = C + 2;
         eld_total = t->lb.as<double>();
+        if (tighten_k > 0) {
+            int rc = tighten_cutoff(t, G, Q, &P, tighten_k, extra_evals);
+            if (rc) return rc;
+            if (P.cutoff < P0.cutoff) {
+                // GLB again over the first survivors at the tighter bound
+                v = verify_args(t, G, Q, P);
+                HIP_TRY(t->aux.ensure(sizeof(double) * (size_t)G.ts_n));
+                HIP_TRY(hipMemsetAsync(C + 5, 0, sizeof(unsigned long long), t->stream));
+                HIP_TRY(launch_glb(t->stream, v, t->list_b.as<int64_t>(), C + 2,
+                                   t->list_a.as<int64_t>(), C + 5, t->aux.as<double>(), nullptr));
+                eld_in = t->list_a.as<int64_t>();
+                eld_cnt = C + 5;
+                eld_total = t->aux.as<double>();
+            }
+            if (cutoff_used) *cutoff_used = P.cutoff;
+        }
     }
     HIP_TRY(hipEventRecord(t->ev[4], t->stream));
     int rc = ensure_eld_scratch(t, P.metric, P.m, P.w);
@@ -475,7 +564,7 @@ void fsm_target_close(fsm_target* t) {
     DevBuf* bufs[] = {&t->MU, &t->SG, &t->INV, &t->Lr, &t->Ur, &t->lb,
                       &t->list_a, &t->list_b, &t->out_idx, &t->out_dist, &t->slab,
                       &t->rep_lb, &t->rep_idx, &t->tu, &t->tl, &t->qtab, &t->scratch,
-                      &t->counters};
+                      &t->counters, &t->aux};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : t->ev)
         if (e) (void)hipEventDestroy(e);
@@ -585,17 +674,21 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
             }
             P.cutoff = xi0;
         }
-        rc = run_filter_verify(t, G, Q, P, true, true);
+        uint64_t tight_evals = 0;
+        double used = P.cutoff;
+        rc = run_filter_verify(t, G, Q, P, true, true, d->mode == FSM_MODE_CUTOFF ? d->k : 0,
+                               &used, &tight_evals);
         if (rc) return bail(rc);
         HIP_TRY(hipStreamSynchronize(t->stream));
         unsigned long long C2[16] = {};
         HIP_TRY(hipMemcpy(C2, t->counters.p, sizeof(C2), hipMemcpyDeviceToHost));
         std::memcpy(C, C2, sizeof(C));
+        const bool tightened = d->mode == FSM_MODE_CUTOFF && used < P.cutoff;
         st.valid = C[1];
         st.invalid = (uint64_t)G.ts_n - C[1];
         st.after_lbip = C[0];
-        st.after_glb = C[2];
-        st.exact_evals = C[2] + rep_evals;
+        st.after_glb = tightened ? C[5] : C[2];
+        st.exact_evals = st.after_glb + rep_evals + tight_evals;
     }
     const uint64_t nres = C[3];
     R->idx.resize(nres);
@@ -882,7 +975,8 @@ uint64_t fsm_query_bytes(int64_t n, int metric, int m, int w) {
     const uint64_t blocks = (uint64_t)std::min<int64_t>(G.S, per > (1u << 20) ? 1024 : 2048);
     uint64_t b = 17 * rows                          // window mean, std, invalid flag
                  + 16 * (uint64_t)n                 // target envelope
-                 + 40 * (uint64_t)G.ts_n            // LB, two candidate lists, matches
+                 + 48 * (uint64_t)G.ts_n            // LB, two candidate lists, matches,
+                                                    // cutoff's second GLB bounds
                  + 32 * (uint64_t)G.S               // segment thresholds, representatives
                  + per * blocks                     // LB_IP per-block scratch
                  + 80 * (uint64_t)(m + 1) + 128;    // query tables, counters
@@ -896,7 +990,7 @@ void fsm_target_trim(fsm_target* t) {
     (void)hipStreamSynchronize(t->stream);
     DevBuf* bufs[] = {&t->MU, &t->SG, &t->INV, &t->Lr, &t->Ur, &t->lb, &t->list_a, &t->list_b,
                       &t->out_idx, &t->out_dist, &t->slab, &t->rep_lb, &t->rep_idx, &t->tu,
-                      &t->tl, &t->qtab, &t->scratch};
+                      &t->tl, &t->qtab, &t->scratch, &t->aux};
     for (DevBuf* b : bufs) b->release();
     t->stats_m = -1;
     t->env_w = -1;

@@ -226,6 +226,20 @@ def test_command_edge_cases(gpu, case):
             run_all_modes(t, q, 96, metric, 40, arg, 3)
 
 
+@pytest.mark.parametrize("metric,arg", METRICS)
+def test_cutoff_large_target(gpu, metric, arg):
+    """cutoff on 2^18 windows: enough GLB survivors at the first bound that
+    the engine tightens it before verifying; the ranking must still equal the
+    oracle's heap-based restatement bit for bit."""
+    t = make_target(29, 1 << 18, nans=5, runs=8)
+    q, _ = F.synth_query(29, t, 64, 0.3)
+    thr, ranks, rd, _ = O.cutoff(t, q, 64, 5, metric, 4, arg)
+    g = F.Target(t).cutoff(q, 64, 5, metric, 4, arg)
+    assert bits(g.threshold) == bits(thr)
+    assert np.array_equal(g.indices, ranks)
+    assert same_bits(g.distances, rd)
+
+
 def test_fewer_than_k(gpu):
     t = make_target(40, 600, nans=0, runs=0)
     t[:] = np.nan
