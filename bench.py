@@ -37,6 +37,10 @@ profiler run can launch the kernels in the timed configuration only.
 w = 5% of m, metric uniform over the five, cutoffs from k=10): the plain path
 (one fsm_query at a time) against fsm_query_batch at several concurrencies;
 every batched result is compared with the plain one.
+--workload precision: the 2^26 target at m = 128 and 1024 (w = 5% of m):
+LB_IP's pruning rate at the k=10 cutoff, match-set identity of search and
+exact, and LB <= ELD on a sample of windows (the error against an FP64 oracle
+is measured by tests/test_gpu_precision.py).
 --workload serve: the 2^26 target, Poisson arrivals at --rates queries/s of
 queries with m log-uniform in [64, 1024]: latency from submission to result
 through the serving loop (fsm_server_*) against the same arrivals served one
@@ -97,7 +101,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cutoff-cache", default=None)
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--workload", default="plain", choices=["plain", "batch", "serve"])
+    ap.add_argument("--workload", default="plain", choices=["plain", "batch", "serve", "precision"])
     ap.add_argument("--concurrency", default="4,8")
     ap.add_argument("--rates", default="50,200,1000")
     ap.add_argument("--requests", type=int, default=60, help="arrivals per rate (serve)")
@@ -116,6 +120,9 @@ def main():
         return
     if args.workload == "serve":
         run_serve(args)
+        return
+    if args.workload == "precision":
+        run_precision(args)
         return
 
     import fsmeld_amd as F
@@ -428,6 +435,44 @@ def run_serve(args):
                                f"{args.requests} Poisson arrivals per rate; {args.workers} workers, "
                                f"{budget >> 30} GiB budget"},
         "rates": rows}), flush=True)
+
+
+def run_precision(args):
+    import fsmeld_amd as F
+    from fsmeld_amd import workloads
+    t = workloads.plain_large()["target"]
+    T = F.Target(t)
+    rows = []
+    for m, w in ((128, 6), (1024, 51)):
+        for metric, arg in workloads.CONFIG2_METRICS:
+            q, _ = F.synth_query(3 * m + 1, t, m, 0.1)
+            c = T.cutoff(q, m, 10, metric, w, arg)
+            s = T.search(q, m, c.threshold, metric, w, arg)
+            e = T.exact(q, m, c.threshold, metric, w, arg)
+            extra = np.setdiff1d(s.indices, e.indices).size
+            missing = np.setdiff1d(e.indices, s.indices)
+            at_cut = bool(np.all(e.distances[np.isin(e.indices, missing)] == c.threshold))
+            lb, _, _ = T.lbip(q, m, metric, w, arg)
+            win = np.flatnonzero(np.isfinite(lb))[::65537]
+            d = T.eld(q, m, metric, w, arg, win)
+            viol = int(np.sum(lb[win] > d + 1e-9 * np.maximum(1.0, d)))
+            st = s.stats
+            rows.append(dict(m=m, w=w, metric=metric, cutoff=c.threshold,
+                             lbip_pruned_frac=round(1.0 - st["after_lbip"] / st["valid"], 6),
+                             glb_pruned_frac=round(1.0 - st["after_glb"] / st["valid"], 6),
+                             matches=int(st["matches"]),
+                             search_equals_exact=(extra == 0 and (missing.size == 0 or at_cut)),
+                             lb_above_eld=viol, lb_checked=int(win.size),
+                             search_ms=round(st["ms_total"], 2)))
+            log(f"m={m} {metric}: {rows[-1]}")
+    print(json.dumps({
+        "metric": "LB_IP precision at long queries (config 3): pruning, identity, admissibility",
+        "value": float(np.mean([r["lbip_pruned_frac"] for r in rows])), "unit": "fraction pruned",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f64 (LB_IP sums: exact 64-bit fixed point)",
+        "data": "synthetic",
+        "config": {"workload": "precision: plain large 2^26 target, m in {128, 1024}, w = 5% m, "
+                               "five metrics, cutoffs k=10"},
+        "rows": rows}), flush=True)
 
 
 if __name__ == "__main__":
