@@ -204,6 +204,90 @@ __device__ void bitonic_sort(uint64_t* keys, int n) {
     }
 }
 
+// Bitonic sort of n = kLbThreads * R keys with the keys in registers: thread t
+// holds keys [t R, t R + R).  Compare distances j < R stay inside a thread,
+// R <= j < 64 R pair lanes of one wave (one 64-bit shuffle per key), and only
+// j >= 64 R exchange through `keys` (LDS or global scratch) with a barrier.
+// Leaves the sorted keys in `keys`.
+template <int R, int J>
+__device__ __forceinline__ void bitonic_local(uint64_t* v, int k, int base) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int p = r ^ J;
+        if (p > r) {
+            const bool asc = ((base + r) & k) == 0;
+            const uint64_t a = v[r], b = v[p];
+            const bool sw = (a > b) == asc;
+            v[r] = sw ? b : a;
+            v[p] = sw ? a : b;
+        }
+    }
+}
+
+template <int R>
+__device__ void bitonic_sort_reg(uint64_t* keys, int n) {
+    const int t = threadIdx.x;
+    const int base = t * R;
+    uint64_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = keys[base + r];
+    for (int k = 2; k <= n; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64 * R) {
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < R; r++) keys[base + r] = v[r];
+                __threadfence_block();
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int i = base + r, p = i ^ j;
+                    const uint64_t u = keys[p];
+                    const bool keep_min = (((i & k) == 0) == (i < p));
+                    v[r] = keep_min ? (v[r] < u ? v[r] : u) : (v[r] > u ? v[r] : u);
+                }
+            } else if (j >= R) {
+                const int lm = j / R;
+                const bool lower = (t & lm) == 0;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const uint64_t u = __shfl_xor(v[r], lm, 64);
+                    const bool keep_min = (((base + r) & k) == 0) == lower;
+                    v[r] = keep_min ? (v[r] < u ? v[r] : u) : (v[r] > u ? v[r] : u);
+                }
+            } else {
+                switch (j) {
+                    case 1: bitonic_local<R, 1>(v, k, base); break;
+                    case 2: if (R > 2) bitonic_local<R, 2>(v, k, base); break;
+                    case 4: if (R > 4) bitonic_local<R, 4>(v, k, base); break;
+                    case 8: if (R > 8) bitonic_local<R, 8>(v, k, base); break;
+                    case 16: if (R > 16) bitonic_local<R, 16>(v, k, base); break;
+                    case 32: if (R > 32) bitonic_local<R, 32>(v, k, base); break;
+                    default: break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) keys[base + r] = v[r];
+    __threadfence_block();
+    __syncthreads();
+}
+
+// R = n / kLbThreads keys per thread in registers, or 0: the LDS / scratch sort
+template <int R>
+__device__ void sort_keys(uint64_t* keys, int n) {
+    if constexpr (R > 0) bitonic_sort_reg<R>(keys, n);
+    else bitonic_sort(keys, n);
+}
+
+// register-sort class of a padded key count (one kernel instantiation each)
+__host__ __device__ inline int sort_class(int npad) {
+    const int r = npad / kLbThreads;
+    return (npad % kLbThreads == 0 && (r == 2 || r == 4 || r == 8 || r == 16 || r == 32)) ? r : 0;
+}
+
 // Block argmax with "first index wins" among equal scores; NaN never wins.
 struct ArgBest {
     double v;
@@ -299,6 +383,7 @@ __device__ inline int64_t run_sum(const uint64_t* bm, const int64_t* CH, int j, 
 }
 
 // ------------------------------------------------------------------ kernel ---
+template <int R>
 __global__ void __launch_bounds__(kLbThreads) k_lbip(LbipArgs a) {
     extern __shared__ unsigned char lds_raw[];
     __shared__ ArgBest sh_best[kLbThreads / 64];
@@ -335,31 +420,48 @@ __global__ void __launch_bounds__(kLbThreads) k_lbip(LbipArgs a) {
         const int jfirst = tail ? G.tail_first : 0;
         const int64_t gbase = s * G.kml;
 
-        // 1. van Herk prefix (dir 0) / suffix (dir 1) scans over blocks of m windows
-        const int nb = (kml + m - 1) / m;
-        for (int item = threadIdx.x; item < 8 * nb; item += blockDim.x) {
-            const int st = item / (2 * nb);
-            const int rem = item - st * 2 * nb;
-            const int dir = rem / nb;
-            const int b = rem - dir * nb;
-            const double* src = (st < 2) ? MU : SG;
-            const bool is_min = (st & 1) == 0;
-            double* out = PQ + (size_t)(2 * st + dir) * kml;
-            const int j0 = b * m;
-            const int j1 = (j0 + m < kml) ? j0 + m : kml;
-            if (dir == 0) {
-                double acc = src[j0];
-                out[j0] = acc;
-                for (int j = j0 + 1; j < j1; j++) {
-                    acc = is_min ? nmin(acc, src[j]) : nmax(acc, src[j]);
-                    out[j] = acc;
-                }
-            } else {
-                double acc = src[j1 - 1];
-                out[j1 - 1] = acc;
-                for (int j = j1 - 2; j >= j0; j--) {
-                    acc = is_min ? nmin(acc, src[j]) : nmax(acc, src[j]);
-                    out[j] = acc;
+        // 1. van Herk prefix (dir 0) / suffix (dir 1) scans over blocks of m
+        //    windows: one wave per (statistic, direction, block), 64 windows
+        //    per step, min and max of the same statistic in one pass.  The
+        //    NaN-propagating min / max are associative, so the values equal a
+        //    sequential scan's.
+        {
+            const int nb = (kml + m - 1) / m;
+            const int lane = threadIdx.x & 63;
+            const double inf = __builtin_inf();
+            for (int item = threadIdx.x >> 6; item < 4 * nb; item += kLbThreads / 64) {
+                const int sg = item / (2 * nb);  // 0: mean, 1: std
+                const int rem = item - sg * 2 * nb;
+                const int dir = rem / nb;
+                const int b = rem - dir * nb;
+                const double* src = sg ? SG : MU;
+                double* omin = PQ + (size_t)(2 * (2 * sg) + dir) * kml;
+                double* omax = PQ + (size_t)(2 * (2 * sg + 1) + dir) * kml;
+                const int j0 = b * m;
+                const int j1 = (j0 + m < kml) ? j0 + m : kml;
+                double cmin = inf, cmax = -inf;
+                for (int c = 0; c < j1 - j0; c += 64) {
+                    // prefix: ascending windows; suffix: descending windows
+                    const int j = (dir == 0) ? j0 + c + lane : j1 - 1 - c - lane;
+                    const bool in = (dir == 0) ? (j < j1) : (j >= j0);
+                    const double v = in ? src[j] : 0.0;
+                    double vmin = in ? v : inf, vmax = in ? v : -inf;
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const double pmin = __shfl_up(vmin, o, 64);
+                        const double pmax = __shfl_up(vmax, o, 64);
+                        if (lane >= o) {
+                            vmin = nmin(pmin, vmin);
+                            vmax = nmax(pmax, vmax);
+                        }
+                    }
+                    vmin = nmin(cmin, vmin);
+                    vmax = nmax(cmax, vmax);
+                    if (in) {
+                        omin[j] = vmin;
+                        omax[j] = vmax;
+                    }
+                    cmin = __shfl(vmin, 63, 64);
+                    cmax = __shfl(vmax, 63, 64);
                 }
             }
         }
@@ -402,7 +504,7 @@ __global__ void __launch_bounds__(kLbThreads) k_lbip(LbipArgs a) {
         __syncthreads();
 
         // 4a. upper threshold: ascending order, G = rank + 1
-        bitonic_sort(keys, npad);
+        sort_keys<R>(keys, npad);
         double t_u;
         {
             int cnt = 0;
@@ -438,7 +540,7 @@ __global__ void __launch_bounds__(kLbThreads) k_lbip(LbipArgs a) {
         for (int p = nthr + threadIdx.x; p < npad; p += blockDim.x) keys[p] = 0ull;
         __threadfence_block();
         __syncthreads();
-        bitonic_sort(keys, npad);
+        sort_keys<R>(keys, npad);
         double t_l;
         {
             int cnt = 0;
@@ -565,7 +667,16 @@ __global__ void __launch_bounds__(kLbThreads) k_lbip(LbipArgs a) {
 
 hipError_t launch_lbip(hipStream_t st, const LbipArgs& a) {
     const size_t lds = lbip_lds_bytes(a.G);
-    hipLaunchKernelGGL(k_lbip, dim3((unsigned)a.blocks), dim3(kLbThreads), lds, st, a);
+    const int npad = npad_of(a.G.m_k - 2 * a.G.m + 2);
+    const dim3 g((unsigned)a.blocks), b(kLbThreads);
+    switch (sort_class(npad)) {
+        case 2: hipLaunchKernelGGL(k_lbip<2>, g, b, lds, st, a); break;
+        case 4: hipLaunchKernelGGL(k_lbip<4>, g, b, lds, st, a); break;
+        case 8: hipLaunchKernelGGL(k_lbip<8>, g, b, lds, st, a); break;
+        case 16: hipLaunchKernelGGL(k_lbip<16>, g, b, lds, st, a); break;
+        case 32: hipLaunchKernelGGL(k_lbip<32>, g, b, lds, st, a); break;
+        default: hipLaunchKernelGGL(k_lbip<0>, g, b, lds, st, a); break;
+    }
     return hipGetLastError();
 }
 
