@@ -115,7 +115,8 @@ def test_glb_filter_decisions(gpu, small, metric, arg, m, w):
 
 
 @pytest.mark.parametrize("metric,arg", METRICS)
-@pytest.mark.parametrize("m,w", [(64, 4), (64, 12), (100, 30), (80, 36), (160, 70)])
+@pytest.mark.parametrize("m,w", [(64, 0), (48, 1), (64, 4), (64, 12), (96, 27), (100, 30), (80, 36),
+                                 (160, 70)])
 def test_exact_command_abandon_is_safe(gpu, small, metric, arg, m, w):
     """The exact command abandons windows by DP frontier + suffix bound; with
     the cutoff set exactly on windows' distances it must return precisely the
