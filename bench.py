@@ -63,6 +63,17 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 ELD_OPS_PER_CELL = {"dtw": 5, "erp": 9, "edr": 7, "lcss": 6, "msm": 12}
 METRIC_CODE = {"dtw": 1, "edr": 2, "erp": 3, "lcss": 4, "msm": 5}
+
+
+def eld_kernel_name(code, m, w):
+    """The ELD kernel the engine launches for (m, w): the lane-per-candidate
+    register band for w <= 25, one wave per candidate up to w = 127, the row
+    band beyond (or where the query staging exceeds 64 KiB of LDS)."""
+    if w <= 25 and 24 * (m + 80) <= 65536:
+        return f"k_eld<{code},{w}>"
+    if w <= 127 and 16 * (m + 2) <= 65536:
+        return f"k_eld_wave<{code},{1 if w <= 63 else 2}>"
+    return f"k_eld_rows<{code}>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
@@ -209,7 +220,7 @@ def main():
         cells = pq["after_glb"] * float(band)
         flops = cells * ELD_OPS_PER_CELL[metric]
         done = per_metric[metric]["eld_cells"] / steps
-        kernels.append(dict(kernel=f"k_eld<{code},{w},true>", metric=metric, bound="mfma",
+        kernels.append(dict(kernel=eld_kernel_name(code, m, w), metric=metric, bound="mfma",
                             launches=steps, avg_launch_ms=round(e_ms, 4),
                             total_ms=round(per_metric[metric]["eld"], 3), work_per_launch=flops,
                             achieved=flops / (e_ms / 1e3) / 1e12, peak=FP64_PEAK_TFLOPS,
