@@ -24,7 +24,6 @@
 
 namespace fsm {
 
-constexpr int kLbThreads = 256;
 constexpr int kLdsKeys = 8192;
 
 __host__ __device__ inline int npad_of(int n) {
@@ -32,6 +31,11 @@ __host__ __device__ inline int npad_of(int n) {
     while (p < n) p <<= 1;
     return p;
 }
+
+// Workgroup size per segment: 256 threads up to 4096 keys, 1024 beyond, so
+// that every sort up to 16384 keys runs in registers with at most 16 keys per
+// thread, and long segments spread their per-position work over more waves.
+inline int lbip_threads(int npad) { return npad <= 4096 ? 256 : 1024; }
 
 size_t lbip_slab_bytes(const Geometry& G) {
     const int nthr = G.m_k - 2 * G.m + 2;
@@ -204,7 +208,7 @@ __device__ void bitonic_sort(uint64_t* keys, int n) {
     }
 }
 
-// Bitonic sort of n = kLbThreads * R keys with the keys in registers: thread t
+// Bitonic sort of n = blockDim.x * R keys with the keys in registers: thread t
 // holds keys [t R, t R + R).  Compare distances j < R stay inside a thread,
 // R <= j < 64 R pair lanes of one wave (one 64-bit shuffle per key), and only
 // j >= 64 R exchange through `keys` (LDS or global scratch) with a barrier.
@@ -275,17 +279,18 @@ __device__ void bitonic_sort_reg(uint64_t* keys, int n) {
     __syncthreads();
 }
 
-// R = n / kLbThreads keys per thread in registers, or 0: the LDS / scratch sort
+// R = n / blockDim.x keys per thread in registers, or 0: the LDS / scratch sort
 template <int R>
 __device__ void sort_keys(uint64_t* keys, int n) {
     if constexpr (R > 0) bitonic_sort_reg<R>(keys, n);
     else bitonic_sort(keys, n);
 }
 
-// register-sort class of a padded key count (one kernel instantiation each)
-__host__ __device__ inline int sort_class(int npad) {
-    const int r = npad / kLbThreads;
-    return (npad % kLbThreads == 0 && (r == 2 || r == 4 || r == 8 || r == 16 || r == 32)) ? r : 0;
+// register-sort class of a padded key count for nt threads (one kernel
+// instantiation each): keys per thread, 0 for the LDS / scratch sort
+inline int sort_class(int npad, int nt) {
+    const int r = npad / nt;
+    return (npad % nt == 0 && (r == 2 || r == 4 || r == 8 || r == 16)) ? r : 0;
 }
 
 // Block argmax with "first index wins" among equal scores; NaN never wins.
@@ -383,13 +388,13 @@ __device__ inline int64_t run_sum(const uint64_t* bm, const int64_t* CH, int j, 
 }
 
 // ------------------------------------------------------------------ kernel ---
-template <int R>
-__global__ void __launch_bounds__(kLbThreads) k_lbip(LbipArgs a) {
+template <int R, int NT>
+__global__ void __launch_bounds__(NT) k_lbip(LbipArgs a) {
     extern __shared__ unsigned char lds_raw[];
-    __shared__ ArgBest sh_best[kLbThreads / 64];
-    __shared__ double sh_sum[kLbThreads / 64];
-    __shared__ int64_t sh_scan[kLbThreads / 64];
-    __shared__ int sh_cnt[kLbThreads / 64];
+    __shared__ ArgBest sh_best[NT / 64];
+    __shared__ double sh_sum[NT / 64];
+    __shared__ int64_t sh_scan[NT / 64];
+    __shared__ int sh_cnt[NT / 64];
 
     const Geometry G = a.G;
     const QueryTables Q = a.q;
@@ -429,7 +434,7 @@ __global__ void __launch_bounds__(kLbThreads) k_lbip(LbipArgs a) {
             const int nb = (kml + m - 1) / m;
             const int lane = threadIdx.x & 63;
             const double inf = __builtin_inf();
-            for (int item = threadIdx.x >> 6; item < 4 * nb; item += kLbThreads / 64) {
+            for (int item = threadIdx.x >> 6; item < 4 * nb; item += NT / 64) {
                 const int sg = item / (2 * nb);  // 0: mean, 1: std
                 const int rem = item - sg * 2 * nb;
                 const int dir = rem / nb;
@@ -513,7 +518,7 @@ __global__ void __launch_bounds__(kLbThreads) k_lbip(LbipArgs a) {
             if ((threadIdx.x & 63) == 0) sh_cnt[threadIdx.x >> 6] = cnt;
             __syncthreads();
             int nval = 0;
-            for (int k = 0; k < kLbThreads / 64; k++) nval += sh_cnt[k];
+            for (int k = 0; k < NT / 64; k++) nval += sh_cnt[k];
             __syncthreads();
             ArgBest best{__builtin_nan(""), 0};
             for (int p = threadIdx.x; p < nval; p += blockDim.x) {
@@ -549,7 +554,7 @@ __global__ void __launch_bounds__(kLbThreads) k_lbip(LbipArgs a) {
             if ((threadIdx.x & 63) == 0) sh_cnt[threadIdx.x >> 6] = cnt;
             __syncthreads();
             int nval = 0;
-            for (int k = 0; k < kLbThreads / 64; k++) nval += sh_cnt[k];
+            for (int k = 0; k < NT / 64; k++) nval += sh_cnt[k];
             __syncthreads();
             ArgBest best{__builtin_nan(""), 0};
             for (int r = threadIdx.x; r < nval; r += blockDim.x) {
@@ -585,7 +590,7 @@ __global__ void __launch_bounds__(kLbThreads) k_lbip(LbipArgs a) {
             if (F > 1000) F = 1000;
         }
         {
-            const int per = (m + kLbThreads - 1) / kLbThreads;
+            const int per = (m + NT - 1) / NT;
             const int i0 = threadIdx.x * per;
             const int i1 = (i0 + per < m) ? i0 + per : m;
             int64_t tl_sum = 0, tu_sum = 0;
@@ -619,7 +624,7 @@ __global__ void __launch_bounds__(kLbThreads) k_lbip(LbipArgs a) {
         // 6. mask bitmaps: g_l = L_n >= t_l, g_u = U_n <= t_u
         {
             const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-            for (int wi = wid; wi < nw; wi += kLbThreads / 64) {
+            for (int wi = wid; wi < nw; wi += NT / 64) {
                 const int pos = wi * 64 + lane;
                 const bool in = pos < mk;
                 const bool gl = in && (Ln[pos] >= t_l);
@@ -668,15 +673,25 @@ __global__ void __launch_bounds__(kLbThreads) k_lbip(LbipArgs a) {
 hipError_t launch_lbip(hipStream_t st, const LbipArgs& a) {
     const size_t lds = lbip_lds_bytes(a.G);
     const int npad = npad_of(a.G.m_k - 2 * a.G.m + 2);
-    const dim3 g((unsigned)a.blocks), b(kLbThreads);
-    switch (sort_class(npad)) {
-        case 2: hipLaunchKernelGGL(k_lbip<2>, g, b, lds, st, a); break;
-        case 4: hipLaunchKernelGGL(k_lbip<4>, g, b, lds, st, a); break;
-        case 8: hipLaunchKernelGGL(k_lbip<8>, g, b, lds, st, a); break;
-        case 16: hipLaunchKernelGGL(k_lbip<16>, g, b, lds, st, a); break;
-        case 32: hipLaunchKernelGGL(k_lbip<32>, g, b, lds, st, a); break;
-        default: hipLaunchKernelGGL(k_lbip<0>, g, b, lds, st, a); break;
+    const int nt = lbip_threads(npad);
+    const dim3 g((unsigned)a.blocks), b(nt);
+#define FSM_LB(RR, TT) hipLaunchKernelGGL((k_lbip<RR, TT>), g, b, lds, st, a)
+    if (nt == 256) {
+        switch (sort_class(npad, 256)) {
+            case 2: FSM_LB(2, 256); break;
+            case 4: FSM_LB(4, 256); break;
+            case 8: FSM_LB(8, 256); break;
+            case 16: FSM_LB(16, 256); break;
+            default: FSM_LB(0, 256); break;
+        }
+    } else {
+        switch (sort_class(npad, 1024)) {
+            case 8: FSM_LB(8, 1024); break;
+            case 16: FSM_LB(16, 1024); break;
+            default: FSM_LB(0, 1024); break;
+        }
     }
+#undef FSM_LB
     return hipGetLastError();
 }
 

@@ -116,7 +116,7 @@ def test_glb_filter_decisions(gpu, small, metric, arg, m, w):
 
 @pytest.mark.parametrize("metric,arg", METRICS)
 @pytest.mark.parametrize("m,w", [(64, 0), (48, 1), (64, 4), (64, 12), (96, 27), (100, 30), (80, 36),
-                                 (160, 70)])
+                                 (160, 70), (300, 130)])
 def test_exact_command_abandon_is_safe(gpu, small, metric, arg, m, w):
     """The exact command abandons windows by DP frontier + suffix bound; with
     the cutoff set exactly on windows' distances it must return precisely the
@@ -135,10 +135,13 @@ def test_exact_command_abandon_is_safe(gpu, small, metric, arg, m, w):
 
 
 @pytest.mark.parametrize("metric,arg", METRICS)
-def test_lbip_values(gpu, metric, arg):
-    t = make_target(8, 1 << 14, nans=0, runs=0, spikes=0.0)  # NaN-free: deques exact
+@pytest.mark.parametrize("m,w,logn", [(32, 3, 14), (512, 25, 17), (1024, 51, 18), (2048, 102, 19)])
+def test_lbip_values(gpu, metric, arg, m, w, logn):
+    """One case per LB_IP kernel shape: 256 threads with the keys of a segment
+    in the LDS sort (m = 32) or 16 per thread in registers (m = 512), 1024
+    threads with 8 (m = 1024) or 16 (m = 2048) keys per thread."""
+    t = make_target(8, 1 << logn, nans=0, runs=0, spikes=0.0)  # NaN-free: deques exact
     T = F.Target(t)
-    m, w = 32, 3
     q, _ = F.synth_query(8, t, m, 0.1)
     lb, tu, tl = T.lbip(q, m, metric, w, arg)
     olb, otu, otl = O.lbip_all(t, q, m, metric, w, arg)
