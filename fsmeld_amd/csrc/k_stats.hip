@@ -4,33 +4,38 @@
 // Window statistics must equal the reference's bits: every window's mean and
 // std come from the running-sum recurrence of the macro-segment that
 // evaluates it (src/mvmean.cpp:9-46), so one lane owns one segment and runs
-// that recurrence in the reference's order.  Build with -ffp-contract=off.
+// that recurrence in the reference's order; the per-window division and
+// square root run afterwards in a parallel pass.  Build with -ffp-contract=off.
+#include <algorithm>
+
 #include "engine_internal.h"
 
 namespace fsm {
 
-// One lane per macro-segment (src/mvmean.cpp:9-46 and the invalid test of
-// src/fsmeld.cpp:110-129).  Outputs are segment-major [S][kml].
+// One lane per macro-segment runs the running-sum recurrence of
+// src/mvmean.cpp:9-46 in the reference's order (its order fixes the bits);
+// only the sums and the non-finite count are loop-carried, so this pass
+// stores them (sum1 in MU, sum2 in SG, count > 0 in INV) and k_window_finish
+// turns every window's sums into mean / std / invalid flag in parallel
+// (src/mvmean.cpp:30-44, the invalid test of src/fsmeld.cpp:110-129), with
+// the same expressions.  Outputs are segment-major [S][kml].
 constexpr int kStatsU = 16;
 
-__global__ void __launch_bounds__(64) k_window_stats(const double* __restrict__ T, Geometry G,
-                                                     double* __restrict__ MU,
-                                                     double* __restrict__ SG,
-                                                     uint8_t* __restrict__ INV) {
+__global__ void __launch_bounds__(64) k_window_sums(const double* __restrict__ T, Geometry G,
+                                                    double* __restrict__ MU,
+                                                    double* __restrict__ SG,
+                                                    uint8_t* __restrict__ INV) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= G.S) return;
     const double* a = (s < G.forsize) ? T + s * G.kml : T + (G.N - G.m_k);
-    double* mu = MU + s * G.kml;
-    double* si = SG + s * G.kml;
-    uint8_t* inv = INV + s * G.kml;
+    double* s1o = MU + s * G.kml;
+    double* s2o = SG + s * G.kml;
+    uint8_t* bo = INV + s * G.kml;
     const int l = G.m;
-    const double dl = (double)l;
-    const double nan = __builtin_nan("");
     double sum1 = 0.0, sum2 = 0.0;
     int bad = 0;
-    // The recurrences are sequential per segment (their order fixes the bits);
     // loads are issued kStatsU at a time ahead of the chain so that each lane
-    // keeps many in flight instead of waiting on one per sample.
+    // keeps many in flight instead of waiting on one per sample
     auto first = [&](double v) {
         if (__builtin_isfinite(v)) {
             sum1 += v;
@@ -48,13 +53,9 @@ __global__ void __launch_bounds__(64) k_window_stats(const double* __restrict__ 
         for (int u = 0; u < kStatsU; u++) first(v[u]);
     }
     for (; i0 < l; i0++) first(a[i0]);
-    {
-        const double m0 = bad ? nan : sum1 / dl;
-        const double s0 = bad ? nan : sqrt(sum2 / dl - m0 * m0);
-        mu[0] = m0;
-        si[0] = s0;
-        inv[0] = (bad > 0) || !__builtin_isfinite(s0) || s0 <= 1e-12;
-    }
+    s1o[0] = sum1;
+    s2o[0] = sum2;
+    bo[0] = bad > 0;
     const int64_t kml = G.kml;
     auto step = [&](int64_t i, double vo, double vi) {
         const bool of = __builtin_isfinite(vo);
@@ -64,11 +65,9 @@ __global__ void __launch_bounds__(64) k_window_stats(const double* __restrict__ 
         bad += (int)!nf - (int)!of;
         sum1 = sum1 - out + in;
         sum2 = sum2 - out * out + in * in;
-        const double mi = bad ? nan : sum1 / dl;
-        const double sg = bad ? nan : sqrt(sum2 / dl - mi * mi);
-        mu[i] = mi;
-        si[i] = sg;
-        inv[i] = (bad > 0) || !__builtin_isfinite(sg) || sg <= 1e-12;
+        s1o[i] = sum1;
+        s2o[i] = sum2;
+        bo[i] = bad > 0;
     };
     int64_t i = 1;
     for (; i + kStatsU <= kml; i += kStatsU) {
@@ -84,12 +83,32 @@ __global__ void __launch_bounds__(64) k_window_stats(const double* __restrict__ 
     for (; i < kml; i++) step(i, a[i - 1], a[i + l - 1]);
 }
 
+// mean = sum1 / m, std = sqrt(sum2 / m - mean^2), NaN when the window holds a
+// non-finite sample; invalid when so, or std is not finite or <= 1e-12
+__global__ void __launch_bounds__(256) k_window_finish(int64_t n, int m, double* __restrict__ MU,
+                                                       double* __restrict__ SG,
+                                                       uint8_t* __restrict__ INV) {
+    const double dl = (double)m;
+    const double nan = __builtin_nan("");
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const bool bad = INV[i] != 0;
+        const double mi = bad ? nan : MU[i] / dl;
+        const double sg = bad ? nan : sqrt(SG[i] / dl - mi * mi);
+        MU[i] = mi;
+        SG[i] = sg;
+        INV[i] = bad || !__builtin_isfinite(sg) || sg <= 1e-12;
+    }
+}
+
 hipError_t launch_window_stats(hipStream_t st, const double* T, const Geometry& G, double* MU,
                                double* SG, uint8_t* INV) {
     const int bs = 64;
     const int64_t blocks = (G.S + bs - 1) / bs;
-    hipLaunchKernelGGL(k_window_stats, dim3((unsigned)blocks), dim3(bs), 0, st, T, G, MU, SG,
-                       INV);
+    hipLaunchKernelGGL(k_window_sums, dim3((unsigned)blocks), dim3(bs), 0, st, T, G, MU, SG, INV);
+    const int64_t n = G.S * G.kml;
+    const int64_t fb = std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_window_finish, dim3((unsigned)fb), dim3(256), 0, st, n, G.m, MU, SG, INV);
     return hipGetLastError();
 }
 
