@@ -181,12 +181,14 @@ __device__ inline void penalties(int metric, double q, double tl, double tu, dou
 // std::min(C, x) of the reference returns C when equal; fmin returns the
 // same value, and penalties are never NaN for finite t.
 
-__device__ inline double stat_range(const double* P, const double* Q, int64_t lo, int64_t hi,
-                                    int m, bool is_min) {
+// min / max of a window statistic over windows [lo, hi] (hi - lo < m) from
+// the van Herk prefix value at hi (ph) and suffix value at lo (sl)
+__device__ inline double stat_range(double ph, double sl, int64_t lo, int64_t hi, int m,
+                                    bool is_min) {
     const int64_t blo = lo / m, bhi = hi / m;
-    if (blo != bhi) return is_min ? nmin(Q[lo], P[hi]) : nmax(Q[lo], P[hi]);
-    if (lo == blo * m) return P[hi];
-    return Q[lo];
+    if (blo != bhi) return is_min ? nmin(sl, ph) : nmax(sl, ph);
+    if (lo == blo * m) return ph;
+    return sl;
 }
 
 __device__ void bitonic_sort(uint64_t* keys, int n) {
@@ -445,37 +447,75 @@ __global__ void __launch_bounds__(NT) k_lbip(LbipArgs a) {
                 const int j0 = b * m;
                 const int j1 = (j0 + m < kml) ? j0 + m : kml;
                 double cmin = inf, cmax = -inf;
-                for (int c = 0; c < j1 - j0; c += 64) {
-                    // prefix: ascending windows; suffix: descending windows
-                    const int j = (dir == 0) ? j0 + c + lane : j1 - 1 - c - lane;
-                    const bool in = (dir == 0) ? (j < j1) : (j >= j0);
-                    const double v = in ? src[j] : 0.0;
-                    double vmin = in ? v : inf, vmax = in ? v : -inf;
-                    for (int o = 1; o < 64; o <<= 1) {
-                        const double pmin = __shfl_up(vmin, o, 64);
-                        const double pmax = __shfl_up(vmax, o, 64);
-                        if (lane >= o) {
-                            vmin = nmin(pmin, vmin);
-                            vmax = nmax(pmax, vmax);
+                // the loads of up to kScanG chunks are issued before the
+                // chunk-to-chunk carry chain that consumes them
+                constexpr int kScanG = 4;
+                for (int c0 = 0; c0 < j1 - j0; c0 += 64 * kScanG) {
+                    double v[kScanG];
+#pragma unroll
+                    for (int u = 0; u < kScanG; u++) {
+                        const int c = c0 + 64 * u;
+                        // prefix: ascending windows; suffix: descending windows
+                        const int j = (dir == 0) ? j0 + c + lane : j1 - 1 - c - lane;
+                        const bool in = (dir == 0) ? (j < j1) : (j >= j0);
+                        v[u] = in ? src[j] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kScanG; u++) {
+                        const int c = c0 + 64 * u;
+                        if (c >= j1 - j0) break;
+                        const int j = (dir == 0) ? j0 + c + lane : j1 - 1 - c - lane;
+                        const bool in = (dir == 0) ? (j < j1) : (j >= j0);
+                        double vmin = in ? v[u] : inf, vmax = in ? v[u] : -inf;
+                        for (int o = 1; o < 64; o <<= 1) {
+                            const double pmin = __shfl_up(vmin, o, 64);
+                            const double pmax = __shfl_up(vmax, o, 64);
+                            if (lane >= o) {
+                                vmin = nmin(pmin, vmin);
+                                vmax = nmax(pmax, vmax);
+                            }
                         }
+                        vmin = nmin(cmin, vmin);
+                        vmax = nmax(cmax, vmax);
+                        if (in) {
+                            omin[j] = vmin;
+                            omax[j] = vmax;
+                        }
+                        cmin = __shfl(vmin, 63, 64);
+                        cmax = __shfl(vmax, 63, 64);
                     }
-                    vmin = nmin(cmin, vmin);
-                    vmax = nmax(cmax, vmax);
-                    if (in) {
-                        omin[j] = vmin;
-                        omax[j] = vmax;
-                    }
-                    cmin = __shfl(vmin, 63, 64);
-                    cmax = __shfl(vmax, 63, 64);
                 }
             }
         }
         __threadfence_block();
         __syncthreads();
 
-        // 2-3. envelopes, stat ranges, normalised envelopes; upper keys
-        for (int i = threadIdx.x; i < mk; i += blockDim.x) {
-            double Lv, Uv;
+        // 2-3. envelopes, stat ranges, normalised envelopes; upper keys.
+        //     Each thread takes kEnvG positions at a time and issues their
+        //     loads (envelope, the eight prefix / suffix values) first.
+        constexpr int kEnvG = 2;
+        for (int i0 = threadIdx.x; i0 < mk; i0 += kEnvG * NT) {
+            double Lg[kEnvG], Ug[kEnvG], PS[kEnvG][8];
+#pragma unroll
+            for (int e = 0; e < kEnvG; e++) {
+                const int i = i0 + e * NT;
+                if (i >= mk) break;
+                const bool edge = i < w || i >= mk - w;
+                Lg[e] = edge ? 0.0 : a.Lraw[seg0 + i];
+                Ug[e] = edge ? 0.0 : a.Uraw[seg0 + i];
+                const int64_t lo = i - m + 1 > 0 ? i - m + 1 : 0;
+                const int64_t hi = i < kml - 1 ? i : kml - 1;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    PS[e][2 * q] = PQ[(size_t)(2 * q) * kml + hi];          // prefix at hi
+                    PS[e][2 * q + 1] = PQ[(size_t)(2 * q + 1) * kml + lo];  // suffix at lo
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < kEnvG; e++) {
+            const int i = i0 + e * NT;
+            if (i >= mk) break;
+            double Lv = Lg[e], Uv = Ug[e];
             if (i < w || i >= mk - w) {
                 const int lo = i - w > 0 ? i - w : 0;
                 const int hi = i + w < mk - 1 ? i + w : mk - 1;
@@ -485,16 +525,13 @@ __global__ void __launch_bounds__(NT) k_lbip(LbipArgs a) {
                     Lv = nmin(Lv, t[x]);
                     Uv = nmax(Uv, t[x]);
                 }
-            } else {
-                Lv = a.Lraw[seg0 + i];
-                Uv = a.Uraw[seg0 + i];
             }
             const int64_t lo = i - m + 1 > 0 ? i - m + 1 : 0;
             const int64_t hi = i < kml - 1 ? i : kml - 1;
-            const double mumin = stat_range(PQ + 0 * kml, PQ + 1 * (size_t)kml, lo, hi, m, true);
-            const double mumax = stat_range(PQ + 2 * (size_t)kml, PQ + 3 * (size_t)kml, lo, hi, m, false);
-            const double sgmin = stat_range(PQ + 4 * (size_t)kml, PQ + 5 * (size_t)kml, lo, hi, m, true);
-            const double sgmax = stat_range(PQ + 6 * (size_t)kml, PQ + 7 * (size_t)kml, lo, hi, m, false);
+            const double mumin = stat_range(PS[e][0], PS[e][1], lo, hi, m, true);
+            const double mumax = stat_range(PS[e][2], PS[e][3], lo, hi, m, false);
+            const double sgmin = stat_range(PS[e][4], PS[e][5], lo, hi, m, true);
+            const double sgmax = stat_range(PS[e][6], PS[e][7], lo, hi, m, false);
             const double nu = Uv - mumin;
             const double un = (nu >= 0) ? nu / sgmin : nu / sgmax;
             const double nl = Lv - mumax;
@@ -503,6 +540,7 @@ __global__ void __launch_bounds__(NT) k_lbip(LbipArgs a) {
             Ln[i] = ln;
             if (i >= m - 1 && i <= mk - m)
                 keys[i - (m - 1)] = (un != un) ? ~0ull : key_of(un);
+            }
         }
         for (int p = nthr + threadIdx.x; p < npad; p += blockDim.x) keys[p] = ~0ull;
         __threadfence_block();
