@@ -230,6 +230,33 @@ def test_command_edge_cases(gpu, case):
             run_all_modes(t, q, 96, metric, 40, arg, 3)
 
 
+@pytest.mark.parametrize("metric,arg", [("dtw", 0.0), ("edr", 0.25), ("msm", 0.5)])
+@pytest.mark.parametrize("m,w", [(3000, 40), (4100, 8), (2600, 130)])
+def test_long_queries_search_equals_exact(gpu, metric, arg, m, w):
+    """Queries past the register sort (2^15 keys per segment: the LDS / scratch
+    bitonic sort with 1024 threads) and past the register ELD band (wave and
+    row-band kernels): search returns exactly the exact scan's windows with the
+    same distance bits, cutoff's k-th distance bounds them, and no LB_IP bound
+    exceeds its window's exact distance."""
+    t = make_target(50 + m % 7, 1 << 16, nans=2, runs=3)
+    q, _ = F.synth_query(51, t, m, 0.1)
+    T = F.Target(t)
+    c = T.cutoff(q, m, 5, metric, w, arg)
+    s = T.search(q, m, c.threshold, metric, w, arg)
+    e = T.exact(q, m, c.threshold, metric, w, arg)
+    assert s.indices.size >= 5
+    assert np.all(np.isin(c.indices, e.indices))
+    missing = np.setdiff1d(e.indices, s.indices)
+    assert np.setdiff1d(s.indices, e.indices).size == 0
+    assert np.all(e.distances[np.isin(e.indices, missing)] == c.threshold)
+    assert same_bits(s.distances, e.distances[np.isin(e.indices, s.indices)])
+    lb, _, _ = T.lbip(q, m, metric, w, arg)
+    _, _, inv = O.window_stats(t, m)
+    win = np.flatnonzero(~inv)[::211]
+    d = T.eld(q, m, metric, w, arg, win)
+    assert np.all(~(lb[win] > d + 1e-9 * np.maximum(1.0, d)))
+
+
 @pytest.mark.parametrize("metric,arg", METRICS)
 def test_cutoff_large_target(gpu, metric, arg):
     """cutoff on 2^18 windows: enough GLB survivors at the first bound that
