@@ -321,7 +321,7 @@ __device__ inline void wave_append(bool keep, int64_t g, double d, int64_t* out_
 // stays in registers (two anti-diagonals and the candidate ring) fits half
 // the register file.
 template <int METRIC, int W>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W <= 25 ? 2 : 1)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_eld(VerifyArgs a, const double* __restrict__ LQ,
                                              const double* __restrict__ UQ,
                                              const int64_t* __restrict__ in,
@@ -794,7 +794,7 @@ inline size_t wave_lds_bytes(int m) { return sizeof(double) * 2 * (size_t)(m + 2
 
 // register band: w <= 32; one wave per candidate: w <= 127; the row-band path
 // beyond, or where the LDS staging does not fit
-constexpr int kRegMaxW = 25;
+constexpr int kRegMaxW = 25;  // the launcher instantiates k_eld for W = 0 .. kRegMaxW
 inline bool uses_register_band(int metric, int m, int w) {
     (void)metric;
     return reg_lds_bytes(m) <= kEldLdsCap && w <= kRegMaxW;
@@ -848,8 +848,7 @@ hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64_t* in,
         FSM_ELD(0) FSM_ELD(1) FSM_ELD(2) FSM_ELD(3) FSM_ELD(4) FSM_ELD(5) FSM_ELD(6) FSM_ELD(7)
         FSM_ELD(8) FSM_ELD(9) FSM_ELD(10) FSM_ELD(11) FSM_ELD(12) FSM_ELD(13) FSM_ELD(14)
         FSM_ELD(15) FSM_ELD(16) FSM_ELD(17) FSM_ELD(18) FSM_ELD(19) FSM_ELD(20) FSM_ELD(21)
-        FSM_ELD(22) FSM_ELD(23) FSM_ELD(24) FSM_ELD(25) FSM_ELD(26) FSM_ELD(27) FSM_ELD(28)
-        FSM_ELD(29) FSM_ELD(30) FSM_ELD(31) FSM_ELD(32)
+        FSM_ELD(22) FSM_ELD(23) FSM_ELD(24) FSM_ELD(25)
 #undef FSM_ELD
         default: return hipErrorInvalidValue;
     }
