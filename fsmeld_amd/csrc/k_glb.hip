@@ -303,7 +303,7 @@ __device__ inline void fast_term(double e, double xg2, double arg, double eta, d
 // envelope terms (DTW / ERP / MSM), the count of certain mismatches (EDR),
 // or an upper bound on the samples that can match at all (LCSS).
 template <int METRIC>
-__device__ int glb_fast(const VerifyArgs& A, const double* __restrict__ Q,
+__device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __restrict__ Q,
                         const double* __restrict__ LQ, const double* __restrict__ UQ,
                         const double* __restrict__ XG, const double* __restrict__ t,
                         const double* __restrict__ Lr, const double* __restrict__ Ur,
@@ -462,7 +462,7 @@ __device__ int glb_fast(const VerifyArgs& A, const double* __restrict__ Q,
 }
 
 template <int METRIC>
-__global__ void __launch_bounds__(256) k_glb(VerifyArgs a, const double* __restrict__ Q,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_glb(VerifyArgs a, const double* __restrict__ Q,
                                              const double* __restrict__ LQ,
                                              const double* __restrict__ UQ,
                                              const double* __restrict__ XG,
@@ -471,8 +471,14 @@ __global__ void __launch_bounds__(256) k_glb(VerifyArgs a, const double* __restr
                                              int64_t* __restrict__ out,
                                              unsigned long long* out_count,
                                              double* __restrict__ out_total,
-                                             double* __restrict__ vals) {
+                                             double* __restrict__ vals, int stage_len) {
     __shared__ unsigned long long sh[256 / 64 + 1];
+    __shared__ int64_t rng[2 * (256 / 64)];
+    // staged samples / envelope of the block's windows (3 x stage_len doubles)
+    extern __shared__ double glb_stage[];
+    double* Ts = glb_stage;
+    double* Ls = Ts + stage_len;
+    double* Us = Ls + stage_len;
     const int64_t n = (int64_t)*in_count;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t iters = (n + stride - 1) / stride;
@@ -491,9 +497,54 @@ __global__ void __launch_bounds__(256) k_glb(VerifyArgs a, const double* __restr
         int verdict = 0;
         // without the fast pass's bound: nothing (distances), every sample (LCSS)
         double total = (METRIC == LCSS) ? (double)a.p.m : 0.0;
-        if (fast)
+        // When the block's candidate windows span at most stage_len - m
+        // windows, their samples and envelope are read once per block into
+        // LDS (the passes read each value up to m times).  The list is
+        // ascending only within each compaction block's run, so the span
+        // comes from the block's min / max window.
+        bool staged = false;
+        int64_t gfirst = 0;
+        if (fast && stage_len > 0) {
+            int64_t lo = active ? g : INT64_MAX, hi = active ? g : -1;
+            for (int o = 32; o > 0; o >>= 1) {
+                const int64_t ol = __shfl_xor(lo, o, 64), oh = __shfl_xor(hi, o, 64);
+                lo = ol < lo ? ol : lo;
+                hi = oh > hi ? oh : hi;
+            }
+            const int wid = threadIdx.x >> 6;
+            if ((threadIdx.x & 63) == 0) {
+                rng[2 * wid] = lo;
+                rng[2 * wid + 1] = hi;
+            }
+            __syncthreads();
+            lo = rng[0];
+            hi = rng[1];
+            for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+                lo = rng[2 * k] < lo ? rng[2 * k] : lo;
+                hi = rng[2 * k + 1] > hi ? rng[2 * k + 1] : hi;
+            }
+            __syncthreads();
+            gfirst = lo;
+            const int64_t glast = hi;
+            staged = glast >= 0 && glast - gfirst + a.p.m <= stage_len;
+            if (staged) {
+                const int len = (int)(glast - gfirst) + a.p.m;
+                for (int x = threadIdx.x; x < len; x += blockDim.x) {
+                    Ts[x] = a.T[gfirst + x];
+                    Ls[x] = a.Lraw[gfirst + x];
+                    Us[x] = a.Uraw[gfirst + x];
+                }
+                __syncthreads();
+            }
+        }
+        if (fast && staged) {
+            const int64_t off = (active ? g : gfirst) - gfirst;
+            verdict = glb_fast<METRIC>(a, Q, LQ, UQ, XG, Ts + off, Ls + off, Us + off, mu, sd,
+                                       active, &total);
+        } else if (fast) {
             verdict = glb_fast<METRIC>(a, Q, LQ, UQ, XG, a.T + g, a.Lraw + g, a.Uraw + g, mu, sd,
                                        active, &total);
+        }
         bool keep = verdict > 0;
         if (active && verdict == 0) {
             const double v = glb_one<METRIC>(a.T + g, mu, sd, Q, LQ, UQ, a.Lraw + g, a.Uraw + g,
@@ -503,18 +554,24 @@ __global__ void __launch_bounds__(256) k_glb(VerifyArgs a, const double* __restr
         }
         keep = keep && active;
         if (out) block_append(keep, g, total, out, out_total, out_count, sh);
+        if (staged) __syncthreads();  // before the next iteration's staging
     }
 }
 
 static int verify_blocks() { return 2048; }
+constexpr int kGlbStage = 1664;
 
 hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
                       const unsigned long long* in_count, int64_t* out,
                       unsigned long long* out_count, double* out_total, double* vals) {
     const dim3 g(verify_blocks()), b(256);
     const QueryTables& q = a.q;
+    // staging: 3 arrays of up to kGlbStage doubles (39 KiB, four blocks per CU)
+    // when the block's 256 windows leave at least 256 more of the span
+    const int stage_len = (a.p.m + 512 <= kGlbStage) ? kGlbStage : 0;
+    const size_t lds = sizeof(double) * 3 * (size_t)stage_len;
 #define FSM_GLB(MM) \
-    hipLaunchKernelGGL(k_glb<MM>, g, b, 0, st, a, q.qn, q.LQ, q.UQ, q.XG, in, in_count, out, out_count, out_total, vals)
+    hipLaunchKernelGGL(k_glb<MM>, g, b, lds, st, a, q.qn, q.LQ, q.UQ, q.XG, in, in_count, out, out_count, out_total, vals, stage_len)
     switch (a.p.metric) {
         case DTW: FSM_GLB(DTW); break;
         case ERP: FSM_GLB(ERP); break;
