@@ -786,7 +786,9 @@ __global__ void __launch_bounds__(64) k_eld_wave(VerifyArgs a, const double* __r
 }
 
 // ------------------------------------------------------------ launching ---
-static inline __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+// (a template: one definition per translation unit is allowed)
+template <int = 0>
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
 inline size_t reg_lds_bytes(int m) { return sizeof(double) * (size_t)(m + 2 * kXPad) * 3; }
 
@@ -821,7 +823,7 @@ hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64_t* in,
         return hipGetLastError();
     }
     if (!queue) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, queue);
+    hipLaunchKernelGGL(k_zero_u64<>, dim3(1), dim3(1), 0, st, queue);
     if (uses_wave_band(METRIC, a.p.m, w)) {
         const size_t wl = wave_lds_bytes(a.p.m);
         if (w <= 63)
