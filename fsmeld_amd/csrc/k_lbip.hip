@@ -55,15 +55,6 @@ size_t lbip_lds_bytes(const Geometry& G) {
 }
 
 // ---------------------------------------------------------------- helpers ---
-__device__ inline uint64_t key_of(double x) {
-    const uint64_t b = (uint64_t)__double_as_longlong(x);
-    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-__device__ inline double val_of(uint64_t k) {
-    const uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
-    return __longlong_as_double((long long)b);
-}
-
 __device__ inline int upper_bound(const double* a, int n, double x) {  // first a[i] > x
     int lo = 0, hi = n;
     while (lo < hi) {
@@ -191,18 +182,30 @@ __device__ inline double stat_range(double ph, double sl, int64_t lo, int64_t hi
     return sl;
 }
 
-__device__ void bitonic_sort(uint64_t* keys, int n) {
+// Bitonic sorts of the segment's envelope values, ascending.  NaN values are
+// replaced by +inf or -inf before sorting (the caller counts them out), so the
+// values compare totally and each comparator is one min and one max.  Every
+// comparator is ascending: the first step of each merge pairs index i with
+// its mirror i ^ (k - 1) in the k-block, the later steps with i ^ j.
+// Equal values (+0 / -0 included) may leave in either order; the thresholds
+// read values and ranks only, as std::sort leaves equal keys unordered.
+
+// min / max of two values that are never NaN (a compare and two selects,
+// without fmin's NaN quieting)
+__device__ __forceinline__ double vmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double vmax(double a, double b) { return (b < a) ? a : b; }
+
+// In LDS or global scratch, any power-of-two n.
+__device__ void bitonic_sort(double* keys, int n) {
     for (int k = 2; k <= n; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
+            const bool mirror = j == (k >> 1);
             for (int q = threadIdx.x; q < (n >> 1); q += blockDim.x) {
-                const int i = ((q / j) * (j << 1)) + (q % j);
-                const int ixj = i + j;
-                const uint64_t a = keys[i], b = keys[ixj];
-                const bool asc = (i & k) == 0;
-                if ((a > b) == asc) {
-                    keys[i] = b;
-                    keys[ixj] = a;
-                }
+                const int i = ((q / j) * (j << 1)) + (q % j);  // bit j of i clear
+                const int p = mirror ? (i ^ (k - 1)) : (i + j);
+                const double a = keys[i], b = keys[p];
+                keys[i] = vmin(a, b);
+                keys[p] = vmax(a, b);
             }
             __threadfence_block();
             __syncthreads();
@@ -210,35 +213,45 @@ __device__ void bitonic_sort(uint64_t* keys, int n) {
     }
 }
 
-// Bitonic sort of n = blockDim.x * R keys with the keys in registers: thread t
-// holds keys [t R, t R + R).  Compare distances j < R stay inside a thread,
-// R <= j < 64 R pair lanes of one wave (one 64-bit shuffle per key), and only
-// j >= 64 R exchange through `keys` (LDS or global scratch) with a barrier.
-// Leaves the sorted keys in `keys`.
-template <int R, int J>
-__device__ __forceinline__ void bitonic_local(uint64_t* v, int k, int base) {
+// n = blockDim.x * R values in registers: thread t holds [t R, t R + R).
+// Distances below R stay inside a thread, below 64 R pair lanes of one wave
+// (a mirror step reads the partner lane's register R - 1 - r), and only
+// larger ones exchange through `keys` (LDS or global scratch) with a barrier.
+// Leaves the sorted values in `keys`.
+// In-thread step at distance J (MIRROR: the first step of a 2J merge);
+// register indices are compile-time constants.
+template <int R, int J, bool MIRROR>
+__device__ __forceinline__ void cx_local(double* v) {
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        const int p = r ^ J;
-        if (p > r) {
-            const bool asc = ((base + r) & k) == 0;
-            const uint64_t a = v[r], b = v[p];
-            const bool sw = (a > b) == asc;
-            v[r] = sw ? b : a;
-            v[p] = sw ? a : b;
+        constexpr int M = MIRROR ? 2 * J - 1 : J;
+        const int p = r ^ M;
+        if ((r & J) == 0 && p < R) {
+            const double a = v[r], b = v[p];
+            v[r] = vmin(a, b);
+            v[p] = vmax(a, b);
         }
     }
 }
 
+template <int R, int J>
+__device__ __forceinline__ void cx_local(double* v, bool mirror) {
+    if constexpr (J < R) {
+        if (mirror) cx_local<R, J, true>(v);
+        else cx_local<R, J, false>(v);
+    }
+}
+
 template <int R>
-__device__ void bitonic_sort_reg(uint64_t* keys, int n) {
+__device__ void bitonic_sort_reg(double* keys, int n) {
     const int t = threadIdx.x;
     const int base = t * R;
-    uint64_t v[R];
+    double v[R];
 #pragma unroll
     for (int r = 0; r < R; r++) v[r] = keys[base + r];
     for (int k = 2; k <= n; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
+            const bool mirror = j == (k >> 1);
             if (j >= 64 * R) {
                 __syncthreads();
 #pragma unroll
@@ -247,28 +260,35 @@ __device__ void bitonic_sort_reg(uint64_t* keys, int n) {
                 __syncthreads();
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    const int i = base + r, p = i ^ j;
-                    const uint64_t u = keys[p];
-                    const bool keep_min = (((i & k) == 0) == (i < p));
-                    v[r] = keep_min ? (v[r] < u ? v[r] : u) : (v[r] > u ? v[r] : u);
+                    const int i = base + r, p = mirror ? (i ^ (k - 1)) : (i ^ j);
+                    const double u = keys[p];
+                    v[r] = (i < p) ? vmin(v[r], u) : vmax(v[r], u);
                 }
             } else if (j >= R) {
-                const int lm = j / R;
-                const bool lower = (t & lm) == 0;
+                const bool lower = (t & (j / R)) == 0;
+                if (mirror) {  // registers r and R - 1 - r trade with the partner lane
+                    const int lm = k / R - 1;
 #pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const uint64_t u = __shfl_xor(v[r], lm, 64);
-                    const bool keep_min = (((base + r) & k) == 0) == lower;
-                    v[r] = keep_min ? (v[r] < u ? v[r] : u) : (v[r] > u ? v[r] : u);
+                    for (int r = 0; r < R / 2; r++) {
+                        const double a = __shfl_xor(v[R - 1 - r], lm, 64);
+                        const double b = __shfl_xor(v[r], lm, 64);
+                        v[r] = lower ? vmin(v[r], a) : vmax(v[r], a);
+                        v[R - 1 - r] = lower ? vmin(v[R - 1 - r], b) : vmax(v[R - 1 - r], b);
+                    }
+                } else {
+                    const int lm = j / R;
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const double u = __shfl_xor(v[r], lm, 64);
+                        v[r] = lower ? vmin(v[r], u) : vmax(v[r], u);
+                    }
                 }
             } else {
                 switch (j) {
-                    case 1: bitonic_local<R, 1>(v, k, base); break;
-                    case 2: if (R > 2) bitonic_local<R, 2>(v, k, base); break;
-                    case 4: if (R > 4) bitonic_local<R, 4>(v, k, base); break;
-                    case 8: if (R > 8) bitonic_local<R, 8>(v, k, base); break;
-                    case 16: if (R > 16) bitonic_local<R, 16>(v, k, base); break;
-                    case 32: if (R > 32) bitonic_local<R, 32>(v, k, base); break;
+                    case 1: cx_local<R, 1>(v, mirror); break;
+                    case 2: cx_local<R, 2>(v, mirror); break;
+                    case 4: cx_local<R, 4>(v, mirror); break;
+                    case 8: cx_local<R, 8>(v, mirror); break;
                     default: break;
                 }
             }
@@ -281,9 +301,9 @@ __device__ void bitonic_sort_reg(uint64_t* keys, int n) {
     __syncthreads();
 }
 
-// R = n / blockDim.x keys per thread in registers, or 0: the LDS / scratch sort
+// R = n / blockDim.x values per thread in registers, or 0: the LDS / scratch sort
 template <int R>
-__device__ void sort_keys(uint64_t* keys, int n) {
+__device__ void sort_keys(double* keys, int n) {
     if constexpr (R > 0) bitonic_sort_reg<R>(keys, n);
     else bitonic_sort(keys, n);
 }
@@ -327,6 +347,17 @@ __device__ ArgBest block_best(ArgBest x, ArgBest* sh) {
     ArgBest r = sh[0];
     for (int k = 1; k < (int)(blockDim.x >> 6); k++)
         r = MAX ? better_max(r, sh[k]) : better_min(r, sh[k]);
+    __syncthreads();
+    return r;
+}
+
+// Block-wide sum of one int per thread (ends with a barrier).
+__device__ int block_count(int c, int* sh) {
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+    __syncthreads();
+    int r = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) r += sh[k];
     __syncthreads();
     return r;
 }
@@ -390,8 +421,10 @@ __device__ inline int64_t run_sum(const uint64_t* bm, const int64_t* CH, int j, 
 }
 
 // ------------------------------------------------------------------ kernel ---
+// Up to 8 values per thread the kernel fits four waves per SIMD, 16 three.
 template <int R, int NT>
-__global__ void __launch_bounds__(NT) k_lbip(LbipArgs a) {
+__global__ void __launch_bounds__(NT)
+    __attribute__((amdgpu_waves_per_eu(NT == 256 ? (R <= 8 ? 4 : 3) : 1))) k_lbip(LbipArgs a) {
     extern __shared__ unsigned char lds_raw[];
     __shared__ ArgBest sh_best[NT / 64];
     __shared__ double sh_sum[NT / 64];
@@ -415,7 +448,7 @@ __global__ void __launch_bounds__(NT) k_lbip(LbipArgs a) {
     double* PQ = slab;  // 8 arrays of kml: (2*stat + dir)
     double* Un = slab + 8 * (size_t)kml;
     double* Ln = Un + mk;
-    uint64_t* keys = (npad <= kLdsKeys) ? (GUw + nw) : (uint64_t*)(Ln + mk);
+    double* keys = (npad <= kLdsKeys) ? (double*)(GUw + nw) : (Ln + mk);
 
     for (int64_t s = blockIdx.x; s < G.S; s += gridDim.x) {
         const bool tail = (s == G.forsize);
@@ -494,6 +527,8 @@ __global__ void __launch_bounds__(NT) k_lbip(LbipArgs a) {
         //     Each thread takes kEnvG positions at a time and issues their
         //     loads (envelope, the eight prefix / suffix values) first.
         constexpr int kEnvG = 2;
+        const double pinf = __builtin_inf();
+        int ucnt = 0;
         for (int i0 = threadIdx.x; i0 < mk; i0 += kEnvG * NT) {
             double Lg[kEnvG], Ug[kEnvG], PS[kEnvG][8];
 #pragma unroll
@@ -538,75 +573,69 @@ __global__ void __launch_bounds__(NT) k_lbip(LbipArgs a) {
             const double ln = (nl <= 0) ? nl / sgmin : nl / sgmax;
             Un[i] = un;
             Ln[i] = ln;
-            if (i >= m - 1 && i <= mk - m)
-                keys[i - (m - 1)] = (un != un) ? ~0ull : key_of(un);
+            if (i >= m - 1 && i <= mk - m) {
+                keys[i - (m - 1)] = (un != un) ? pinf : un;
+                ucnt += (un == un);
+            }
             }
         }
-        for (int p = nthr + threadIdx.x; p < npad; p += blockDim.x) keys[p] = ~0ull;
+        for (int p = nthr + threadIdx.x; p < npad; p += blockDim.x) keys[p] = pinf;
         __threadfence_block();
         __syncthreads();
 
-        // 4a. upper threshold: ascending order, G = rank + 1
+        // 4a. upper threshold: ascending order, G = rank + 1; the first nval
+        //     sorted values are the non-NaN ones (NaN and padding sort last)
+        const int nval_u = block_count(ucnt, sh_cnt);
         sort_keys<R>(keys, npad);
         double t_u;
         {
-            int cnt = 0;
-            for (int p = threadIdx.x; p < npad; p += blockDim.x) cnt += (keys[p] != ~0ull);
-            for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
-            if ((threadIdx.x & 63) == 0) sh_cnt[threadIdx.x >> 6] = cnt;
-            __syncthreads();
-            int nval = 0;
-            for (int k = 0; k < NT / 64; k++) nval += sh_cnt[k];
-            __syncthreads();
+            const int nval = nval_u;
             ArgBest best{__builtin_nan(""), 0};
             for (int p = threadIdx.x; p < nval; p += blockDim.x) {
-                const double tv = val_of(keys[p]);
+                const double tv = keys[p];
                 const double sc = h_upper(Q, metric, m, C, tv) * (double)(p + 1);
                 best = better_max(best, ArgBest{sc, p});
             }
             best = block_best<true>(best, sh_best);
             int64_t idx = best.i;
             if (nval > 0) {
-                const double t0 = val_of(keys[0]);
+                const double t0 = keys[0];
                 const double s0 = h_upper(Q, metric, m, C, t0);
                 if (s0 != s0) idx = 0;  // src/find_t_lp.cpp:206-216 keeps index 0
             }
-            t_u = (nval > 0) ? val_of(keys[idx]) : __builtin_nan("");
+            t_u = (nval > 0) ? keys[idx] : __builtin_nan("");
         }
         __syncthreads();
 
-        // 4b. lower threshold: descending order, G = rank-from-top + 1
+        // 4b. lower threshold: descending order, G = rank-from-top + 1; NaN
+        //     and padding sort first
+        int lcnt = 0;
         for (int i = threadIdx.x; i < nthr; i += blockDim.x) {
             const double ln = Ln[i + m - 1];
-            keys[i] = (ln != ln) ? 0ull : key_of(ln);
+            keys[i] = (ln != ln) ? -pinf : ln;
+            lcnt += (ln == ln);
         }
-        for (int p = nthr + threadIdx.x; p < npad; p += blockDim.x) keys[p] = 0ull;
+        for (int p = nthr + threadIdx.x; p < npad; p += blockDim.x) keys[p] = -pinf;
         __threadfence_block();
         __syncthreads();
+        const int nval_l = block_count(lcnt, sh_cnt);
         sort_keys<R>(keys, npad);
         double t_l;
         {
-            int cnt = 0;
-            for (int p = threadIdx.x; p < npad; p += blockDim.x) cnt += (keys[p] != 0ull);
-            for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
-            if ((threadIdx.x & 63) == 0) sh_cnt[threadIdx.x >> 6] = cnt;
-            __syncthreads();
-            int nval = 0;
-            for (int k = 0; k < NT / 64; k++) nval += sh_cnt[k];
-            __syncthreads();
+            const int nval = nval_l;
             ArgBest best{__builtin_nan(""), 0};
             for (int r = threadIdx.x; r < nval; r += blockDim.x) {
-                const double tv = val_of(keys[npad - 1 - r]);
+                const double tv = keys[npad - 1 - r];
                 const double sc = h_lower(Q, metric, m, C, tv) * (double)(r + 1);
                 best = better_max(best, ArgBest{sc, r});
             }
             best = block_best<true>(best, sh_best);
             int64_t idx = best.i;
             if (nval > 0) {
-                const double s0 = h_lower(Q, metric, m, C, val_of(keys[npad - 1]));
+                const double s0 = h_lower(Q, metric, m, C, keys[npad - 1]);
                 if (s0 != s0) idx = 0;
             }
-            t_l = (nval > 0) ? val_of(keys[npad - 1 - idx]) : __builtin_nan("");
+            t_l = (nval > 0) ? keys[npad - 1 - idx] : __builtin_nan("");
         }
         if (threadIdx.x == 0) {
             if (a.TUo) a.TUo[s] = t_u;
