@@ -20,6 +20,8 @@
 // Min / max / compare steps are exact, so for NaN-free segments steps 1-3
 // reproduce the reference's bits.  Near NaN samples the min / max propagate
 // NaN, which only clears mask bits (admissible).
+#include <algorithm>
+
 #include "engine_internal.h"
 
 namespace fsm {
@@ -37,11 +39,20 @@ __host__ __device__ inline int npad_of(int n) {
 // thread, and long segments spread their per-position work over more waves.
 inline int lbip_threads(int npad) { return npad <= 4096 ? 256 : 1024; }
 
+// The segment's sort keys live in LDS up to 8192 of them, 16384 with a
+// 1024-thread workgroup (one per CU: 128 KiB of its 160), else in the
+// block's global scratch.  The penalty prefix sums and mask bitmaps of the
+// later steps reuse the keys' LDS space: the keys are dead once both
+// thresholds are known.
+__host__ __device__ inline bool keys_in_lds(int npad, int nt) {
+    return npad <= (nt >= 1024 ? 16384 : kLdsKeys);
+}
+
 size_t lbip_slab_bytes(const Geometry& G) {
     const int nthr = G.m_k - 2 * G.m + 2;
     const int npad = npad_of(nthr);
     size_t b = sizeof(double) * (8 * (size_t)G.kml + 2 * (size_t)G.m_k);
-    if (npad > kLdsKeys) b += sizeof(uint64_t) * (size_t)npad;
+    if (!keys_in_lds(npad, lbip_threads(npad))) b += sizeof(double) * (size_t)npad;
     return (b + 255) & ~(size_t)255;
 }
 
@@ -49,9 +60,9 @@ size_t lbip_lds_bytes(const Geometry& G) {
     const int nthr = G.m_k - 2 * G.m + 2;
     const int npad = npad_of(nthr);
     const int nw = (G.m_k + 63) / 64;
-    size_t b = sizeof(int64_t) * 2 * (size_t)(G.m + 1) + sizeof(uint64_t) * 2 * (size_t)nw;
-    if (npad <= kLdsKeys) b += sizeof(uint64_t) * (size_t)npad;
-    return b;
+    const size_t late = sizeof(int64_t) * 2 * (size_t)(G.m + 1) + sizeof(uint64_t) * 2 * (size_t)nw;
+    const size_t k = keys_in_lds(npad, lbip_threads(npad)) ? sizeof(double) * (size_t)npad : 0;
+    return std::max(late, k);
 }
 
 // ---------------------------------------------------------------- helpers ---
@@ -448,7 +459,7 @@ __global__ void __launch_bounds__(NT)
     double* PQ = slab;  // 8 arrays of kml: (2*stat + dir)
     double* Un = slab + 8 * (size_t)kml;
     double* Ln = Un + mk;
-    double* keys = (npad <= kLdsKeys) ? (double*)(GUw + nw) : (Ln + mk);
+    double* keys = keys_in_lds(npad, NT) ? (double*)lds_raw : (Ln + mk);
 
     for (int64_t s = blockIdx.x; s < G.S; s += gridDim.x) {
         const bool tail = (s == G.forsize);
