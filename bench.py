@@ -37,6 +37,8 @@ profiler run can launch the kernels in the timed configuration only.
 w = 5% of m, metric uniform over the five, cutoffs from k=10): the plain path
 (one fsm_query at a time) against fsm_query_batch at several concurrencies;
 every batched result is compared with the plain one.
+--workload small: the 2^20 plain small configuration (m=128, w=6, DTW, k=10
+cutoff): GPU search against the CPU restatement on 16 host threads.
 --workload precision: the 2^26 target at m = 128 and 1024 (w = 5% of m):
 LB_IP's pruning rate at the k=10 cutoff, match-set identity of search and
 exact, and LB <= ELD on a sample of windows (the error against an FP64 oracle
@@ -112,7 +114,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cutoff-cache", default=None)
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--workload", default="plain", choices=["plain", "batch", "serve", "precision"])
+    ap.add_argument("--workload", default="plain",
+                    choices=["plain", "small", "batch", "serve", "precision"])
     ap.add_argument("--concurrency", default="4,8")
     ap.add_argument("--rates", default="50,200,1000")
     ap.add_argument("--requests", type=int, default=60, help="arrivals per rate (serve)")
@@ -134,6 +137,9 @@ def main():
         return
     if args.workload == "precision":
         run_precision(args)
+        return
+    if args.workload == "small":
+        run_small(args)
         return
 
     import fsmeld_amd as F
@@ -446,6 +452,48 @@ def run_serve(args):
                                f"{args.requests} Poisson arrivals per rate; {args.workers} workers, "
                                f"{budget >> 30} GiB budget"},
         "rates": rows}), flush=True)
+
+
+def run_small(args):
+    """The plain small configuration end to end: a k = 10 cutoff query, then
+    `search` at its threshold, on the GPU (steps timed after warmup) against
+    the CPU restatement of the same search on 16 host threads (the whole
+    target, no sampling); the two must return the same windows."""
+    import fsmeld_amd as F
+    import oracle as O
+    from fsmeld_amd import workloads
+    cfg = workloads.plain_small()
+    t, q, m, w, metric, k = cfg["target"], cfg["query"], cfg["m"], cfg["w"], cfg["metric"], cfg["k"]
+    T = F.Target(t)
+    c = T.cutoff(q, m, k, metric, w, 0.0)
+    thr = c.threshold
+    for _ in range(max(args.warmup, 1)):
+        T.search(q, m, thr, metric, w, 0.0)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps * 20):
+        s0 = time.perf_counter()
+        r = T.search(q, m, thr, metric, w, 0.0)
+        lat.append((time.perf_counter() - s0) * 1000.0)
+    gpu_s = (time.perf_counter() - t0) / (args.steps * 20)
+    threads = 16
+    c0 = time.perf_counter()
+    cpu_idx, cst = O.search(t, q, m, thr, metric, w, 0.0, threads=threads)
+    cpu_s = time.perf_counter() - c0
+    same = bool(np.array_equal(np.asarray(cpu_idx), r.indices))
+    log(f"small: gpu {gpu_s * 1e3:.2f} ms/search, cpu ({threads} threads) {cpu_s * 1e3:.1f} ms, "
+        f"same windows: {same}")
+    print(json.dumps({
+        "metric": "plain small configuration: GPU search vs the CPU restatement on the host cores",
+        "value": round(1.0 / gpu_s, 3), "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "plain small: 2^20 random walk, 10 constant runs of 300, 10 NaNs; "
+                               f"m={m}, w={w}, {metric}, cutoff from k={k}", "n": int(t.size)},
+        "p50_ms": round(_pct(lat, 50), 3), "p99_ms": round(_pct(lat, 99), 3),
+        "cutoff": thr, "matches": int(r.indices.size),
+        "cpu_baseline": {"value": round(1.0 / cpu_s, 4), "unit": "queries/s", "cores": threads,
+                         "kind": "port", "sample": "the whole search on the host"},
+        "speedup_vs_cpu": round(cpu_s / gpu_s, 1), "same_windows_as_cpu": same}), flush=True)
 
 
 def run_precision(args):

@@ -332,7 +332,7 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
                                              double* __restrict__ out_dist,
                                              unsigned long long* out_count,
                                              unsigned long long* queue,
-                                             unsigned long long* work) {
+                                             unsigned long long* work, int64_t n_min) {
     extern __shared__ double xlds[];
     constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
     // samples 0 .. b + KOFF lie at or before the frontier's largest column
@@ -341,6 +341,7 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
     // first iteration: anti-diagonals 0 and 1 (W even) or -1 and 0 (W odd)
     constexpr int BS = -((W + 1) / 2);
     const int64_t n = in_count ? (int64_t)*in_count : n_direct;
+    if (n < n_min) return;  // few candidates: the wave kernel takes them
     const int m = a.p.m;
     const int M = GRID1 ? m : m - 1;
     const int bend = (2 * M - W) >= 0 ? (2 * M - W) / 2 : -((W - 2 * M + 1) / 2);
@@ -671,11 +672,12 @@ __global__ void __launch_bounds__(64) k_eld_wave(VerifyArgs a, const double* __r
                                                  double* __restrict__ out_dist,
                                                  unsigned long long* out_count,
                                                  unsigned long long* queue,
-                                                 unsigned long long* work) {
+                                                 unsigned long long* work, int64_t n_max) {
     extern __shared__ double wlds[];
     constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
     constexpr int Q = 2 * CPL;
     const int64_t n = in_count ? (int64_t)*in_count : n_direct;
+    if (n >= n_max) return;  // many candidates: the register kernel takes them
     const int m = a.p.m, w = a.p.w;
     const int M = GRID1 ? m : m - 1;
     const double arg = a.p.arg, cut = a.p.cutoff;
@@ -797,6 +799,7 @@ inline size_t wave_lds_bytes(int m) { return sizeof(double) * 2 * (size_t)(m + 2
 // register band: w <= 32; one wave per candidate: w <= 127; the row-band path
 // beyond, or where the LDS staging does not fit
 constexpr int kRegMaxW = 25;  // the launcher instantiates k_eld for W = 0 .. kRegMaxW
+constexpr int64_t kEldFewCandidates = 1024;
 inline bool uses_register_band(int metric, int m, int w) {
     (void)metric;
     return reg_lds_bytes(m) <= kEldLdsCap && w <= kRegMaxW;
@@ -824,17 +827,34 @@ hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64_t* in,
     }
     if (!queue) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_zero_u64<>, dim3(1), dim3(1), 0, st, queue);
-    if (uses_wave_band(METRIC, a.p.m, w)) {
-        const size_t wl = wave_lds_bytes(a.p.m);
+    const size_t wl = wave_lds_bytes(a.p.m);
+    auto launch_wave = [&](int64_t n_max) {
         if (w <= 63)
             hipLaunchKernelGGL((k_eld_wave<METRIC, 1>), dim3(2048), dim3(64), wl, st, a, a.q.LQ,
                                a.q.UQ, in, in_total, in_count, n_direct, rows, ka, out_idx,
-                               out_dist, out_count, queue, work);
+                               out_dist, out_count, queue, work, n_max);
         else
             hipLaunchKernelGGL((k_eld_wave<METRIC, 2>), dim3(2048), dim3(64), wl, st, a, a.q.LQ,
                                a.q.UQ, in, in_total, in_count, n_direct, rows, ka, out_idx,
-                               out_dist, out_count, queue, work);
+                               out_dist, out_count, queue, work, n_max);
+    };
+    if (uses_wave_band(METRIC, a.p.m, w)) {
+        launch_wave(INT64_MAX);
         return hipGetLastError();
+    }
+    // Register band.  Below kEldFewCandidates candidates a lane per
+    // candidate leaves the GPU idle while each lane walks its whole band, so
+    // the wave kernel (one wave per candidate) takes them; a count on the
+    // device gates both launches, each kernel exiting at once outside its range.
+    int64_t n_min = 0;
+    if (w <= 127 && wl <= kEldLdsCap) {
+        if (in_count) {
+            launch_wave(kEldFewCandidates);
+            n_min = kEldFewCandidates;
+        } else if (n_direct < kEldFewCandidates) {
+            launch_wave(INT64_MAX);
+            return hipGetLastError();
+        }
     }
     // persistent: enough waves to fill every SIMD; later blocks find the
     // queue drained and exit
@@ -845,7 +865,7 @@ hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64_t* in,
     case WW:                                                                                   \
         hipLaunchKernelGGL((k_eld<METRIC, WW>), grid, blk, lds, st, a, a.q.LQ, a.q.UQ, in,     \
                            in_total, in_count, n_direct, rows, ka, out_idx, out_dist,          \
-                           out_count, queue, work);                                            \
+                           out_count, queue, work, n_min);                                     \
         break;
         FSM_ELD(0) FSM_ELD(1) FSM_ELD(2) FSM_ELD(3) FSM_ELD(4) FSM_ELD(5) FSM_ELD(6) FSM_ELD(7)
         FSM_ELD(8) FSM_ELD(9) FSM_ELD(10) FSM_ELD(11) FSM_ELD(12) FSM_ELD(13) FSM_ELD(14)
