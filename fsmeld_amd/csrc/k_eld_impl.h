@@ -36,6 +36,8 @@
 // that lies within eps of the envelope.  The distances of surviving windows
 // are unchanged.
 #pragma once
+#include <algorithm>
+
 #include "engine_internal.h"
 
 namespace fsm {
@@ -616,14 +618,62 @@ __global__ void __launch_bounds__(256) k_eld_rows(VerifyArgs a, const double* __
 }
 
 // ------------------------------------------------------------ wide bands ---
-// One wavefront per candidate (radii above the register variants).  The band
-// slots k = i - j + w in [0, 2w] are spread over the lanes, 2 CPL slots per
-// lane (lane l holds k = 2 CPL l + q, q in [0, 2 CPL)); anti-diagonal d
-// updates the slots of parity (d + w) & 1 from the other parity (up k-1,
-// left k+1) and its own old value (diag), so one step needs one neighbour
-// value from the adjacent lane.  Cells use the reference's boundary forms
-// (cell<METRIC, true>).  Query and candidate samples sit in LDS; a wave takes
-// its next candidate from the work queue when the last one ends.
+// Data-parallel-primitive moves of a double (two 32-bit DPP movs): quad
+// swaps xor 1 / xor 2, row rotations (row_ror:N: lane l reads lane
+// (l - N) mod 16 of its row, so row_ror:12 reads l + 4), and the whole-wave
+// shifts by one lane of the gfx9 family (wave_shl:1 reads lane l + 1,
+// wave_shr:1 lane l - 1; the lane without a source reads 0).
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppRorNext4 = 0x12C, kDppRor8 = 0x128;
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(
+        (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
+}
+
+// cell<METRIC, true> with every form evaluated and the boundary one selected:
+// the same expressions on the same operands, so the same bits, and no branch.
+template <int METRIC>
+__device__ __forceinline__ double cell_sel(int i, int j, double up, double left, double diag,
+                                           double xi, double xp, double yj, double yp,
+                                           double xg, double yg, double arg) {
+    const bool i0 = i == 0, j0 = j == 0;
+    if (METRIC == DTW) {
+        const double S = kEldSentinel;
+        const double d = (xi - yj) * (xi - yj);
+        const double r = dmin3(i0 ? S : up, j0 ? S : left, (i0 || j0) ? S : diag) + d;
+        return (i0 && j0) ? d : r;
+    } else if (METRIC == ERP) {
+        const double e1 = left + yg, e2 = up + xg;
+        const double r = dmin3(diag + (xp - yp) * (xp - yp), e1, e2);
+        return i0 ? (j0 ? 0.0 : e1) : (j0 ? e2 : r);
+    } else if (METRIC == EDR) {
+        const double sub = (fabs(xp - yp) <= arg) ? 0.0 : 1.0;
+        const double r = dmin3(up + 1.0, left + 1.0, diag + sub);
+        return i0 ? (double)j : (j0 ? (double)i : r);
+    } else if (METRIC == LCSS) {
+        return cell<LCSS, true>(i, j, up, left, diag, xi, xp, yj, yp, xg, yg, arg);
+    } else {  // MSM
+        const double t = fabs(xi - yj);
+        const double c2 = up + msm_cost(xi, xp, yj, arg);
+        const double c3 = left + msm_cost(yj, xi, yp, arg);
+        const double r = dmin3(diag + t, c2, c3);
+        return i0 ? (j0 ? t : c3) : (j0 ? c2 : r);
+    }
+}
+
+// One wavefront per candidate (few candidates, or radii above the register
+// variants).  The band slots k = i - j + w in [0, 2w] are spread over the
+// lanes, 2 CPL slots per lane (lane l holds k = 2 CPL l + q, q in [0, 2 CPL));
+// anti-diagonal d updates the slots of parity (d + w) & 1 from the other
+// parity (up k-1, left k+1) and its own old value (diag), so one step needs
+// one neighbour value from the adjacent lane (a DPP whole-wave shift).  Cells
+// use the reference's boundary forms (cell_sel).  Query and candidate samples
+// sit in LDS; a wave takes its next candidate from the work queue when the
+// last one ends.
 template <int METRIC, int CPL>
 __device__ __forceinline__ void wave_step(int par, int d, int w, int M, int m, double arg,
                                           const double* xs, const double* ys, double* V,
@@ -635,7 +685,9 @@ __device__ __forceinline__ void wave_step(int par, int d, int w, int M, int m, d
     // neighbour slot from the adjacent lane: the previous lane's last slot
     // (parity 0 updates need it as "up" of q = 0), or the next lane's first
     // (parity 1 updates need it as "left" of q = Q - 1)
-    const double nb = (par == 0) ? __shfl_up(V[Q - 1], 1, 64) : __shfl_down(V[0], 1, 64);
+    // (the lanes without a source, 0 resp. 63, hold k = 0 resp. k > 2w and
+    // never use nb)
+    const double nb = (par == 0) ? dpp_f64<kDppWaveShr1>(V[Q - 1]) : dpp_f64<kDppWaveShl1>(V[0]);
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         if ((q & 1) != par) continue;
@@ -643,22 +695,86 @@ __device__ __forceinline__ void wave_step(int par, int d, int w, int M, int m, d
         const int sft = k - w;
         const int i = (d + sft) >> 1, j = (d - sft) >> 1;
         const bool inr = k <= 2 * w && i >= 0 && j >= 0 && i <= M && j <= M;
-        double v = S;
-        if (inr) {
-            const double up = (k == 0) ? S : (q >= 1 ? V[q - 1] : nb);
-            const double left = (k == 2 * w) ? S : (q + 1 < Q ? V[q + 1] : nb);
-            if (GRID1) {
-                const double xp = xs[i - 1], yp = ys[j - 1];
-                v = cell<METRIC, true>(i, j, up, left, V[q], 0.0, xp, 0.0, yp,
-                                       (xp - arg) * (xp - arg), (yp - arg) * (yp - arg), arg);
-            } else {
-                v = cell<METRIC, true>(i, j, up, left, V[q], xs[i], xs[i - 1], ys[j], ys[j - 1],
-                                       0.0, 0.0, arg);
-            }
-            live_cells++;
+        // every slot reads samples: indices clamped into the staged range
+        // [-1, m] (cells outside the band or the matrix are replaced by S)
+        const int ic = min(max(i, 0), m), jc = min(max(j, 0), m);
+        const double up = (k == 0) ? S : (q >= 1 ? V[q - 1] : nb);
+        const double left = (k == 2 * w) ? S : (q + 1 < Q ? V[q + 1] : nb);
+        double v;
+        if (GRID1) {
+            const double xp = xs[ic - 1], yp = ys[jc - 1];
+            v = cell_sel<METRIC>(i, j, up, left, V[q], 0.0, xp, 0.0, yp,
+                                 (xp - arg) * (xp - arg), (yp - arg) * (yp - arg), arg);
+        } else {
+            v = cell_sel<METRIC>(i, j, up, left, V[q], xs[ic], xs[ic - 1], ys[jc], ys[jc - 1],
+                                 0.0, 0.0, arg);
         }
-        V[q] = v;
+        live_cells += inr ? 1 : 0;
+        V[q] = inr ? v : S;
     }
+}
+
+// Anti-diagonals 0 .. 2M of one candidate, in blocks of 8 whose first
+// anti-diagonal has parity P0 = w & 1 (slot parity of d0 + w, d0 even).  With
+// abandoning, after anti-diagonal d = 8 t + 7 < 2M: the frontier (anti-
+// diagonals d - 1 and d) reduced over the nl lanes that hold band slots,
+// plus the bound over the samples beyond it, against thr.  Every lane adds
+// the new samples' consumption bounds itself (uniform reads), so p is the
+// same in all lanes.  Returns false when the candidate is abandoned.
+template <int METRIC, int CPL, int P0>
+__device__ __forceinline__ bool wave_band(int w, int M, int m, double arg, const double* xs,
+                                          const double* ys, double* V, int& live_cells,
+                                          bool check, double thr, double total,
+                                          const double* __restrict__ LQ,
+                                          const double* __restrict__ UQ) {
+    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
+    constexpr int Q = 2 * CPL;
+    const int D = 2 * M;
+    const int nl = (2 * w + Q) / Q;  // lanes holding slots 0 .. 2w
+    double p = 0.0;
+    int kp = 0;  // samples 0 .. kp-1 are in p
+    int d0 = 0;
+    for (; d0 + 8 <= D; d0 += 8) {
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+            wave_step<METRIC, CPL>((P0 + s) & 1, d0 + s, w, M, m, arg, xs, ys, V, live_cells);
+        if (!check) continue;
+        const int d = d0 + 7;
+        // samples at or before the frontier's largest column (LCSS: before
+        // its smallest), floor((d + w) / 2) resp. floor((d - 1 - w) / 2)
+        const int K = (METRIC == LCSS) ? ((d - 1 - w) >= 0 ? (d - 1 - w) / 2 : -1)
+                                       : ((d + w) >> 1) - (GRID1 ? 1 : 0);
+        const int kn = min(K, m - 1) + 1;
+        for (int s = kp; s < kn; s++) p += consume_lb<METRIC>(ys[s], LQ[s], UQ[s], arg);
+        if (kn > kp) kp = kn;
+        double e = V[0];
+#pragma unroll
+        for (int q = 1; q < Q; q++) e = (METRIC == LCSS) ? fmax(e, V[q]) : fmin(e, V[q]);
+        // lane 0 ends with the extremum over lanes 0 .. 2^ceil(log2 nl) - 1
+        // (lanes past the band hold S): quads, then lane 0 takes quad 1
+        // (lane 4), then lanes 8 .. 15 (lane 8)
+        auto red = [](double x, double y) { return (METRIC == LCSS) ? fmax(x, y) : fmin(x, y); };
+        e = red(e, dpp_f64<kDppXor1>(e));
+        if (nl > 2) e = red(e, dpp_f64<kDppXor2>(e));
+        if (nl > 4) e = red(e, dpp_f64<kDppRorNext4>(e));
+        if (nl > 8) e = red(e, dpp_f64<kDppRor8>(e));
+        if (nl > 16) e = red(e, __shfl_xor(e, 16, 64));
+        if (nl > 32) e = red(e, __shfl_xor(e, 32, 64));
+        bool alive;
+        if (METRIC == LCSS) {
+            const double rem = fmin((double)((D - (d - 1)) / 2 + 1), total - p);
+            alive = !(1.0 - (e + rem) / (double)m > thr);
+        } else {
+            const double cb = fmax(total - p * (1.0 + (double)(m + 8) * 1.2e-16), 0.0);
+            alive = !((e + cb) * (1.0 - (double)(2 * m + 24) * 1.2e-16) > thr);
+        }
+        if (!__builtin_amdgcn_readfirstlane(alive ? 1 : 0)) return false;
+    }
+    for (int d = d0; d <= D; d++) {
+        if (((d + w) & 1) == 0) wave_step<METRIC, CPL>(0, d, w, M, m, arg, xs, ys, V, live_cells);
+        else wave_step<METRIC, CPL>(1, d, w, M, m, arg, xs, ys, V, live_cells);
+    }
+    return true;
 }
 
 template <int METRIC, int CPL>
@@ -722,49 +838,12 @@ __global__ void __launch_bounds__(64) k_eld_wave(VerifyArgs a, const double* __r
         double V[Q];
 #pragma unroll
         for (int q = 0; q < Q; q++) V[q] = S;
-        double p = 0.0;
-        int kp = 0;  // samples 0 .. kp-1 are in p
         int live_cells = 0;
-        bool alive = true;
-        for (int d = 0; d <= 2 * M; d++) {
-            const int par = (d + w) & 1;
-            if (par == 0) wave_step<METRIC, CPL>(0, d, w, M, m, arg, xs, ys, V, live_cells);
-            else wave_step<METRIC, CPL>(1, d, w, M, m, arg, xs, ys, V, live_cells);
-            if (check && (d & 7) == 7 && d < 2 * M) {
-                // frontier: anti-diagonals d - 1 and d
-                // samples at or before the frontier's largest column (LCSS:
-                // before its smallest), floor((d + w) / 2) resp. floor((d-1-w) / 2)
-                const int K = (METRIC == LCSS) ? ((d - 1 - w) >= 0 ? (d - 1 - w) / 2 : -1)
-                                               : ((d + w) >> 1) - (GRID1 ? 1 : 0);
-                const int kn = min(K, m - 1) + 1;  // new kp
-                for (int base = kp; base < kn; base += 64) {
-                    double add = 0.0;
-                    if (base + lane < kn) {
-                        const int s = base + lane;
-                        add = consume_lb<METRIC>(ys[s], LQ[s], UQ[s], arg);
-                    }
-                    for (int o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o, 64);
-                    p += add;
-                }
-                if (kn > kp) kp = kn;
-                double e = V[0];
-#pragma unroll
-                for (int q = 1; q < Q; q++)
-                    e = (METRIC == LCSS) ? fmax(e, V[q]) : fmin(e, V[q]);
-                for (int o = 32; o > 0; o >>= 1) {
-                    const double t = __shfl_xor(e, o, 64);
-                    e = (METRIC == LCSS) ? fmax(e, t) : fmin(e, t);
-                }
-                if (METRIC == LCSS) {
-                    const double rem = fmin((double)((2 * M - (d - 1)) / 2 + 1), total - p);
-                    alive = !(1.0 - (e + rem) / (double)m > thr);
-                } else {
-                    const double cb = fmax(total - p * (1.0 + (double)(m + 8) * 1.2e-16), 0.0);
-                    alive = !((e + cb) * (1.0 - (double)(2 * m + 24) * 1.2e-16) > thr);
-                }
-                if (!alive) break;
-            }
-        }
+        const bool alive =
+            (w & 1) ? wave_band<METRIC, CPL, 1>(w, M, m, arg, xs, ys, V, live_cells, check, thr,
+                                                total, LQ, UQ)
+                    : wave_band<METRIC, CPL, 0>(w, M, m, arg, xs, ys, V, live_cells, check, thr,
+                                                total, LQ, UQ);
         live_total += (unsigned long long)live_cells;
         if (!alive) continue;
         // cell (M, M): slot k = w
@@ -829,12 +908,15 @@ hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64_t* in,
     hipLaunchKernelGGL(k_zero_u64<>, dim3(1), dim3(1), 0, st, queue);
     const size_t wl = wave_lds_bytes(a.p.m);
     auto launch_wave = [&](int64_t n_max) {
+        // persistent waves; no more than the candidates there can be
+        const int64_t bound = in_count ? n_max : n_direct;
+        const dim3 wgrid((unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, bound)));
         if (w <= 63)
-            hipLaunchKernelGGL((k_eld_wave<METRIC, 1>), dim3(2048), dim3(64), wl, st, a, a.q.LQ,
+            hipLaunchKernelGGL((k_eld_wave<METRIC, 1>), wgrid, dim3(64), wl, st, a, a.q.LQ,
                                a.q.UQ, in, in_total, in_count, n_direct, rows, ka, out_idx,
                                out_dist, out_count, queue, work, n_max);
         else
-            hipLaunchKernelGGL((k_eld_wave<METRIC, 2>), dim3(2048), dim3(64), wl, st, a, a.q.LQ,
+            hipLaunchKernelGGL((k_eld_wave<METRIC, 2>), wgrid, dim3(64), wl, st, a, a.q.LQ,
                                a.q.UQ, in, in_total, in_count, n_direct, rows, ka, out_idx,
                                out_dist, out_count, queue, work, n_max);
     };
