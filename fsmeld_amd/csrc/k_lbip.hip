@@ -8,8 +8,9 @@
 //      taken from the target-wide envelope except within w of the edges;
 //   3. normalised envelopes (src/lb_normalize.cpp:9-32);
 //   4. thresholds t_u, t_l: the argmax over the segment's envelope values of
-//      h(t) * G(t), the objective of src/find_t_lp.cpp (sorted values via a
-//      bitonic sort, h(t) in closed form over the sorted query);
+//      h(t) * G(t), the objective of src/find_t_lp.cpp (h(t) in closed form
+//      over the sorted query; the ranks G of the few values whose bucket can
+//      hold the maximum, from a sort of those values only: select_threshold);
 //   5. penalties h (src/q_to_h.cpp) in exact 64-bit fixed point, rounded
 //      down, and their prefix sums CH;
 //   6. masks g (src/lb.cpp) as bitmaps built with wavefront ballots;
@@ -35,13 +36,12 @@ __host__ __device__ inline int npad_of(int n) {
 }
 
 // Workgroup size per segment: 256 threads up to 4096 keys, 1024 beyond, so
-// that every sort up to 16384 keys runs in registers with at most 16 keys per
-// thread, and long segments spread their per-position work over more waves.
+// that long segments spread their per-position work over more waves.
 inline int lbip_threads(int npad) { return npad <= 4096 ? 256 : 1024; }
 
 // The segment's sort keys live in LDS up to 8192 of them, 16384 with a
-// 1024-thread workgroup (one per CU: 128 KiB of its 160), else in the
-// block's global scratch.  The penalty prefix sums and mask bitmaps of the
+// 1024-thread workgroup (one per CU: 128 KiB of its 160, and 24 KiB for the
+// threshold buckets), else in the block's global scratch.  The penalty prefix sums and mask bitmaps of the
 // later steps reuse the keys' LDS space: the keys are dead once both
 // thresholds are known.
 __host__ __device__ inline bool keys_in_lds(int npad, int nt) {
@@ -56,13 +56,22 @@ size_t lbip_slab_bytes(const Geometry& G) {
     return (b + 255) & ~(size_t)255;
 }
 
+// Threshold selection by buckets (select_threshold): kThrBuckets buckets of
+// value-ordered keys, and a candidate list of up to thr_cap(npad) keys.
+constexpr int kThrBuckets = 256;
+__host__ __device__ inline int thr_cap(int npad) { return npad / 8 > 64 ? npad / 8 : 64; }
+__host__ __device__ inline size_t thr_lds_bytes(int npad) {
+    return sizeof(double) * (size_t)thr_cap(npad) +
+           (size_t)kThrBuckets * (2 * sizeof(uint64_t) + 4 * sizeof(int));
+}
+
 size_t lbip_lds_bytes(const Geometry& G) {
     const int nthr = G.m_k - 2 * G.m + 2;
     const int npad = npad_of(nthr);
     const int nw = (G.m_k + 63) / 64;
     const size_t late = sizeof(int64_t) * 2 * (size_t)(G.m + 1) + sizeof(uint64_t) * 2 * (size_t)nw;
     const size_t k = keys_in_lds(npad, lbip_threads(npad)) ? sizeof(double) * (size_t)npad : 0;
-    return std::max(late, k);
+    return std::max(late, k + thr_lds_bytes(npad));
 }
 
 // ---------------------------------------------------------------- helpers ---
@@ -193,13 +202,13 @@ __device__ inline double stat_range(double ph, double sl, int64_t lo, int64_t hi
     return sl;
 }
 
-// Bitonic sorts of the segment's envelope values, ascending.  NaN values are
-// replaced by +inf or -inf before sorting (the caller counts them out), so the
-// values compare totally and each comparator is one min and one max.  Every
-// comparator is ascending: the first step of each merge pairs index i with
-// its mirror i ^ (k - 1) in the k-block, the later steps with i ^ j.
-// Equal values (+0 / -0 included) may leave in either order; the thresholds
-// read values and ranks only, as std::sort leaves equal keys unordered.
+// Bitonic sort, ascending, of keys that are never NaN (the threshold choice
+// maps NaN to +inf and counts it out), so the values compare totally and
+// each comparator is one min and one max.  Every comparator is ascending: the
+// first step of each merge pairs index i with its mirror i ^ (k - 1) in the
+// k-block, the later steps with i ^ j.  Equal values (+0 / -0 included) may
+// leave in either order; the thresholds read values and ranks only, as
+// std::sort leaves equal keys unordered.
 
 // min / max of two values that are never NaN (a compare and two selects,
 // without fmin's NaN quieting)
@@ -222,108 +231,6 @@ __device__ void bitonic_sort(double* keys, int n) {
             __syncthreads();
         }
     }
-}
-
-// n = blockDim.x * R values in registers: thread t holds [t R, t R + R).
-// Distances below R stay inside a thread, below 64 R pair lanes of one wave
-// (a mirror step reads the partner lane's register R - 1 - r), and only
-// larger ones exchange through `keys` (LDS or global scratch) with a barrier.
-// Leaves the sorted values in `keys`.
-// In-thread step at distance J (MIRROR: the first step of a 2J merge);
-// register indices are compile-time constants.
-template <int R, int J, bool MIRROR>
-__device__ __forceinline__ void cx_local(double* v) {
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        constexpr int M = MIRROR ? 2 * J - 1 : J;
-        const int p = r ^ M;
-        if ((r & J) == 0 && p < R) {
-            const double a = v[r], b = v[p];
-            v[r] = vmin(a, b);
-            v[p] = vmax(a, b);
-        }
-    }
-}
-
-template <int R, int J>
-__device__ __forceinline__ void cx_local(double* v, bool mirror) {
-    if constexpr (J < R) {
-        if (mirror) cx_local<R, J, true>(v);
-        else cx_local<R, J, false>(v);
-    }
-}
-
-template <int R>
-__device__ void bitonic_sort_reg(double* keys, int n) {
-    const int t = threadIdx.x;
-    const int base = t * R;
-    double v[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) v[r] = keys[base + r];
-    for (int k = 2; k <= n; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const bool mirror = j == (k >> 1);
-            if (j >= 64 * R) {
-                __syncthreads();
-#pragma unroll
-                for (int r = 0; r < R; r++) keys[base + r] = v[r];
-                __threadfence_block();
-                __syncthreads();
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const int i = base + r, p = mirror ? (i ^ (k - 1)) : (i ^ j);
-                    const double u = keys[p];
-                    v[r] = (i < p) ? vmin(v[r], u) : vmax(v[r], u);
-                }
-            } else if (j >= R) {
-                const bool lower = (t & (j / R)) == 0;
-                if (mirror) {  // registers r and R - 1 - r trade with the partner lane
-                    const int lm = k / R - 1;
-#pragma unroll
-                    for (int r = 0; r < R / 2; r++) {
-                        const double a = __shfl_xor(v[R - 1 - r], lm, 64);
-                        const double b = __shfl_xor(v[r], lm, 64);
-                        v[r] = lower ? vmin(v[r], a) : vmax(v[r], a);
-                        v[R - 1 - r] = lower ? vmin(v[R - 1 - r], b) : vmax(v[R - 1 - r], b);
-                    }
-                } else {
-                    const int lm = j / R;
-#pragma unroll
-                    for (int r = 0; r < R; r++) {
-                        const double u = __shfl_xor(v[r], lm, 64);
-                        v[r] = lower ? vmin(v[r], u) : vmax(v[r], u);
-                    }
-                }
-            } else {
-                switch (j) {
-                    case 1: cx_local<R, 1>(v, mirror); break;
-                    case 2: cx_local<R, 2>(v, mirror); break;
-                    case 4: cx_local<R, 4>(v, mirror); break;
-                    case 8: cx_local<R, 8>(v, mirror); break;
-                    default: break;
-                }
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < R; r++) keys[base + r] = v[r];
-    __threadfence_block();
-    __syncthreads();
-}
-
-// R = n / blockDim.x values per thread in registers, or 0: the LDS / scratch sort
-template <int R>
-__device__ void sort_keys(double* keys, int n) {
-    if constexpr (R > 0) bitonic_sort_reg<R>(keys, n);
-    else bitonic_sort(keys, n);
-}
-
-// register-sort class of a padded key count for nt threads (one kernel
-// instantiation each): keys per thread, 0 for the LDS / scratch sort
-inline int sort_class(int npad, int nt) {
-    const int r = npad / nt;
-    return (npad % nt == 0 && (r == 2 || r == 4 || r == 8 || r == 16)) ? r : 0;
 }
 
 // Block argmax with "first index wins" among equal scores; NaN never wins.
@@ -400,6 +307,174 @@ __device__ int64_t block_exscan(int64_t x, int64_t* sh) {
     return off + inc - x;
 }
 
+// Block-wide min and max of one value each per thread (ends with a barrier).
+__device__ void block_min_max(double* lo, double* hi, double* sh) {
+    double a = *lo, b = *hi;
+    for (int o = 32; o > 0; o >>= 1) {
+        a = fmin(a, __shfl_xor(a, o, 64));
+        b = fmax(b, __shfl_xor(b, o, 64));
+    }
+    const int nwv = (int)(blockDim.x >> 6), wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[wid] = a;
+        sh[nwv + wid] = b;
+    }
+    __syncthreads();
+    a = sh[0];
+    b = sh[nwv];
+    for (int k = 1; k < nwv; k++) {
+        a = fmin(a, sh[k]);
+        b = fmax(b, sh[nwv + k]);
+    }
+    __syncthreads();
+    *lo = a;
+    *hi = b;
+}
+
+// ------------------------------------------------------- threshold choice ---
+// The threshold is the key at the argmax, over the ascending ranks p of the
+// nval non-NaN keys, of h(key_p) (p + 1) (better_max: the larger score, then
+// the smaller rank; NaN scores never win), or the smallest key when h of it is
+// NaN (src/find_t_lp.cpp:206-216).  h is non-increasing in exact arithmetic
+// and herr(a) bounds |computed h(t) - exact h(t)| for |t| <= a.
+//
+// Buckets.  Keys map monotonically to value-ordered buckets (0: -inf, 1 ..
+// kThrBuckets-2: equal widths of the finite range, last: +inf), each with its
+// count and its min / max as order-preserving integers.  A bucket's keys take
+// the ranks [start, end) of the full order, and its largest key sits at rank
+// end - 1, so h(max) end is a score the full order attains.  Every score in
+// the bucket is at most (h(min) + 2 herr) end, so a bucket whose bound is
+// below the best attained score holds no maximum.  The kept buckets' keys are
+// gathered, sorted, ranked start + their place in the bucket and scored as the
+// full order scores them.  Beyond thr_cap candidates: the full sort.
+__device__ inline unsigned long long ord_key(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ inline double key_of(unsigned long long o) {
+    return __longlong_as_double((long long)((o >> 63) ? (o & 0x7fffffffffffffffull) : ~o));
+}
+__device__ inline int bucket_of(double v, double lo, double scale) {
+    if (v == -__builtin_inf()) return 0;
+    if (v == __builtin_inf()) return kThrBuckets - 1;
+    const double x = (v - lo) * scale;  // non-decreasing in v (NaN only when scale is 0)
+    return 1 + (int)fmin(fmax(x, 0.0), (double)(kThrBuckets - 3));
+}
+
+// keys[0 .. nthr): the values, NaN for positions that do not count; [lo, hi]:
+// the finite keys' range.  tlds: thr_lds_bytes(npad) of LDS.  The full sort
+// (fallback) reorders keys[0 .. npad) in place.
+template <int NT, class HF, class EF>
+__device__ double select_threshold(double* keys, int nthr, int npad, int nval, double lo,
+                                   double hi, HF hf, EF herr, unsigned char* tlds,
+                                   ArgBest* sh_best, int* sh_cnt, int64_t* sh_scan,
+                                   double* sh_key) {
+    if (nval == 0) return __builtin_nan("");
+    const int tid = threadIdx.x;
+    const int cap = thr_cap(npad);
+    double* cand = (double*)tlds;
+    unsigned long long* bmin = (unsigned long long*)(cand + cap);
+    unsigned long long* bmax = bmin + kThrBuckets;
+    int* cnt = (int*)(bmax + kThrBuckets);
+    int* bst = cnt + kThrBuckets;  // first rank
+    int* cst = bst + kThrBuckets;  // first candidate slot; -1: bucket dropped
+    int* fil = cst + kThrBuckets;
+    const double scale =
+        (hi > lo && hi - lo < __builtin_inf()) ? (double)(kThrBuckets - 2) / (hi - lo) : 0.0;
+    for (int b = tid; b < kThrBuckets; b += NT) {
+        cnt[b] = 0;
+        fil[b] = 0;
+        bmin[b] = ~0ull;
+        bmax[b] = 0ull;
+    }
+    __syncthreads();
+    for (int i = tid; i < nthr; i += NT) {
+        const double v = keys[i];
+        if (v != v) continue;
+        const int b = bucket_of(v, lo, scale);
+        const unsigned long long o = ord_key(v);
+        atomicAdd(&cnt[b], 1);
+        atomicMin(&bmin[b], o);
+        atomicMax(&bmax[b], o);
+    }
+    __syncthreads();
+    const int c = (tid < kThrBuckets) ? cnt[tid] : 0;
+    const int st = (int)block_exscan(c, sh_scan);
+    if (tid < kThrBuckets) bst[tid] = st;
+    if (tid < kThrBuckets && c > 0 && st == 0) *sh_key = key_of(bmin[tid]);  // rank 0
+    __syncthreads();
+    const double kmin = *sh_key;
+    __syncthreads();
+    const double h0 = hf(kmin);
+    if (h0 != h0) return kmin;  // src/find_t_lp.cpp:206-216 keeps index 0
+
+    ArgBest att{__builtin_nan(""), 0};
+    double ub = 0.0;
+    if (tid < kThrBuckets && c > 0) {
+        const double vmn = key_of(bmin[tid]), vmx = key_of(bmax[tid]);
+        const int e = st + c;
+        att = ArgBest{hf(vmx) * (double)e, (int64_t)(e - 1)};
+        const double hb = hf(vmn) + 2.0 * herr(fmax(fabs(vmn), fabs(vmx)));
+        ub = (hb >= 0.0) ? hb * (double)e : hb * (double)(st + 1);
+        ub += fabs(ub) * 1e-12 + 1e-300;
+    }
+    const ArgBest lb = block_best<true>(att, sh_best);
+    const bool keep = tid < kThrBuckets && c > 0 && !(ub < lb.v);
+    const int kc = keep ? c : 0;
+    const int cs = (int)block_exscan(kc, sh_scan);
+    const int K = block_count(kc, sh_cnt);
+    if (tid < kThrBuckets) cst[tid] = keep ? cs : -1;
+    __syncthreads();
+    if (K <= cap) {
+        const int kpad = npad_of(K);
+        for (int i = tid; i < nthr; i += NT) {
+            const double v = keys[i];
+            if (v != v) continue;
+            const int b = bucket_of(v, lo, scale);
+            const int o = cst[b];
+            if (o >= 0) cand[o + atomicAdd(&fil[b], 1)] = v;
+        }
+        for (int j = K + tid; j < kpad; j += NT) cand[j] = __builtin_inf();
+        __threadfence_block();
+        __syncthreads();
+        bitonic_sort(cand, kpad);
+        ArgBest best{__builtin_nan(""), 0};
+        double bkey = kmin;
+        for (int j = tid; j < K; j += NT) {
+            const double v = cand[j];
+            const int b = bucket_of(v, lo, scale);
+            const int64_t p = (int64_t)bst[b] + (j - cst[b]);
+            const ArgBest x{hf(v) * (double)(p + 1), p};
+            const ArgBest nb = better_max(best, x);
+            if (nb.i == p && nb.v == nb.v) bkey = v;
+            best = nb;
+        }
+        const ArgBest r = block_best<true>(best, sh_best);
+        if (r.v == r.v && best.v == r.v && best.i == r.i) *sh_key = bkey;
+        __syncthreads();
+        const double t = (r.v == r.v) ? *sh_key : kmin;
+        __syncthreads();
+        return t;
+    }
+    // full order: NaN and padding sort last
+    for (int i = tid; i < npad; i += NT) {
+        const double v = (i < nthr) ? keys[i] : __builtin_nan("");
+        if (v != v) keys[i] = __builtin_inf();
+    }
+    __threadfence_block();
+    __syncthreads();
+    bitonic_sort(keys, npad);
+    ArgBest best{__builtin_nan(""), 0};
+    for (int p = tid; p < nval; p += NT) {
+        const double tv = keys[p];
+        best = better_max(best, ArgBest{hf(tv) * (double)(p + 1), p});
+    }
+    best = block_best<true>(best, sh_best);
+    const double t = keys[best.i];
+    __syncthreads();
+    return t;
+}
+
 // First position p in [x, end) whose mask bit equals `want`, else end.
 __device__ inline int find_bit(const uint64_t* bm, int x, int end, bool want) {
     while (x < end) {
@@ -432,15 +507,17 @@ __device__ inline int64_t run_sum(const uint64_t* bm, const int64_t* CH, int j, 
 }
 
 // ------------------------------------------------------------------ kernel ---
-// Up to 8 values per thread the kernel fits four waves per SIMD, 16 three.
-template <int R, int NT>
+// 256-thread workgroups fit four waves per SIMD.
+template <int NT>
 __global__ void __launch_bounds__(NT)
-    __attribute__((amdgpu_waves_per_eu(NT == 256 ? (R <= 8 ? 4 : 3) : 1))) k_lbip(LbipArgs a) {
+    __attribute__((amdgpu_waves_per_eu(NT == 256 ? 4 : 1))) k_lbip(LbipArgs a) {
     extern __shared__ unsigned char lds_raw[];
     __shared__ ArgBest sh_best[NT / 64];
     __shared__ double sh_sum[NT / 64];
     __shared__ int64_t sh_scan[NT / 64];
     __shared__ int sh_cnt[NT / 64];
+    __shared__ double sh_mm[2 * (NT / 64)];
+    __shared__ double sh_key;
 
     const Geometry G = a.G;
     const QueryTables Q = a.q;
@@ -538,8 +615,8 @@ __global__ void __launch_bounds__(NT)
         //     Each thread takes kEnvG positions at a time and issues their
         //     loads (envelope, the eight prefix / suffix values) first.
         constexpr int kEnvG = 2;
-        const double pinf = __builtin_inf();
         int ucnt = 0;
+        double klo = __builtin_inf(), khi = -__builtin_inf();  // finite keys
         for (int i0 = threadIdx.x; i0 < mk; i0 += kEnvG * NT) {
             double Lg[kEnvG], Ug[kEnvG], PS[kEnvG][8];
 #pragma unroll
@@ -585,69 +662,57 @@ __global__ void __launch_bounds__(NT)
             Un[i] = un;
             Ln[i] = ln;
             if (i >= m - 1 && i <= mk - m) {
-                keys[i - (m - 1)] = (un != un) ? pinf : un;
+                keys[i - (m - 1)] = un;
                 ucnt += (un == un);
+                if (__builtin_isfinite(un)) {
+                    klo = fmin(klo, un);
+                    khi = fmax(khi, un);
+                }
             }
             }
         }
-        for (int p = nthr + threadIdx.x; p < npad; p += blockDim.x) keys[p] = pinf;
         __threadfence_block();
         __syncthreads();
 
-        // 4a. upper threshold: ascending order, G = rank + 1; the first nval
-        //     sorted values are the non-NaN ones (NaN and padding sort last)
+        // 4a. upper threshold: ascending ranks of U_n, G = rank + 1
+        // (|computed h - exact h| is far below 1e-9 of the terms' magnitude:
+        // prefix sums of at most m terms of |q| + |t| + |C| or its square)
+        const double qmax = fmax(fabs(Q.qs[0]), fabs(Q.qs[m - 1]));
+        auto herr = [&](double at) {
+            const double A = qmax + at + fabs(C);
+            return 1e-9 * (double)m * (A * A + A);
+        };
+        unsigned char* tlds = lds_raw + (keys_in_lds(npad, NT) ? sizeof(double) * npad : 0);
         const int nval_u = block_count(ucnt, sh_cnt);
-        sort_keys<R>(keys, npad);
-        double t_u;
-        {
-            const int nval = nval_u;
-            ArgBest best{__builtin_nan(""), 0};
-            for (int p = threadIdx.x; p < nval; p += blockDim.x) {
-                const double tv = keys[p];
-                const double sc = h_upper(Q, metric, m, C, tv) * (double)(p + 1);
-                best = better_max(best, ArgBest{sc, p});
-            }
-            best = block_best<true>(best, sh_best);
-            int64_t idx = best.i;
-            if (nval > 0) {
-                const double t0 = keys[0];
-                const double s0 = h_upper(Q, metric, m, C, t0);
-                if (s0 != s0) idx = 0;  // src/find_t_lp.cpp:206-216 keeps index 0
-            }
-            t_u = (nval > 0) ? keys[idx] : __builtin_nan("");
-        }
-        __syncthreads();
+        block_min_max(&klo, &khi, sh_mm);
+        const double t_u = select_threshold<NT>(
+            keys, nthr, npad, nval_u, klo, khi,
+            [&](double t) { return h_upper(Q, metric, m, C, t); }, herr, tlds, sh_best, sh_cnt,
+            sh_scan, &sh_key);
 
-        // 4b. lower threshold: descending order, G = rank-from-top + 1; NaN
-        //     and padding sort first
+        // 4b. lower threshold: ranks from the top of L_n, G = rank + 1: the
+        //     ascending ranks of -L_n (negation is exact)
         int lcnt = 0;
+        klo = __builtin_inf();
+        khi = -__builtin_inf();
         for (int i = threadIdx.x; i < nthr; i += blockDim.x) {
-            const double ln = Ln[i + m - 1];
-            keys[i] = (ln != ln) ? -pinf : ln;
-            lcnt += (ln == ln);
+            const double nl = -Ln[i + m - 1];
+            keys[i] = nl;
+            lcnt += (nl == nl);
+            if (__builtin_isfinite(nl)) {
+                klo = fmin(klo, nl);
+                khi = fmax(khi, nl);
+            }
         }
-        for (int p = nthr + threadIdx.x; p < npad; p += blockDim.x) keys[p] = -pinf;
         __threadfence_block();
         __syncthreads();
         const int nval_l = block_count(lcnt, sh_cnt);
-        sort_keys<R>(keys, npad);
-        double t_l;
-        {
-            const int nval = nval_l;
-            ArgBest best{__builtin_nan(""), 0};
-            for (int r = threadIdx.x; r < nval; r += blockDim.x) {
-                const double tv = keys[npad - 1 - r];
-                const double sc = h_lower(Q, metric, m, C, tv) * (double)(r + 1);
-                best = better_max(best, ArgBest{sc, r});
-            }
-            best = block_best<true>(best, sh_best);
-            int64_t idx = best.i;
-            if (nval > 0) {
-                const double s0 = h_lower(Q, metric, m, C, keys[npad - 1]);
-                if (s0 != s0) idx = 0;
-            }
-            t_l = (nval > 0) ? keys[npad - 1 - idx] : __builtin_nan("");
-        }
+        block_min_max(&klo, &khi, sh_mm);
+        const double tln = select_threshold<NT>(
+            keys, nthr, npad, nval_l, klo, khi,
+            [&](double t) { return h_lower(Q, metric, m, C, -t); }, herr, tlds, sh_best, sh_cnt,
+            sh_scan, &sh_key);
+        const double t_l = (tln == tln) ? -tln : tln;
         if (threadIdx.x == 0) {
             if (a.TUo) a.TUo[s] = t_u;
             if (a.TLo) a.TLo[s] = t_l;
@@ -751,25 +816,9 @@ __global__ void __launch_bounds__(NT)
 hipError_t launch_lbip(hipStream_t st, const LbipArgs& a) {
     const size_t lds = lbip_lds_bytes(a.G);
     const int npad = npad_of(a.G.m_k - 2 * a.G.m + 2);
-    const int nt = lbip_threads(npad);
-    const dim3 g((unsigned)a.blocks), b(nt);
-#define FSM_LB(RR, TT) hipLaunchKernelGGL((k_lbip<RR, TT>), g, b, lds, st, a)
-    if (nt == 256) {
-        switch (sort_class(npad, 256)) {
-            case 2: FSM_LB(2, 256); break;
-            case 4: FSM_LB(4, 256); break;
-            case 8: FSM_LB(8, 256); break;
-            case 16: FSM_LB(16, 256); break;
-            default: FSM_LB(0, 256); break;
-        }
-    } else {
-        switch (sort_class(npad, 1024)) {
-            case 8: FSM_LB(8, 1024); break;
-            case 16: FSM_LB(16, 1024); break;
-            default: FSM_LB(0, 1024); break;
-        }
-    }
-#undef FSM_LB
+    const dim3 g((unsigned)a.blocks);
+    if (lbip_threads(npad) == 256) hipLaunchKernelGGL(k_lbip<256>, g, dim3(256), lds, st, a);
+    else hipLaunchKernelGGL(k_lbip<1024>, g, dim3(1024), lds, st, a);
     return hipGetLastError();
 }
 

@@ -137,9 +137,8 @@ def test_exact_command_abandon_is_safe(gpu, small, metric, arg, m, w):
 @pytest.mark.parametrize("metric,arg", METRICS)
 @pytest.mark.parametrize("m,w,logn", [(32, 3, 14), (512, 25, 17), (1024, 51, 18), (2048, 102, 19)])
 def test_lbip_values(gpu, metric, arg, m, w, logn):
-    """One case per LB_IP kernel shape: 256 threads with the keys of a segment
-    in the LDS sort (m = 32) or 16 per thread in registers (m = 512), 1024
-    threads with 8 (m = 1024) or 16 (m = 2048) keys per thread."""
+    """One case per LB_IP workgroup shape and key count: 256 threads (m = 32,
+    512), 1024 threads with 8192 (m = 1024) and 16384 (m = 2048) keys in LDS."""
     t = make_target(8, 1 << logn, nans=0, runs=0, spikes=0.0)  # NaN-free: deques exact
     T = F.Target(t)
     q, _ = F.synth_query(8, t, m, 0.1)
@@ -157,6 +156,24 @@ def test_lbip_values(gpu, metric, arg, m, w, logn):
     win = np.flatnonzero(~exact_inv)[::37]
     d = T.eld(q, m, metric, w, arg, win)
     assert np.all(lb[win] <= d + 1e-9)
+
+
+@pytest.mark.parametrize("metric", ["edr", "lcss"])
+@pytest.mark.parametrize("m,w,logn", [(32, 3, 14), (512, 25, 17), (1024, 51, 18), (2100, 105, 19)])
+def test_lbip_flat_objective(gpu, metric, m, w, logn):
+    """eps = 10 lies beyond the query's range, so h(t) = 0 at every envelope
+    value: every score ties at 0, rank 0 wins, and the thresholds are each
+    segment's smallest U_n and largest L_n.  No bucket of values can be set
+    aside, so this runs the full sort of the segment's values (in LDS; in
+    global scratch for m = 2100)."""
+    t = make_target(9, 1 << logn, nans=0, runs=0, spikes=0.0)
+    T = F.Target(t)
+    q, _ = F.synth_query(9, t, m, 0.1)
+    lb, tu, tl = T.lbip(q, m, metric, w, 10.0)
+    olb, otu, otl = O.lbip_all(t, q, m, metric, w, 10.0)
+    assert np.array_equal(tu, otu)
+    assert np.array_equal(tl, otl)
+    assert np.array_equal(lb, olb, equal_nan=True)
 
 
 def run_all_modes(t, q, m, metric, w, arg, k, exclude=None):
