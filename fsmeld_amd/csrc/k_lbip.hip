@@ -291,6 +291,47 @@ __device__ double block_sum(double x, double* sh) {
     return r;
 }
 
+// Inclusive NaN-propagating min and max scans over the wave's 64 lanes in lane
+// order, with gfx9 data-parallel primitives: row_shr 1, 2, 4, 8 within each
+// 16-lane row, then row_bcast:15 (lane 15 of rows 0 / 2 into rows 1 / 3) and
+// row_bcast:31 (lane 31 into rows 2 and 3).  A lane without a source, or in a
+// row outside the step's row mask, keeps the identity (+inf for the min, -inf
+// for the max).  min / max are associative, so the values equal a sequential
+// scan's (up to the sign of a zero and NaN payloads).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64_or(double v, double id) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned long long o = (unsigned long long)__double_as_longlong(id);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)b, CTRL, ROWS, 0xf,
+                                               false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(b >> 32),
+                                               CTRL, ROWS, 0xf, false);
+    return __longlong_as_double(
+        (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void scan_step(double& vmin, double& vmax) {
+    const double a = dpp_f64_or<CTRL, ROWS>(vmin, __builtin_inf());
+    const double b = dpp_f64_or<CTRL, ROWS>(vmax, -__builtin_inf());
+    vmin = nmin(a, vmin);
+    vmax = nmax(b, vmax);
+}
+__device__ __forceinline__ void wave_scan_min_max(double& vmin, double& vmax) {
+    scan_step<0x111, 0xf>(vmin, vmax);  // row_shr:1
+    scan_step<0x112, 0xf>(vmin, vmax);  // row_shr:2
+    scan_step<0x114, 0xf>(vmin, vmax);  // row_shr:4
+    scan_step<0x118, 0xf>(vmin, vmax);  // row_shr:8
+    scan_step<0x142, 0xa>(vmin, vmax);  // row_bcast:15
+    scan_step<0x143, 0xc>(vmin, vmax);  // row_bcast:31
+}
+// lane 63's value, in every lane
+__device__ __forceinline__ double lane63(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Exclusive scan of per-thread int64 totals; returns this thread's offset.
 __device__ int64_t block_exscan(int64_t x, int64_t* sh) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -588,22 +629,15 @@ __global__ void __launch_bounds__(NT)
                         const int j = (dir == 0) ? j0 + c + lane : j1 - 1 - c - lane;
                         const bool in = (dir == 0) ? (j < j1) : (j >= j0);
                         double vmin = in ? v[u] : inf, vmax = in ? v[u] : -inf;
-                        for (int o = 1; o < 64; o <<= 1) {
-                            const double pmin = __shfl_up(vmin, o, 64);
-                            const double pmax = __shfl_up(vmax, o, 64);
-                            if (lane >= o) {
-                                vmin = nmin(pmin, vmin);
-                                vmax = nmax(pmax, vmax);
-                            }
-                        }
+                        wave_scan_min_max(vmin, vmax);
                         vmin = nmin(cmin, vmin);
                         vmax = nmax(cmax, vmax);
                         if (in) {
                             omin[j] = vmin;
                             omax[j] = vmax;
                         }
-                        cmin = __shfl(vmin, 63, 64);
-                        cmax = __shfl(vmax, 63, 64);
+                        cmin = lane63(vmin);
+                        cmax = lane63(vmax);
                     }
                 }
             }
