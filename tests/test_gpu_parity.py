@@ -73,6 +73,32 @@ def test_eld_pairs_bitexact(gpu, metric, arg, m, w):
 
 
 @pytest.mark.parametrize("metric,arg", METRICS)
+@pytest.mark.parametrize("m,w", [(40, 0), (33, 3), (64, 12), (64, 25)])
+def test_eld_register_and_wave_kernels_agree(gpu, small, metric, arg, m, w):
+    """Below 1024 candidates the one-wave-per-candidate kernel computes the
+    distances, from 1024 on the lane-per-candidate register kernel: the same
+    rows and windows through both give the same bits, and the register
+    kernel's agree with the oracle."""
+    t, T = small
+    rng = np.random.default_rng(m * 7 + w)
+    q = rng.normal(size=m)
+    c = rng.normal(size=(1100, m))
+    c[:8] = q + rng.normal(size=(8, m)) * 1e-3
+    many = F.eld_pairs(metric, q, c, w, arg)            # register kernel
+    few = F.eld_pairs(metric, q, c[:600], w, arg)       # wave kernel
+    assert same_bits(many[:600], few)
+    want = np.array([O.eld(metric, q, row, w, arg) for row in c[:40]])
+    assert same_bits(many[:40], want)
+    qw, _ = F.synth_query(6, t, m, 0.1)
+    _, _, inv = O.window_stats(t, m)
+    win = np.flatnonzero(~inv)[::7][:1100]
+    assert win.size >= 1024
+    dm = T.eld(qw, m, metric, w, arg, win)
+    df = T.eld(qw, m, metric, w, arg, win[:600])
+    assert same_bits(dm[:600], df)
+
+
+@pytest.mark.parametrize("metric,arg", METRICS)
 def test_eld_and_glb_windows_bitexact(gpu, small, metric, arg):
     t, T = small
     m, w = 64, 6
