@@ -15,7 +15,8 @@
 //      down, and their prefix sums CH;
 //   6. masks g (src/lb.cpp) as bitmaps built with wavefront ballots;
 //   7. per window j: z = sum over runs [a, b) of set mask bits inside
-//      [j, j+m) of CH[b-j] - CH[a-j] -- the sliding inner product of
+//      [j, j+m) of CH[b-j] - CH[a-j] (walked as the mask's transitions:
+//      trans_sum) -- the sliding inner product of
 //      src/sip.cpp evaluated exactly, so the bound is a certified lower bound
 //      of the FP64 penalties; then trans (src/fsmeld.cpp:84-106).
 // Min / max / compare steps are exact, so for NaN-free segments steps 1-3
@@ -69,7 +70,7 @@ size_t lbip_lds_bytes(const Geometry& G) {
     const int nthr = G.m_k - 2 * G.m + 2;
     const int npad = npad_of(nthr);
     const int nw = (G.m_k + 63) / 64;
-    const size_t late = sizeof(int64_t) * 2 * (size_t)(G.m + 1) + sizeof(uint64_t) * 2 * (size_t)nw;
+    const size_t late = sizeof(int64_t) * 2 * (size_t)(G.m + 1) + sizeof(uint64_t) * 4 * (size_t)nw;
     const size_t k = keys_in_lds(npad, lbip_threads(npad)) ? sizeof(double) * (size_t)npad : 0;
     return std::max(late, k + thr_lds_bytes(npad));
 }
@@ -516,34 +517,35 @@ __device__ double select_threshold(double* keys, int nthr, int npad, int nval, d
     return t;
 }
 
-// First position p in [x, end) whose mask bit equals `want`, else end.
-__device__ inline int find_bit(const uint64_t* bm, int x, int end, bool want) {
-    while (x < end) {
-        const int wi = x >> 6, b = x & 63;
-        uint64_t word = bm[wi];
-        if (!want) word = ~word;
-        word >>= b;
-        if (word) {
-            const int r = x + (int)__builtin_ctzll(word);
-            return r < end ? r : end;
-        }
-        x = (wi + 1) << 6;
-    }
-    return end;
-}
-
-// sum over runs [a, b) of set bits inside [j, j+m) of CH[b-j] - CH[a-j]
-__device__ inline int64_t run_sum(const uint64_t* bm, const int64_t* CH, int j, int m) {
-    const int end = j + m;
+// sum over runs [a, b) of set mask bits inside [j, j+m) of CH[b-j] - CH[a-j],
+// by parts: over the transitions t of the mask inside (j, j+m) (bit t of tw:
+// g[t] != g[t-1]) of +CH[t-j] where a run ends and -CH[t-j] where one starts
+// (they alternate, the first an end when g[j] is set; CH[0] = 0), plus CH[m]
+// when g[j+m-1] is set.  Exact int64: every partial sum lies in [-CH[m], CH[m]].
+__device__ inline int64_t trans_sum(const uint64_t* gw, const uint64_t* tw, const int64_t* CH,
+                                    int j, int m) {
     int64_t acc = 0;
-    int x = j;
-    while (x < end) {
-        const int a = find_bit(bm, x, end, true);
-        if (a >= end) break;
-        const int b = find_bit(bm, a, end, false);
-        acc += CH[b - j] - CH[a - j];
-        x = b;
+    const int hi = j + m - 1;  // last position of the window
+    if (m > 1) {
+        bool end = (gw[j >> 6] >> (j & 63)) & 1;
+        const int lo = j + 1;
+        int wi = lo >> 6;
+        const int wend = hi >> 6;
+        uint64_t word = tw[wi] & (~0ull << (lo & 63));
+        while (true) {
+            if (wi == wend) word &= (2ull << (hi & 63)) - 1;  // bits <= hi
+            while (word) {
+                const int t = (wi << 6) + __builtin_ctzll(word);
+                const int64_t c = CH[t - j];
+                acc += end ? c : -c;
+                end = !end;
+                word &= word - 1;
+            }
+            if (wi == wend) break;
+            word = tw[++wi];
+        }
     }
+    if ((gw[hi >> 6] >> (hi & 63)) & 1) acc += CH[m];
     return acc;
 }
 
@@ -573,6 +575,8 @@ __global__ void __launch_bounds__(NT)
     int64_t* CHu = CHl + (m + 1);
     uint64_t* GLw = (uint64_t*)(CHu + (m + 1));
     uint64_t* GUw = GLw + nw;
+    uint64_t* TLw = GUw + nw;  // transitions: bit p = g[p] xor g[p - 1]
+    uint64_t* TUw = TLw + nw;
     double* slab = (double*)((char*)a.slab + (size_t)blockIdx.x * a.slab_bytes_per_block);
     double* PQ = slab;  // 8 arrays of kml: (2*stat + dir)
     double* Un = slab + 8 * (size_t)kml;
@@ -815,6 +819,13 @@ __global__ void __launch_bounds__(NT)
             }
         }
         __syncthreads();
+        for (int wi = threadIdx.x; wi < nw; wi += NT) {
+            const uint64_t pl = wi ? GLw[wi - 1] >> 63 : 0ull;
+            const uint64_t pu = wi ? GUw[wi - 1] >> 63 : 0ull;
+            TLw[wi] = GLw[wi] ^ ((GLw[wi] << 1) | pl);
+            TUw[wi] = GUw[wi] ^ ((GUw[wi] << 1) | pu);
+        }
+        __syncthreads();
 
         // 7. per-window bound
         const double scale = ldexp(1.0, -F);
@@ -827,7 +838,7 @@ __global__ void __launch_bounds__(NT)
             } else if (!finite) {
                 lb = __builtin_nan("");
             } else {
-                const int64_t z = run_sum(GLw, CHl, j, m) + run_sum(GUw, CHu, j, m);
+                const int64_t z = trans_sum(GLw, TLw, CHl, j, m) + trans_sum(GUw, TUw, CHu, j, m);
                 const double zd = __ll2double_rd(z) * scale;
                 if (metric == DTW || metric == ERP) lb = sqrt(fmax(0.0, zd));
                 else if (metric == LCSS) lb = fmax(0.0, zd / (double)m);
