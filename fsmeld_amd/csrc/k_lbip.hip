@@ -57,13 +57,22 @@ size_t lbip_slab_bytes(const Geometry& G) {
     return (b + 255) & ~(size_t)255;
 }
 
-// Threshold selection by buckets (select_threshold): kThrBuckets buckets of
-// value-ordered keys, and a candidate list of up to thr_cap(npad) keys.
-constexpr int kThrBuckets = 256;
-__host__ __device__ inline int thr_cap(int npad) { return npad / 8 > 64 ? npad / 8 : 64; }
-__host__ __device__ inline size_t thr_lds_bytes(int npad) {
-    return sizeof(double) * (size_t)thr_cap(npad) +
-           (size_t)kThrBuckets * (2 * sizeof(uint64_t) + 4 * sizeof(int));
+// Threshold selection by buckets (select_threshold): thr_buckets buckets of
+// value-ordered keys and a candidate list of up to thr_cap keys.  A
+// 256-thread workgroup with 4096 keys takes the smaller figures, so that its
+// LDS (about 38 KiB) lets four workgroups share a CU instead of three.
+constexpr int kThrBucketsMax = 256;
+__host__ __device__ inline bool thr_small(int npad, int nt) { return nt < 1024 && npad >= 4096; }
+__host__ __device__ inline int thr_buckets(int npad, int nt) {
+    return thr_small(npad, nt) ? 128 : kThrBucketsMax;
+}
+__host__ __device__ inline int thr_cap(int npad, int nt) {
+    const int c = thr_small(npad, nt) ? npad / 16 : npad / 8;
+    return c > 64 ? c : 64;
+}
+__host__ __device__ inline size_t thr_lds_bytes(int npad, int nt) {
+    return sizeof(double) * (size_t)thr_cap(npad, nt) +
+           (size_t)thr_buckets(npad, nt) * (2 * sizeof(uint64_t) + 4 * sizeof(int));
 }
 
 size_t lbip_lds_bytes(const Geometry& G) {
@@ -72,7 +81,7 @@ size_t lbip_lds_bytes(const Geometry& G) {
     const int nw = (G.m_k + 63) / 64;
     const size_t late = sizeof(int64_t) * 2 * (size_t)(G.m + 1) + sizeof(uint64_t) * 4 * (size_t)nw;
     const size_t k = keys_in_lds(npad, lbip_threads(npad)) ? sizeof(double) * (size_t)npad : 0;
-    return std::max(late, k + thr_lds_bytes(npad));
+    return std::max(late, k + thr_lds_bytes(npad, lbip_threads(npad)));
 }
 
 // ---------------------------------------------------------------- helpers ---
@@ -381,7 +390,7 @@ __device__ void block_min_max(double* lo, double* hi, double* sh) {
 // and herr(a) bounds |computed h(t) - exact h(t)| for |t| <= a.
 //
 // Buckets.  Keys map monotonically to value-ordered buckets (0: -inf, 1 ..
-// kThrBuckets-2: equal widths of the finite range, last: +inf), each with its
+// nbk-2: equal widths of the finite range, last: +inf), each with its
 // count and its min / max as order-preserving integers.  A bucket's keys take
 // the ranks [start, end) of the full order, and its largest key sits at rank
 // end - 1, so h(max) end is a score the full order attains.  Every score in
@@ -396,15 +405,15 @@ __device__ inline unsigned long long ord_key(double v) {
 __device__ inline double key_of(unsigned long long o) {
     return __longlong_as_double((long long)((o >> 63) ? (o & 0x7fffffffffffffffull) : ~o));
 }
-__device__ inline int bucket_of(double v, double lo, double scale) {
+__device__ inline int bucket_of(double v, double lo, double scale, int nb) {
     if (v == -__builtin_inf()) return 0;
-    if (v == __builtin_inf()) return kThrBuckets - 1;
+    if (v == __builtin_inf()) return nb - 1;
     const double x = (v - lo) * scale;  // non-decreasing in v (NaN only when scale is 0)
-    return 1 + (int)fmin(fmax(x, 0.0), (double)(kThrBuckets - 3));
+    return 1 + (int)fmin(fmax(x, 0.0), (double)(nb - 3));
 }
 
 // keys[0 .. nthr): the values, NaN for positions that do not count; [lo, hi]:
-// the finite keys' range.  tlds: thr_lds_bytes(npad) of LDS.  The full sort
+// the finite keys' range.  tlds: thr_lds_bytes(npad, NT) of LDS.  The full sort
 // (fallback) reorders keys[0 .. npad) in place.
 template <int NT, class HF, class EF>
 __device__ double select_threshold(double* keys, int nthr, int npad, int nval, double lo,
@@ -412,18 +421,19 @@ __device__ double select_threshold(double* keys, int nthr, int npad, int nval, d
                                    ArgBest* sh_best, int* sh_cnt, int64_t* sh_scan,
                                    double* sh_key) {
     if (nval == 0) return __builtin_nan("");
+    const int nbk = thr_buckets(npad, NT);
     const int tid = threadIdx.x;
-    const int cap = thr_cap(npad);
+    const int cap = thr_cap(npad, NT);
     double* cand = (double*)tlds;
     unsigned long long* bmin = (unsigned long long*)(cand + cap);
-    unsigned long long* bmax = bmin + kThrBuckets;
-    int* cnt = (int*)(bmax + kThrBuckets);
-    int* bst = cnt + kThrBuckets;  // first rank
-    int* cst = bst + kThrBuckets;  // first candidate slot; -1: bucket dropped
-    int* fil = cst + kThrBuckets;
+    unsigned long long* bmax = bmin + nbk;
+    int* cnt = (int*)(bmax + nbk);
+    int* bst = cnt + nbk;  // first rank
+    int* cst = bst + nbk;  // first candidate slot; -1: bucket dropped
+    int* fil = cst + nbk;
     const double scale =
-        (hi > lo && hi - lo < __builtin_inf()) ? (double)(kThrBuckets - 2) / (hi - lo) : 0.0;
-    for (int b = tid; b < kThrBuckets; b += NT) {
+        (hi > lo && hi - lo < __builtin_inf()) ? (double)(nbk - 2) / (hi - lo) : 0.0;
+    for (int b = tid; b < nbk; b += NT) {
         cnt[b] = 0;
         fil[b] = 0;
         bmin[b] = ~0ull;
@@ -433,17 +443,17 @@ __device__ double select_threshold(double* keys, int nthr, int npad, int nval, d
     for (int i = tid; i < nthr; i += NT) {
         const double v = keys[i];
         if (v != v) continue;
-        const int b = bucket_of(v, lo, scale);
+        const int b = bucket_of(v, lo, scale, nbk);
         const unsigned long long o = ord_key(v);
         atomicAdd(&cnt[b], 1);
         atomicMin(&bmin[b], o);
         atomicMax(&bmax[b], o);
     }
     __syncthreads();
-    const int c = (tid < kThrBuckets) ? cnt[tid] : 0;
+    const int c = (tid < nbk) ? cnt[tid] : 0;
     const int st = (int)block_exscan(c, sh_scan);
-    if (tid < kThrBuckets) bst[tid] = st;
-    if (tid < kThrBuckets && c > 0 && st == 0) *sh_key = key_of(bmin[tid]);  // rank 0
+    if (tid < nbk) bst[tid] = st;
+    if (tid < nbk && c > 0 && st == 0) *sh_key = key_of(bmin[tid]);  // rank 0
     __syncthreads();
     const double kmin = *sh_key;
     __syncthreads();
@@ -452,7 +462,7 @@ __device__ double select_threshold(double* keys, int nthr, int npad, int nval, d
 
     ArgBest att{__builtin_nan(""), 0};
     double ub = 0.0;
-    if (tid < kThrBuckets && c > 0) {
+    if (tid < nbk && c > 0) {
         const double vmn = key_of(bmin[tid]), vmx = key_of(bmax[tid]);
         const int e = st + c;
         att = ArgBest{hf(vmx) * (double)e, (int64_t)(e - 1)};
@@ -461,18 +471,18 @@ __device__ double select_threshold(double* keys, int nthr, int npad, int nval, d
         ub += fabs(ub) * 1e-12 + 1e-300;
     }
     const ArgBest lb = block_best<true>(att, sh_best);
-    const bool keep = tid < kThrBuckets && c > 0 && !(ub < lb.v);
+    const bool keep = tid < nbk && c > 0 && !(ub < lb.v);
     const int kc = keep ? c : 0;
     const int cs = (int)block_exscan(kc, sh_scan);
     const int K = block_count(kc, sh_cnt);
-    if (tid < kThrBuckets) cst[tid] = keep ? cs : -1;
+    if (tid < nbk) cst[tid] = keep ? cs : -1;
     __syncthreads();
     if (K <= cap) {
         const int kpad = npad_of(K);
         for (int i = tid; i < nthr; i += NT) {
             const double v = keys[i];
             if (v != v) continue;
-            const int b = bucket_of(v, lo, scale);
+            const int b = bucket_of(v, lo, scale, nbk);
             const int o = cst[b];
             if (o >= 0) cand[o + atomicAdd(&fil[b], 1)] = v;
         }
@@ -484,7 +494,7 @@ __device__ double select_threshold(double* keys, int nthr, int npad, int nval, d
         double bkey = kmin;
         for (int j = tid; j < K; j += NT) {
             const double v = cand[j];
-            const int b = bucket_of(v, lo, scale);
+            const int b = bucket_of(v, lo, scale, nbk);
             const int64_t p = (int64_t)bst[b] + (j - cst[b]);
             const ArgBest x{hf(v) * (double)(p + 1), p};
             const ArgBest nb = better_max(best, x);
