@@ -15,9 +15,12 @@ __device__ inline bool lbip_exceeds_cutoff(double bound, double cutoff, int metr
 }
 
 // ------------------------------------------------------------- compaction ---
-// Each thread tests kCompactIpt consecutive windows; a block scan orders the
-// survivors of its chunk, one atomicAdd per chunk claims their output slots,
-// so every chunk's survivors are written ascending and contiguously.
+// A block takes chunks of 256 * kCompactIpt windows; at step e its threads
+// test the 256 consecutive windows base + 256 e + t (coalesced loads), and
+// each wave's ballot marks the kept ones.  The (e, wave) masks, in that order,
+// list the chunk's windows ascending, so a scan of their popcounts and one
+// atomicAdd per chunk give every survivor its slot: each chunk's survivors
+// are written ascending and contiguously.
 constexpr int kCompactIpt = 16;
 
 __global__ void __launch_bounds__(256) k_compact(Geometry G, QueryParams P,
@@ -25,44 +28,51 @@ __global__ void __launch_bounds__(256) k_compact(Geometry G, QueryParams P,
                                                  const double* __restrict__ LB, int use_lb,
                                                  int64_t* __restrict__ out,
                                                  unsigned long long* counters) {
-    __shared__ int sh_tot[4];
+    constexpr int kMasks = kCompactIpt * (256 / 64);
+    static_assert(kMasks == 64, "one wave scans the masks");
+    __shared__ uint64_t sh_mask[kMasks];
+    __shared__ int sh_pre[kMasks];
     __shared__ unsigned long long sh_base;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int64_t chunk = (int64_t)blockDim.x * kCompactIpt;
     const int64_t nchunks = (G.ts_n + chunk - 1) / chunk;
+    const uint64_t below = (1ull << lane) - 1;  // lanes before this one
     unsigned long long nvalid = 0;
     for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const int64_t g0 = c * chunk + (int64_t)threadIdx.x * kCompactIpt;
-        uint32_t bits = 0;
+        const int64_t base = c * chunk;
+        uint32_t bits = 0;  // bit e: window base + 256 e + threadIdx.x is kept
         for (int e = 0; e < kCompactIpt; e++) {
-            const int64_t g = g0 + e;
+            const int64_t g = base + (int64_t)blockDim.x * e + threadIdx.x;
+            bool keep = false;
             if (g < G.ts_n) {
                 const bool valid = !INV[stat_index(G, g)] && !is_excluded(P, g);
                 nvalid += valid;
-                if (valid && (!use_lb || !lbip_exceeds_cutoff(LB[g], P.cutoff, P.metric)))
-                    bits |= 1u << e;
+                keep = valid && (!use_lb || !lbip_exceeds_cutoff(LB[g], P.cutoff, P.metric));
             }
+            const uint64_t mask = __ballot(keep);
+            if (lane == 0) sh_mask[e * (256 / 64) + wid] = mask;
+            bits |= (uint32_t)keep << e;
         }
-        const int cnt = __builtin_popcount(bits);
-        int inc = cnt;  // inclusive wave scan
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += y;
-        }
-        if (lane == 63) sh_tot[wid] = inc;
         __syncthreads();
-        int woff = 0, tot = 0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
-            if (k < wid) woff += sh_tot[k];
-            tot += sh_tot[k];
+        if (threadIdx.x < kMasks) {  // one wave: kMasks == 64
+            const int cnt = __builtin_popcountll(sh_mask[threadIdx.x]);
+            int inc = cnt;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += y;
+            }
+            sh_pre[threadIdx.x] = inc - cnt;
+            if (threadIdx.x == kMasks - 1)
+                sh_base = inc ? atomicAdd(&counters[0], (unsigned long long)inc) : 0ull;
         }
-        if (threadIdx.x == 0) sh_base = tot ? atomicAdd(&counters[0], (unsigned long long)tot) : 0ull;
         __syncthreads();
-        unsigned long long pos = sh_base + (unsigned long long)(woff + inc - cnt);
         while (bits) {
             const int e = __builtin_ctz(bits);
             bits &= bits - 1;
-            out[pos++] = g0 + e;
+            const int k = e * (256 / 64) + wid;
+            const unsigned long long pos =
+                sh_base + (unsigned long long)sh_pre[k] + __builtin_popcountll(sh_mask[k] & below);
+            out[pos] = base + (int64_t)blockDim.x * e + threadIdx.x;
         }
         __syncthreads();
     }
