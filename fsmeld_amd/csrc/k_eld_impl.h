@@ -182,13 +182,53 @@ constexpr int kRefillMin = 8;  // idle lanes that trigger a wave's refill
 // and dB = dA + 1 (odd slots 2r+1, cells (b + 1 + r, b + W - r)); the rings
 // give X[k] = x[b - 1 + k], Y[k] = y[b - 1 + k].  A cell reads x[i] = X[r+1]
 // (x[i-1] = X[r]) and y[j] = Y[j - b + 1] (y[j-1] = Y[j - b]).
+// Register-band state.  EDR and LCSS values are edit and match counts:
+// integers far below 2^24, so 32-bit integer arithmetic gives the reference's
+// values exactly (its doubles hold the same integers), with half the registers
+// and fewer instructions per cell.  Their sample comparisons stay in FP64.
+// The out-of-band neighbour is a sentinel no reachable value exceeds (EDR) or
+// 0 (LCSS, where cells take maxima).
+template <int METRIC> struct EldState {
+    using type = double;
+    static constexpr int waves = 2;  // waves per SIMD the register kernel is built for
+};
+template <> struct EldState<EDR> {
+    using type = int;
+    static constexpr int waves = 3;
+};
+template <> struct EldState<LCSS> {
+    using type = int;
+    static constexpr int waves = 3;
+};
+template <int METRIC> using eld_t = typename EldState<METRIC>::type;
+constexpr int kEdrSentinel = 1 << 29;
+__device__ __forceinline__ int band_min(int a, int b) { return min(a, b); }
+__device__ __forceinline__ double band_min(double a, double b) { return fmin(a, b); }
+template <int METRIC>
+__device__ __forceinline__ eld_t<METRIC> band_sentinel() {
+    if constexpr (METRIC == LCSS) return 0;
+    else if constexpr (METRIC == EDR) return kEdrSentinel;
+    else return kEldSentinel;
+}
+
 // Slot update of the even anti-diagonal (A[r], cell (b + r, b + W - r)).
 template <int METRIC, int W>
-__device__ __forceinline__ void step_a(int r, double arg, double* A, const double* B,
-                                       const double* X, const double* Y) {
-    const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;  // out-of-band neighbour
-    const double up = (r >= 1) ? B[r - 1] : S;
-    const double left = (r <= W - 1) ? B[r] : S;
+__device__ __forceinline__ void step_a(int r, double arg, eld_t<METRIC>* A,
+                                       const eld_t<METRIC>* B, const double* X,
+                                       const double* Y) {
+    using VT = eld_t<METRIC>;
+    const VT S = band_sentinel<METRIC>();  // out-of-band neighbour
+    const VT up = (r >= 1) ? B[r - 1] : S;
+    const VT left = (r <= W - 1) ? B[r] : S;
+    if constexpr (METRIC == EDR) {  // x[i-1] = X[r], y[j-1] = Y[W - r]
+        const int sub = (fabs(X[r] - Y[W - r]) <= arg) ? 0 : 1;
+        A[r] = min(min(up, left) + 1, A[r] + sub);
+        return;
+    } else if constexpr (METRIC == LCSS) {  // x[i] = X[r + 1], y[j] = Y[W - r + 1]
+        const int hit = (fabs(X[r + 1] - Y[W - r + 1]) <= arg) ? 1 : 0;
+        A[r] = max(A[r] + hit, max(up, left));
+        return;
+    }
     if (METRIC == MSM) {
         A[r] = msm_cell(up, left, A[r], X[r + 1] - Y[W - r + 1], X[r + 1] - X[r],
                         Y[W - r + 1] - Y[W - r], arg);
@@ -201,8 +241,17 @@ __device__ __forceinline__ void step_a(int r, double arg, double* A, const doubl
 
 // Slot update of the odd anti-diagonal (B[r], cell (b + 1 + r, b + W - r)).
 template <int METRIC, int W>
-__device__ __forceinline__ void step_b(int r, double arg, const double* A, double* B,
-                                       const double* X, const double* Y) {
+__device__ __forceinline__ void step_b(int r, double arg, const eld_t<METRIC>* A,
+                                       eld_t<METRIC>* B, const double* X, const double* Y) {
+    if constexpr (METRIC == EDR) {  // x[i-1] = X[r + 1], y[j-1] = Y[W - r]
+        const int sub = (fabs(X[r + 1] - Y[W - r]) <= arg) ? 0 : 1;
+        B[r] = min(min(A[r], A[r + 1]) + 1, B[r] + sub);
+        return;
+    } else if constexpr (METRIC == LCSS) {  // x[i] = X[r + 2], y[j] = Y[W - r + 1]
+        const int hit = (fabs(X[r + 2] - Y[W - r + 1]) <= arg) ? 1 : 0;
+        B[r] = max(B[r] + hit, max(A[r], A[r + 1]));
+        return;
+    }
     if (METRIC == MSM) {
         B[r] = msm_cell(A[r], A[r + 1], B[r], X[r + 2] - Y[W - r + 1], X[r + 2] - X[r + 1],
                         Y[W - r + 1] - Y[W - r], arg);
@@ -219,8 +268,8 @@ __device__ __forceinline__ void step_b(int r, double arg, const double* A, doubl
 // which keeps the values the two cells share (row / column differences)
 // live only briefly.
 template <int METRIC, int W>
-__device__ __forceinline__ void ad_step(double arg, double* A, double* B, const double* X,
-                                        const double* Y) {
+__device__ __forceinline__ void ad_step(double arg, eld_t<METRIC>* A, eld_t<METRIC>* B,
+                                        const double* X, const double* Y) {
     step_a<METRIC, W>(0, arg, A, B, X, Y);
 #pragma unroll
     for (int r = 1; r <= W; r++) {
@@ -261,23 +310,25 @@ __device__ inline double consume_lb(double y, double lo, double hi, double arg) 
 // column (LCSS: matches beyond the smallest), and its rounded sum loses at
 // most (2m + 24) 1.2e-16 relative.
 template <int METRIC, int W>
-__device__ __forceinline__ bool ad_alive(const double* A, const double* B, int b, int M, int m,
-                                         double thr, double total, double p) {
-    if (METRIC == LCSS) {
-        double mx = A[0];
+__device__ __forceinline__ bool ad_alive(const eld_t<METRIC>* A, const eld_t<METRIC>* B, int b,
+                                         int M, int m, double thr, double total, double p) {
+    if constexpr (METRIC == LCSS) {
+        int mxi = A[0];
 #pragma unroll
-        for (int k = 1; k <= W; k++) mx = fmax(mx, A[k]);
+        for (int k = 1; k <= W; k++) mxi = max(mxi, A[k]);
 #pragma unroll
-        for (int k = 0; k < W; k++) mx = fmax(mx, B[k]);
+        for (int k = 0; k < W; k++) mxi = max(mxi, B[k]);
+        const double mx = (double)mxi;
         // matches still possible after a cell of anti-diagonal >= 2b+W
         const double rem = fmin((double)((2 * M - (2 * b + W)) / 2 + 1), total - p);
         return !(1.0 - (mx + rem) / (double)m > thr);
     }
-    double mn = A[0];
+    eld_t<METRIC> mnv = A[0];
 #pragma unroll
-    for (int k = 1; k <= W; k++) mn = fmin(mn, A[k]);
+    for (int k = 1; k <= W; k++) mnv = band_min(mnv, A[k]);
 #pragma unroll
-    for (int k = 0; k < W; k++) mn = fmin(mn, B[k]);
+    for (int k = 0; k < W; k++) mnv = band_min(mnv, B[k]);
+    const double mn = (double)mnv;
     const double cb = fmax(total - p * (1.0 + (double)(m + 8) * 1.2e-16), 0.0);
     return !((mn + cb) * (1.0 - (double)(2 * m + 24) * 1.2e-16) > thr);
 }
@@ -319,11 +370,13 @@ __device__ inline void wave_append(bool keep, int64_t g, double d, int64_t* out_
 // in[] lists window indices (target mode, in_total[] the GLB pass's bounds or
 // null) or, with rows != null, rows 0 .. n_direct-1 of normalised samples.
 // W is the query's radius w: every slot of both anti-diagonals is in the band.
-// Two waves per SIMD: the query side is read from LDS, so the state that
-// stays in registers (two anti-diagonals and the candidate ring) fits half
-// the register file.
+// Two waves per SIMD (three for the integer EDR / LCSS state): the query
+// side is read from LDS, so the state that stays in registers (two
+// anti-diagonals and the candidate ring) fits half (a third) of the register
+// file.
 template <int METRIC, int W>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(EldState<METRIC>::waves)))
 k_eld(VerifyArgs a, const double* __restrict__ LQ,
                                              const double* __restrict__ UQ,
                                              const int64_t* __restrict__ in,
@@ -356,8 +409,9 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
     }
     const bool check = thr < 1e300;
     const bool raw = rows == nullptr;
-    const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;
-    const double V = (METRIC == EDR && !(arg >= 0.0)) ? -1.0 : 0.0;
+    using VT = eld_t<METRIC>;
+    const VT S = band_sentinel<METRIC>();
+    const VT V = (METRIC == EDR && !(arg >= 0.0)) ? -1 : 0;
 
     const int span = m + 2 * kXPad;
     double* xs = xlds + kXPad;
@@ -377,7 +431,8 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
     const int lane = threadIdx.x & 63;
     // the query side X[k] = xs[b - 1 + k] is read from LDS (lanes run at
     // their own iteration b); the candidate ring Y stays in registers
-    double A[W + 1], B[W + 1], Y[W + 2];
+    VT A[W + 1], B[W + 1];
+    double Y[W + 2];
 #pragma unroll
     for (int k = 0; k <= W; k++) A[k] = B[k] = S;
 #pragma unroll
@@ -451,7 +506,7 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
         live_iters += busy ? 1u : 0u;
         if (busy) b++;
         if (__any(fin)) {
-            const double d = eld_result<METRIC>((W & 1) ? B[(W - 1) / 2] : A[W / 2], m);
+            const double d = eld_result<METRIC>((double)((W & 1) ? B[(W - 1) / 2] : A[W / 2]), m);
             if (keep_all) {
                 if (fin) out_dist[ci] = d;
             } else {
