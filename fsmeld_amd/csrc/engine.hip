@@ -412,7 +412,7 @@ int tighten_cutoff(fsm_target* t, const Geometry& G, const QueryTables& Q, Query
 }
 
 // counters: [0] compacted, [1] valid, [2] after GLB, [3] ELD kept, [8] ELD work
-// queue, [9] ELD lane-iterations on live candidates.  The LB buffer is free once compaction has read it; GLB writes each
+// queue, [9] ELD lane-iterations on live candidates, [10] GLB work queue.  The LB buffer is free once compaction has read it; GLB writes each
 // survivor's consumption bound there for the ELD abandon test.
 // cutoff mode (tighten_k > 0): when GLB leaves many candidates at the first
 // bound, the tighten_k-th smallest distance among the few thousand survivors
@@ -443,7 +443,7 @@ int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
     if (use_glb) {
         HIP_TRY(t->lb.ensure(sizeof(double) * (size_t)G.ts_n));
         HIP_TRY(launch_glb(t->stream, v, t->list_a.as<int64_t>(), C, t->list_b.as<int64_t>(),
-                           C + 2, t->lb.as<double>(), nullptr));
+                           C + 2, t->lb.as<double>(), nullptr, C + 10));
         eld_in = t->list_b.as<int64_t>();
         eld_cnt = C + 2;
         eld_total = t->lb.as<double>();
@@ -456,7 +456,8 @@ int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
                 HIP_TRY(t->aux.ensure(sizeof(double) * (size_t)G.ts_n));
                 HIP_TRY(hipMemsetAsync(C + 5, 0, sizeof(unsigned long long), t->stream));
                 HIP_TRY(launch_glb(t->stream, v, t->list_b.as<int64_t>(), C + 2,
-                                   t->list_a.as<int64_t>(), C + 5, t->aux.as<double>(), nullptr));
+                                   t->list_a.as<int64_t>(), C + 5, t->aux.as<double>(), nullptr,
+                                   C + 10));
                 eld_in = t->list_a.as<int64_t>();
                 eld_cnt = C + 5;
                 eld_total = t->aux.as<double>();
@@ -835,10 +836,10 @@ static int stage_verify(fsm_target* t, const fsm_query_desc* d, const double* q,
     VerifyArgs v = verify_args(t, G, Q, P);
     if (kind == STAGE_GLB_VALUES)
         HIP_TRY(launch_glb(t->stream, v, t->list_a.as<int64_t>(), Cd + 6, nullptr, nullptr,
-                           nullptr, t->out_dist.as<double>()));
+                           nullptr, t->out_dist.as<double>(), Cd + 10));
     else if (kind == STAGE_GLB_KEEP)
         HIP_TRY(launch_glb(t->stream, v, t->list_a.as<int64_t>(), Cd + 6, t->list_b.as<int64_t>(),
-                           Cd + 7, nullptr, nullptr));
+                           Cd + 7, nullptr, nullptr, Cd + 10));
     else
         HIP_TRY(launch_eld(t->stream, v, t->list_a.as<int64_t>(), nullptr, Cd + 6, true, nullptr,
                            t->out_dist.as<double>(), nullptr, Cd + 8, nullptr,

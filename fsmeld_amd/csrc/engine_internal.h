@@ -156,10 +156,12 @@ struct VerifyArgs {
 // GLB filter over in[0 .. *in_count): survivors appended to out (out_count),
 // with, when out_total != null, each survivor's certified consumption bound
 // for the ELD abandon test beside it (out_total[k] belongs to out[k]); values
-// written to vals[i] when vals != null (stage-level entry point).
+// written to vals[i] when vals != null (stage-level entry point).  queue: one
+// device counter the launch resets and uses to hand out candidates.
 hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
                       const unsigned long long* in_count, int64_t* out,
-                      unsigned long long* out_count, double* out_total, double* vals);
+                      unsigned long long* out_count, double* out_total, double* vals,
+                      unsigned long long* queue);
 // Exact distances over in[0 .. *in_count) (in_total: the GLB bounds of in[],
 // or null).  keep_all: write dist[i] for every input; else append (index,
 // distance) for distances <= P.cutoff.  queue: one device counter the launch

@@ -471,24 +471,33 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                                              int64_t* __restrict__ out,
                                              unsigned long long* out_count,
                                              double* __restrict__ out_total,
-                                             double* __restrict__ vals, int stage_len) {
+                                             double* __restrict__ vals,
+                                             unsigned long long* __restrict__ queue,
+                                             int stage_len) {
     __shared__ unsigned long long sh[256 / 64 + 1];
     __shared__ int64_t rng[2 * (256 / 64)];
+    __shared__ unsigned long long chunk;
     // staged samples / envelope of the block's windows (3 x stage_len doubles)
     extern __shared__ double glb_stage[];
     double* Ts = glb_stage;
     double* Ls = Ts + stage_len;
     double* Us = Ls + stage_len;
     const int64_t n = (int64_t)*in_count;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t iters = (n + stride - 1) / stride;
     // the fast decision needs contributions >= 0 and, for EDR / LCSS, an
     // envelope that counts as a match (eps > 0); values (vals) need the exact path
     const bool fast = vals == nullptr && a.p.cutoff < kGlbAbandon && a.p.m > 2 &&
                       ((METRIC == DTW || METRIC == ERP) || (METRIC == MSM && a.p.arg >= 0.0) ||
                        a.p.arg > 0.0);
-    int64_t ci = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (int64_t it = 0; it < iters; it++, ci += stride) {
+    // Blocks take runs of blockDim.x consecutive candidates from the queue
+    // until it is drained: per-candidate work varies (early exits), so a
+    // fixed assignment leaves most of the GPU idle behind the slowest blocks.
+    while (true) {
+        if (threadIdx.x == 0) chunk = atomicAdd(queue, (unsigned long long)blockDim.x);
+        __syncthreads();
+        const int64_t base = (int64_t)chunk;
+        __syncthreads();  // every thread has read chunk before it is replaced
+        if (base >= n) break;
+        const int64_t ci = base + threadIdx.x;
         const bool active = ci < n;
         const int64_t g = active ? in[ci] : 0;
         const int64_t si = stat_index(a.G, g);
@@ -561,17 +570,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 static int verify_blocks() { return 2048; }
 constexpr int kGlbStage = 1664;
 
+__global__ void k_glb_queue_reset(unsigned long long* q) { *q = 0ull; }
+
 hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
                       const unsigned long long* in_count, int64_t* out,
-                      unsigned long long* out_count, double* out_total, double* vals) {
+                      unsigned long long* out_count, double* out_total, double* vals,
+                      unsigned long long* queue) {
+    if (!queue) return hipErrorInvalidValue;
     const dim3 g(verify_blocks()), b(256);
+    hipLaunchKernelGGL(k_glb_queue_reset, dim3(1), dim3(1), 0, st, queue);
     const QueryTables& q = a.q;
     // staging: 3 arrays of up to kGlbStage doubles (39 KiB, four blocks per CU)
     // when the block's 256 windows leave at least 256 more of the span
     const int stage_len = (a.p.m + 512 <= kGlbStage) ? kGlbStage : 0;
     const size_t lds = sizeof(double) * 3 * (size_t)stage_len;
 #define FSM_GLB(MM) \
-    hipLaunchKernelGGL(k_glb<MM>, g, b, lds, st, a, q.qn, q.LQ, q.UQ, q.XG, in, in_count, out, out_count, out_total, vals, stage_len)
+    hipLaunchKernelGGL(k_glb<MM>, g, b, lds, st, a, q.qn, q.LQ, q.UQ, q.XG, in, in_count, out, out_count, out_total, vals, queue, stage_len)
     switch (a.p.metric) {
         case DTW: FSM_GLB(DTW); break;
         case ERP: FSM_GLB(ERP); break;
