@@ -15,6 +15,8 @@
 // this file is compiled with -fno-honor-nans.
 #include "engine_internal.h"
 
+#include <type_traits>
+
 namespace fsm {
 
 __device__ inline double sqd(double x, double y) { return (x - y) * (x - y); }  // include/def.h:5
@@ -271,6 +273,9 @@ __device__ inline double glb_sum_err(double S, int n, double delta, double K) {
 }
 
 constexpr int kGlbChunk = 16;  // positions between wave-level exit tests
+// From this radius on, the bound on |normalised sample| comes from the
+// envelope at every (2w + 1)-th position instead of every sample.
+constexpr int kGlbSampleW = 4;
 
 // Contribution of one position with envelope distance e >= 0 (fast path).
 // xg2 = (x - g)^2 of the moving sample (ERP); for EDR / LCSS, r = e - eps
@@ -323,8 +328,26 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
         B = glb_boundary<METRIC>((t[0] - mu) / sd, (t[n - 1] - mu) / sd, (t[n - 2] - mu) / sd, Q,
                                  n, arg);
     // amax bounds |a| over every sample of the window; the candidate
-    // envelope's normalised values are such samples (normalisation is monotone)
-    double amax = fmax(fabs((t[0] - mu) * r), fabs((t[n - 1] - mu) * r));
+    // envelope's normalised values are such samples (normalisation is
+    // monotone).  With w >= kGlbSampleW it is the normalised window min / max:
+    // the envelope at positions w, 3w + 1, ... and n - 1 - w covers the window
+    // (each spans 2w + 1 samples), and (v - mu) * r is monotone in v, so the
+    // value equals the largest |a| over the samples pass 1 normalises.
+    // Otherwise pass 1 tracks it sample by sample.
+    const bool sampled = w >= kGlbSampleW && n >= 2 * w + 1;
+    double amax;
+    if (sampled) {
+        double lo = Lr[w], hi = Ur[w];
+        for (int p = 3 * w + 1; p < n - 1 - w; p += 2 * w + 1) {
+            lo = fmin(lo, Lr[p]);
+            hi = fmax(hi, Ur[p]);
+        }
+        lo = fmin(lo, Lr[n - 1 - w]);
+        hi = fmax(hi, Ur[n - 1 - w]);
+        amax = fmax(fabs((lo - mu) * r), fabs((hi - mu) * r));
+    } else {
+        amax = fmax(fabs((t[0] - mu) * r), fabs((t[n - 1] - mu) * r));
+    }
     const double kq = fmax(QM, fabs(arg));
     const double ceta = 8e-16 * (QM + fabs(arg)) + 1e-300;
     bool dead = !active;
@@ -332,9 +355,9 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
     // pass 1: candidate samples against the query envelope
     double s1 = 0.0;
     int sure1 = 0, uns1 = 0;
-    auto term1 = [&](int k) {
+    auto term1 = [&](int k, auto track) {
         const double av = (t[k] - mu) * r;
-        amax = fmax(amax, fabs(av));
+        if constexpr (decltype(track)::value) amax = fmax(amax, fabs(av));
         const double e = fmax(fmax(LQ[k] - av, av - UQ[k]), 0.0);
         const double xg = av - arg;
         fast_term<METRIC>(e, xg * xg, arg, fma(fabs(av), 8e-16, ceta), s1, sure1, uns1);
@@ -351,25 +374,37 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
             dead = B + lo > thr;
         }
     };
-    int i = i0;
-    for (; i + kGlbChunk <= i1; i += kGlbChunk) {
+    // returns true when every lane of the wave is dead
+    auto pass1 = [&](auto track) {
+        int i = i0;
+        for (; i + kGlbChunk <= i1; i += kGlbChunk) {
 #pragma unroll
-        for (int u = 0; u < kGlbChunk; u++) term1(i + u);
-        check1(i + kGlbChunk);
-        if (__all(dead)) return -1;
-    }
-    for (; i < i1; i++) term1(i);
+            for (int u = 0; u < kGlbChunk; u++) term1(i + u, track);
+            check1(i + kGlbChunk);
+            if (__all(dead)) return true;
+        }
+        for (; i < i1; i++) term1(i, track);
+        return false;
+    };
+    if (sampled ? pass1(std::false_type{}) : pass1(std::true_type{})) return -1;
 
     // pass 2: query against the candidate envelope (raw window min / max over
-    // positions clipped to the window, normalised like pass 1)
-    const double eta2 = 8e-16 * (QM + amax + fabs(arg)) + 1e-300;
+    // positions clipped to the window).  The distance is taken in raw units,
+    // e = max(lo - qr, qr - hi, 0) * r with qr = fl(q sd + mu), and differs
+    // from the reference's fl(q - UT) / fl(LT - q) on its normalised envelope
+    // by at most 6.02 u amax + 5.02 u QM + 1.01 u |mu| / sd (u = 2^-53):
+    // 2 u |a| from the reference's normalisation, 3 u e from the subtraction,
+    // r and the product, u |q + mu / sd| from qr, u e for the reference's own
+    // subtraction, e <= amax + QM).
+    const double mur = fabs(mu) * r;
+    const double delta2 = 7e-16 * (amax + QM) + 1.2e-16 * mur + 1e-300;
+    const double eta2 = 8e-16 * (QM + amax + fabs(arg)) + 1.3e-16 * mur + 1e-300;
     double s2 = 0.0;
     int sure2 = 0, uns2 = 0;
     int done2 = 0;
     auto term2 = [&](int k, double lo, double hi) {
-        const double aL = (lo - mu) * r, aU = (hi - mu) * r;
-        const double q = Q[k];
-        const double e = fmax(fmax(aL - q, q - aU), 0.0);
+        const double qr = fma(Q[k], sd, mu);
+        const double e = fmax(fmax(lo - qr, qr - hi), 0.0) * r;
         fast_term<METRIC>(e, (METRIC == ERP) ? XG[k] : 0.0, arg, eta2, s2, sure2, uns2);
     };
     auto check2 = [&]() {
@@ -380,7 +415,7 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
         } else if (METRIC == EDR) {
             dead = B + (double)sure2 > bsf;
         } else {
-            const double lo = s2 - glb_sum_err<METRIC>(s2, n, 4e-16 * amax + 1e-300, amax + kq);
+            const double lo = s2 - glb_sum_err<METRIC>(s2, n, delta2, amax + kq);
             dead = B + lo > thr;
         }
     };
@@ -440,7 +475,7 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
     } else {
         const double delta = 4e-16 * amax + 1e-300, K = amax + kq;
         const double e1 = glb_sum_err<METRIC>(s1, n, delta, K);
-        const double e2 = glb_sum_err<METRIC>(s2, n, delta, K);
+        const double e2 = glb_sum_err<METRIC>(s2, n, delta2, K);
         Tlo = fmax(s1 - e1, 0.0);
         Thi = s1 + e1;
         Qlo = fmax(s2 - e2, 0.0);
