@@ -187,7 +187,11 @@ constexpr int kRefillMin = 8;  // idle lanes that trigger a wave's refill
 // values exactly (its doubles hold the same integers), with half the registers
 // and fewer instructions per cell.  Their sample comparisons stay in FP64.
 // The out-of-band neighbour is a sentinel no reachable value exceeds (EDR) or
-// 0 (LCSS, where cells take maxima).
+// 0 (LCSS, where cells take maxima).  EDR slots hold D(i, j) - (i + j), the
+// same for a whole anti-diagonal, so D = min(up + 1, left + 1, diag + sub)
+// becomes min(up', left', diag' + sub - 2): one three-way min and one add per
+// cell on exactly the same integers (band_value adds i + j back).  The
+// sentinel loses at most 2 per anti-diagonal, far from reachable values.
 template <int METRIC> struct EldState {
     using type = double;
     static constexpr int waves = 2;  // waves per SIMD the register kernel is built for
@@ -204,6 +208,12 @@ template <int METRIC> using eld_t = typename EldState<METRIC>::type;
 constexpr int kEdrSentinel = 1 << 29;
 __device__ __forceinline__ int band_min(int a, int b) { return min(a, b); }
 __device__ __forceinline__ double band_min(double a, double b) { return fmin(a, b); }
+// value of a slot on anti-diagonal d (= i + j)
+template <int METRIC>
+__device__ __forceinline__ double band_value(eld_t<METRIC> v, int d) {
+    if constexpr (METRIC == EDR) return (double)(v + d);
+    else return (double)v;
+}
 template <int METRIC>
 __device__ __forceinline__ eld_t<METRIC> band_sentinel() {
     if constexpr (METRIC == LCSS) return 0;
@@ -222,7 +232,7 @@ __device__ __forceinline__ void step_a(int r, double arg, eld_t<METRIC>* A,
     const VT left = (r <= W - 1) ? B[r] : S;
     if constexpr (METRIC == EDR) {  // x[i-1] = X[r], y[j-1] = Y[W - r]
         const int sub = (fabs(X[r] - Y[W - r]) <= arg) ? 0 : 1;
-        A[r] = min(min(up, left) + 1, A[r] + sub);
+        A[r] = min(min(up, left), A[r] - 2 + sub);  // D - (i + j)
         return;
     } else if constexpr (METRIC == LCSS) {  // x[i] = X[r + 1], y[j] = Y[W - r + 1]
         const int hit = (fabs(X[r + 1] - Y[W - r + 1]) <= arg) ? 1 : 0;
@@ -245,7 +255,7 @@ __device__ __forceinline__ void step_b(int r, double arg, const eld_t<METRIC>* A
                                        eld_t<METRIC>* B, const double* X, const double* Y) {
     if constexpr (METRIC == EDR) {  // x[i-1] = X[r + 1], y[j-1] = Y[W - r]
         const int sub = (fabs(X[r + 1] - Y[W - r]) <= arg) ? 0 : 1;
-        B[r] = min(min(A[r], A[r + 1]) + 1, B[r] + sub);
+        B[r] = min(min(A[r], A[r + 1]), B[r] - 2 + sub);  // D - (i + j)
         return;
     } else if constexpr (METRIC == LCSS) {  // x[i] = X[r + 2], y[j] = Y[W - r + 1]
         const int hit = (fabs(X[r + 2] - Y[W - r + 1]) <= arg) ? 1 : 0;
@@ -323,12 +333,14 @@ __device__ __forceinline__ bool ad_alive(const eld_t<METRIC>* A, const eld_t<MET
         const double rem = fmin((double)((2 * M - (2 * b + W)) / 2 + 1), total - p);
         return !(1.0 - (mx + rem) / (double)m > thr);
     }
-    eld_t<METRIC> mnv = A[0];
+    eld_t<METRIC> mna = A[0], mnb = band_sentinel<METRIC>();
 #pragma unroll
-    for (int k = 1; k <= W; k++) mnv = band_min(mnv, A[k]);
+    for (int k = 1; k <= W; k++) mna = band_min(mna, A[k]);
 #pragma unroll
-    for (int k = 0; k < W; k++) mnv = band_min(mnv, B[k]);
-    const double mn = (double)mnv;
+    for (int k = 0; k < W; k++) mnb = band_min(mnb, B[k]);
+    // A holds anti-diagonal 2b + W, B the next
+    const double mn = fmin(band_value<METRIC>(mna, 2 * b + W),
+                           band_value<METRIC>(mnb, 2 * b + W + 1));
     const double cb = fmax(total - p * (1.0 + (double)(m + 8) * 1.2e-16), 0.0);
     return !((mn + cb) * (1.0 - (double)(2 * m + 24) * 1.2e-16) > thr);
 }
@@ -411,7 +423,9 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
     const bool raw = rows == nullptr;
     using VT = eld_t<METRIC>;
     const VT S = band_sentinel<METRIC>();
-    const VT V = (METRIC == EDR && !(arg >= 0.0)) ? -1 : 0;
+    // the virtual cell (-1, -1): 0, or -1 for EDR with eps < 0; EDR stores
+    // D - (i + j) = D + 2
+    const VT V = (METRIC == EDR) ? ((arg >= 0.0) ? 2 : 1) : 0;
 
     const int span = m + 2 * kXPad;
     double* xs = xlds + kXPad;
@@ -493,6 +507,17 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
         if (!__any(busy)) break;
 
         ad_step<METRIC, W>(arg, A, B, xs + (b - 1), Y);
+        if constexpr (METRIC == EDR || METRIC == LCSS) {
+            // The slots are materialised here: left to itself the compiler
+            // sinks the integer updates past the rest of the iteration and
+            // keeps all 2W + 1 comparison masks live in scalar registers,
+            // which then spill to vector lanes.
+#pragma unroll
+            for (int k = 0; k <= W; k++) {
+                asm volatile("" : "+v"(A[k]));
+                asm volatile("" : "+v"(B[k]));
+            }
+        }
         const int jn = b + W + 1;
         const int jj = min(max(jn, 0), m - 1);
         double yv = tp[jj];
@@ -506,7 +531,10 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
         live_iters += busy ? 1u : 0u;
         if (busy) b++;
         if (__any(fin)) {
-            const double d = eld_result<METRIC>((double)((W & 1) ? B[(W - 1) / 2] : A[W / 2]), m);
+            // cell (M, M) sits on anti-diagonal 2 bend + W (A) or + 1 (B)
+            const double d = eld_result<METRIC>(
+                band_value<METRIC>((W & 1) ? B[(W - 1) / 2] : A[W / 2], 2 * bend + W + (W & 1)),
+                m);
             if (keep_all) {
                 if (fin) out_dist[ci] = d;
             } else {
