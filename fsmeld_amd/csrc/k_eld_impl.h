@@ -69,11 +69,20 @@ __device__ inline double msm_cost(double np, double x, double y, double c) {
 // = c + max(min(dx, e), -max(dx, e), 0) and msm_cost(y_j, x_i, y_{j-1})
 // = c + max(min(-e, dy), -max(-e, dy), 0), bit for bit, sharing e with the
 // match cost |x_i - y_j|.
+//
+// Both terms have the form max(min(d, f), -max(d, f), 0), which equals
+// |clamp(f, min(0, d), max(0, d))|: for d > 0 both are clamp(f, 0, d); for
+// d < 0 both are clamp(-f, 0, -d); for d = 0 both are 0 (a zero's sign is
+// absorbed by c + s).  With f = -e and d = dy the clamp is
+// -clamp(e, min(0, -dy), max(0, -dy)), and dyn = y_{j-1} - y_j is exactly -dy.
+// The clamp bounds belong to a row (column) and serve the two cells of an
+// iteration that share it; min, max and |.| are exact, so the values are the
+// reference's bit for bit.
 __device__ inline double msm_cell(double up, double left, double diag, double e, double dx,
-                                  double dy, double c) {
+                                  double dyn, double c) {
     const double c1 = diag + fabs(e);
-    const double s2 = fmax(fmax(fmin(dx, e), -fmax(dx, e)), 0.0);
-    const double s3 = fmax(fmax(fmin(-e, dy), -fmax(-e, dy)), 0.0);
+    const double s2 = fabs(fmax(fmin(e, fmax(dx, 0.0)), fmin(dx, 0.0)));
+    const double s3 = fabs(fmax(fmin(e, fmax(dyn, 0.0)), fmin(dyn, 0.0)));
     const double c2 = up + (c + s2);
     const double c3 = left + (c + s3);
     return dmin3(c1, c2, c3);
@@ -241,7 +250,7 @@ __device__ __forceinline__ void step_a(int r, double arg, eld_t<METRIC>* A,
     }
     if (METRIC == MSM) {
         A[r] = msm_cell(up, left, A[r], X[r + 1] - Y[W - r + 1], X[r + 1] - X[r],
-                        Y[W - r + 1] - Y[W - r], arg);
+                        Y[W - r] - Y[W - r + 1], arg);
         return;
     }
     A[r] = cell<METRIC, false>(0, 0, up, left, A[r], X[r + 1], X[r], Y[W - r + 1], Y[W - r],
@@ -264,7 +273,7 @@ __device__ __forceinline__ void step_b(int r, double arg, const eld_t<METRIC>* A
     }
     if (METRIC == MSM) {
         B[r] = msm_cell(A[r], A[r + 1], B[r], X[r + 2] - Y[W - r + 1], X[r + 2] - X[r + 1],
-                        Y[W - r + 1] - Y[W - r], arg);
+                        Y[W - r] - Y[W - r + 1], arg);
         return;
     }
     B[r] = cell<METRIC, false>(0, 0, A[r], A[r + 1], B[r], X[r + 2], X[r + 1], Y[W - r + 1],
