@@ -78,11 +78,15 @@ __device__ inline double msm_cost(double np, double x, double y, double c) {
 // The clamp bounds belong to a row (column) and serve the two cells of an
 // iteration that share it; min, max and |.| are exact, so the values are the
 // reference's bit for bit.
+__device__ __forceinline__ double msm_split(double d, double e) {
+    return fabs(fmax(fmin(e, fmax(d, 0.0)), fmin(d, 0.0)));
+}
+
 __device__ inline double msm_cell(double up, double left, double diag, double e, double dx,
                                   double dyn, double c) {
     const double c1 = diag + fabs(e);
-    const double s2 = fabs(fmax(fmin(e, fmax(dx, 0.0)), fmin(dx, 0.0)));
-    const double s3 = fabs(fmax(fmin(e, fmax(dyn, 0.0)), fmin(dyn, 0.0)));
+    const double s2 = msm_split(dx, e);
+    const double s3 = msm_split(dyn, e);
     const double c2 = up + (c + s2);
     const double c3 = left + (c + s3);
     return dmin3(c1, c2, c3);
@@ -748,10 +752,11 @@ __device__ __forceinline__ double cell_sel(int i, int j, double up, double left,
         return i0 ? (double)j : (j0 ? (double)i : r);
     } else if (METRIC == LCSS) {
         return cell<LCSS, true>(i, j, up, left, diag, xi, xp, yj, yp, xg, yg, arg);
-    } else {  // MSM
-        const double t = fabs(xi - yj);
-        const double c2 = up + msm_cost(xi, xp, yj, arg);
-        const double c3 = left + msm_cost(yj, xi, yp, arg);
+    } else {  // MSM: msm_cost in the form of msm_cell (the same bits)
+        const double e = xi - yj;
+        const double t = fabs(e);
+        const double c2 = up + (arg + msm_split(xi - xp, e));
+        const double c3 = left + (arg + msm_split(yp - yj, e));
         const double r = dmin3(diag + t, c2, c3);
         return i0 ? (j0 ? t : c3) : (j0 ? c2 : r);
     }
