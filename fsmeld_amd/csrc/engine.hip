@@ -117,7 +117,7 @@ void host_envelope(const std::vector<double>& a, int r, std::vector<double>& L,
 }
 
 struct HostQuery {
-    std::vector<double> qn, LQ, UQ, qs, S1, S2, SD, TU, TL, XG;
+    std::vector<double> qn, LQ, UQ, qs, S1, S2, SD, TU, TL, XG, LQE, UQE;
 };
 
 // src/fsmeld.cpp:622-643 (normalisation and sorted copy), :721 (envelope).
@@ -135,6 +135,12 @@ void prepare_query(const double* q_raw, int m, int w, double arg, HostQuery* H) 
     host_envelope(H->qn, w, H->LQ, H->UQ);
     H->XG.resize(m);
     for (int i = 0; i < m; i++) H->XG[i] = (H->qn[i] - arg) * (H->qn[i] - arg);  // dist(x, g)
+    H->LQE.resize(m);
+    H->UQE.resize(m);
+    for (int i = 0; i < m; i++) {
+        H->LQE[i] = H->LQ[i] - arg;
+        H->UQE[i] = H->UQ[i] + arg;
+    }
     H->qs = H->qn;
     std::sort(H->qs.begin(), H->qs.end());
     H->S1.assign(m + 1, 0.0);
@@ -232,11 +238,11 @@ namespace {
 
 int upload_query(fsm_target* t, const HostQuery& H, int m, QueryTables* Q) {
     const size_t stride = (size_t)m + 1;
-    HIP_TRY(t->qtab.ensure(sizeof(double) * stride * 10));
-    std::vector<double> buf(stride * 10, 0.0);
-    const std::vector<double>* parts[10] = {&H.qn, &H.LQ, &H.UQ, &H.qs, &H.S1,
-                                            &H.S2, &H.SD, &H.TU, &H.TL, &H.XG};
-    for (int k = 0; k < 10; k++)
+    HIP_TRY(t->qtab.ensure(sizeof(double) * stride * 12));
+    std::vector<double> buf(stride * 12, 0.0);
+    const std::vector<double>* parts[12] = {&H.qn, &H.LQ, &H.UQ, &H.qs, &H.S1, &H.S2,
+                                            &H.SD, &H.TU, &H.TL, &H.XG, &H.LQE, &H.UQE};
+    for (int k = 0; k < 12; k++)
         std::copy(parts[k]->begin(), parts[k]->end(), buf.begin() + k * stride);
     HIP_TRY(hipMemcpyAsync(t->qtab.p, buf.data(), sizeof(double) * buf.size(),
                            hipMemcpyHostToDevice, t->stream));
@@ -251,6 +257,8 @@ int upload_query(fsm_target* t, const HostQuery& H, int m, QueryTables* Q) {
     Q->TU = d + 7 * stride;
     Q->TL = d + 8 * stride;
     Q->XG = d + 9 * stride;
+    Q->LQE = d + 10 * stride;
+    Q->UQE = d + 11 * stride;
     Q->qabs = 0.0;
     for (double v : H.qn) Q->qabs = std::max(Q->qabs, std::fabs(v));
     // the staging vector must outlive the async copy

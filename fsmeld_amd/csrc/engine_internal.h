@@ -41,6 +41,8 @@ struct QueryTables {
     const double* TU;   // fl(qs - arg), fl(qs + arg) [m]
     const double* TL;
     const double* XG;   // (qn - arg)^2 [m], the ERP gap cost of each query sample
+    const double* LQE;  // fl(LQ - arg), fl(UQ + arg) [m]: the envelope widened by eps
+    const double* UQE;
     double qabs;        // max |qn|, which also bounds |LQ| and |UQ|
 };
 

@@ -278,23 +278,16 @@ constexpr int kGlbChunk = 16;  // positions between wave-level exit tests
 constexpr int kGlbSampleW = 4;
 
 // Contribution of one position with envelope distance e >= 0 (fast path).
-// xg2 = (x - g)^2 of the moving sample (ERP); for EDR / LCSS, r = e - eps
-// decides mismatch / match unless |r| <= eta.
+// xg2 = (x - g)^2 of the moving sample (ERP).  DTW / ERP / MSM only; the
+// counting metrics take their own form in glb_fast.
 template <int METRIC>
-__device__ inline void fast_term(double e, double xg2, double arg, double eta, double& s,
-                                 int& sure, int& unsure) {
+__device__ inline void fast_term(double e, double xg2, double arg, double& s) {
     if (METRIC == DTW) {
         s = fma(e, e, s);
     } else if (METRIC == ERP) {
         s += fmin(e * e, xg2);
-    } else if (METRIC == MSM) {
-        s += fmin(e, arg);
     } else {
-        const double r = e - arg;
-        const bool u = fabs(r) <= eta;
-        unsure += u ? 1 : 0;
-        if (METRIC == EDR) sure += (!u && r > 0.0) ? 1 : 0;  // mismatch
-        else sure += (!u && r < 0.0) ? 1 : 0;                // match
+        s += fmin(e, arg);
     }
 }
 
@@ -307,10 +300,26 @@ __device__ inline void fast_term(double e, double xg2, double arg, double eta, d
 // lower bound on the real-valued sum over samples 1 .. n-2 of the per-sample
 // envelope terms (DTW / ERP / MSM), the count of certain mismatches (EDR),
 // or an upper bound on the samples that can match at all (LCSS).
+//
+// EDR / LCSS count positions.  With eps > 0 a position mismatches (EDR) or
+// fails to match (LCSS) exactly when R = max(L - a, a - U) - eps > 0 for its
+// envelope [L, U] (the clamp at 0 cannot reach eps), and the fast pass takes
+// R from the envelope widened by eps, LQE = fl(LQ - eps) and UQE = fl(UQ + eps),
+// in one max of two differences.  Against the reference's rounded quantities
+// (its own normalisation fl(fl(t - mu) / sd) and subtraction) the computed R
+// is off by at most 7.01 u |a| + 3 u max|q| + 2 u eps (pass 1: 3 u |a| from a,
+// u (max|q| + eps) from the widened envelope, u |R + eps| from the difference,
+// 3 u |a| + u max|q| on the reference's side), and pass 2, taken in raw units
+// against eps sd, by at most 5 u amax + 4 u max|q| + 2 u eps + u |mu| / sd.
+// eta = 1.2e-15 (|a| + max|q| + eps) (pass 2: plus 1.3e-16 |mu| / sd, scaled
+// by sd) covers both; a position with R > eta is certainly out, with R < -eta
+// certainly in, so counting R > eta and R >= -eta gives both ends of the
+// reference's count.
 template <int METRIC>
 __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __restrict__ Q,
                         const double* __restrict__ LQ, const double* __restrict__ UQ,
-                        const double* __restrict__ XG, const double* __restrict__ t,
+                        const double* __restrict__ XG, const double* __restrict__ LQE,
+                        const double* __restrict__ UQE, const double* __restrict__ t,
                         const double* __restrict__ Lr, const double* __restrict__ Ur,
                         double mu, double sd, bool active, double* total) {
     constexpr bool COUNT = (METRIC == EDR || METRIC == LCSS);
@@ -349,26 +358,37 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
         amax = fmax(fabs((t[0] - mu) * r), fabs((t[n - 1] - mu) * r));
     }
     const double kq = fmax(QM, fabs(arg));
-    const double ceta = 8e-16 * (QM + fabs(arg)) + 1e-300;
     bool dead = !active;
 
     // pass 1: candidate samples against the query envelope
     double s1 = 0.0;
-    int sure1 = 0, uns1 = 0;
+    // counting metrics: positions with R > eta (cgt) and R >= -eta (cge)
+    constexpr double kEtaC = 1.2e-15;
+    const double cetaC = kEtaC * (QM + arg) + 1e-300;
+    const double eta1 = kEtaC * amax + cetaC;  // with a sampled amax
+    int cgt1 = 0, cge1 = 0;
     auto term1 = [&](int k, auto track) {
         const double av = (t[k] - mu) * r;
         if constexpr (decltype(track)::value) amax = fmax(amax, fabs(av));
-        const double e = fmax(fmax(LQ[k] - av, av - UQ[k]), 0.0);
-        const double xg = av - arg;
-        fast_term<METRIC>(e, xg * xg, arg, fma(fabs(av), 8e-16, ceta), s1, sure1, uns1);
+        if constexpr (COUNT) {
+            const double R = fmax(LQE[k] - av, av - UQE[k]);
+            const double et = decltype(track)::value ? fma(fabs(av), kEtaC, cetaC) : eta1;
+            cgt1 += (R > et) ? 1 : 0;
+            cge1 += (R >= -et) ? 1 : 0;
+        } else {
+            const double e = fmax(fmax(LQ[k] - av, av - UQ[k]), 0.0);
+            const double xg = av - arg;
+            fast_term<METRIC>(e, xg * xg, arg, s1);
+        }
     };
     auto check1 = [&](int ie) {
         if (dead) return;
-        if (METRIC == LCSS) {
-            const double tmax = (double)(sure1 + uns1 + (i1 - ie));
+        if (METRIC == LCSS) {  // every position not certainly out can still match
+            (void)ie;
+            const double tmax = (double)(npos - cgt1);
             dead = smax(0.0, 1.0 - tmax / (double)n) > bsf;
         } else if (METRIC == EDR) {
-            dead = B + (double)sure1 > bsf;
+            dead = B + (double)cgt1 > bsf;
         } else {
             const double lo = s1 - glb_sum_err<METRIC>(s1, n, 4e-16 * amax + 1e-300, amax + kq);
             dead = B + lo > thr;
@@ -398,22 +418,28 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
     // subtraction, e <= amax + QM).
     const double mur = fabs(mu) * r;
     const double delta2 = 7e-16 * (amax + QM) + 1.2e-16 * mur + 1e-300;
-    const double eta2 = 8e-16 * (QM + amax + fabs(arg)) + 1.3e-16 * mur + 1e-300;
     double s2 = 0.0;
-    int sure2 = 0, uns2 = 0;
-    int done2 = 0;
+    const double epsr = arg * sd;
+    const double eta2r = sd * (kEtaC * (amax + QM + arg) + 1.3e-16 * mur) + 1e-300;
+    int cgt2 = 0, cge2 = 0;
     auto term2 = [&](int k, double lo, double hi) {
         const double qr = fma(Q[k], sd, mu);
-        const double e = fmax(fmax(lo - qr, qr - hi), 0.0) * r;
-        fast_term<METRIC>(e, (METRIC == ERP) ? XG[k] : 0.0, arg, eta2, s2, sure2, uns2);
+        if constexpr (COUNT) {  // R in raw units: (max(lo - qr, qr - hi) - eps sd)
+            const double R = fmax(lo - qr, qr - hi) - epsr;
+            cgt2 += (R > eta2r) ? 1 : 0;
+            cge2 += (R >= -eta2r) ? 1 : 0;
+        } else {
+            const double e = fmax(fmax(lo - qr, qr - hi), 0.0) * r;
+            fast_term<METRIC>(e, (METRIC == ERP) ? XG[k] : 0.0, arg, s2);
+        }
     };
     auto check2 = [&]() {
         if (dead) return;
         if (METRIC == LCSS) {
-            const double qmax = (double)(sure2 + uns2 + (npos - done2));
-            dead = smax(0.0, 1.0 - mn((double)(sure1 + uns1), qmax) / (double)n) > bsf;
+            const double qmax = (double)(npos - cgt2);
+            dead = smax(0.0, 1.0 - mn((double)(npos - cgt1), qmax) / (double)n) > bsf;
         } else if (METRIC == EDR) {
-            dead = B + (double)sure2 > bsf;
+            dead = B + (double)cgt2 > bsf;
         } else {
             const double lo = s2 - glb_sum_err<METRIC>(s2, n, delta2, amax + kq);
             dead = B + lo > thr;
@@ -433,7 +459,6 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
             }
             term2(k, plo, phi);
         }
-        done2 += max(h1 - i0, 0);
     }
     // interior: w <= i < n - w, the target-wide envelope
     const int m0 = max(i0, w), m1 = min(i1, n - w);
@@ -441,11 +466,10 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
     for (; k0 + kGlbChunk <= m1; k0 += kGlbChunk) {
 #pragma unroll
         for (int u = 0; u < kGlbChunk; u++) term2(k0 + u, Lr[k0 + u], Ur[k0 + u]);
-        done2 += kGlbChunk;
         check2();
         if (__all(dead)) return -1;
     }
-    for (; k0 < m1; k0++, done2++) term2(k0, Lr[k0], Ur[k0]);
+    for (; k0 < m1; k0++) term2(k0, Lr[k0], Ur[k0]);
     // tail: positions i >= max(w, n - w) see the suffix t[i - w .. n - 1];
     // walked from the right end so the suffix only grows
     const int tb = max(max(i0, w), n - w);
@@ -461,17 +485,21 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
             }
             term2(k, slo, shi);
         }
-        done2 += max(i1 - tb, 0);
     }
     check2();
     if (dead) return -1;
 
     double Tlo, Thi, Qlo, Qhi;
-    if (COUNT) {
-        Tlo = (double)sure1;
-        Thi = (double)(sure1 + uns1);
-        Qlo = (double)sure2;
-        Qhi = (double)(sure2 + uns2);
+    if (METRIC == EDR) {  // mismatch counts
+        Tlo = (double)cgt1;
+        Thi = (double)cge1;
+        Qlo = (double)cgt2;
+        Qhi = (double)cge2;
+    } else if (METRIC == LCSS) {  // match counts
+        Tlo = (double)(npos - cge1);
+        Thi = (double)(npos - cgt1);
+        Qlo = (double)(npos - cge2);
+        Qhi = (double)(npos - cgt2);
     } else {
         const double delta = 4e-16 * amax + 1e-300, K = amax + kq;
         const double e1 = glb_sum_err<METRIC>(s1, n, delta, K);
@@ -501,6 +529,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                                              const double* __restrict__ LQ,
                                              const double* __restrict__ UQ,
                                              const double* __restrict__ XG,
+                                             const double* __restrict__ LQE,
+                                             const double* __restrict__ UQE,
                                              const int64_t* __restrict__ in,
                                              const unsigned long long* __restrict__ in_count,
                                              int64_t* __restrict__ out,
@@ -583,11 +613,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         }
         if (fast && staged) {
             const int64_t off = (active ? g : gfirst) - gfirst;
-            verdict = glb_fast<METRIC>(a, Q, LQ, UQ, XG, Ts + off, Ls + off, Us + off, mu, sd,
-                                       active, &total);
+            verdict = glb_fast<METRIC>(a, Q, LQ, UQ, XG, LQE, UQE, Ts + off, Ls + off, Us + off,
+                                       mu, sd, active, &total);
         } else if (fast) {
-            verdict = glb_fast<METRIC>(a, Q, LQ, UQ, XG, a.T + g, a.Lraw + g, a.Uraw + g, mu, sd,
-                                       active, &total);
+            verdict = glb_fast<METRIC>(a, Q, LQ, UQ, XG, LQE, UQE, a.T + g, a.Lraw + g,
+                                       a.Uraw + g, mu, sd, active, &total);
         }
         bool keep = verdict > 0;
         if (active && verdict == 0) {
@@ -620,7 +650,7 @@ hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
     const int stage_len = (a.p.m + 512 <= kGlbStage) ? kGlbStage : 0;
     const size_t lds = sizeof(double) * 3 * (size_t)stage_len;
 #define FSM_GLB(MM) \
-    hipLaunchKernelGGL(k_glb<MM>, g, b, lds, st, a, q.qn, q.LQ, q.UQ, q.XG, in, in_count, out, out_count, out_total, vals, queue, stage_len)
+    hipLaunchKernelGGL(k_glb<MM>, g, b, lds, st, a, q.qn, q.LQ, q.UQ, q.XG, q.LQE, q.UQE, in, in_count, out, out_count, out_total, vals, queue, stage_len)
     switch (a.p.metric) {
         case DTW: FSM_GLB(DTW); break;
         case ERP: FSM_GLB(ERP); break;
