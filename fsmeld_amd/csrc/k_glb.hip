@@ -291,6 +291,13 @@ __device__ inline void fast_term(double e, double xg2, double arg, double& s) {
     }
 }
 
+// ERP's term from the distance itself: min(e^2, xg^2) = min(e, |xg|)^2 for
+// e, |xg| >= 0, squared and added with one rounding.
+__device__ inline void erp_term(double e, double xg, double& s) {
+    const double d = fmin(e, fabs(xg));
+    s = fma(d, d, s);
+}
+
 // Fast decision for one candidate: 1 keep, -1 prune, 0 undecided (take the
 // exact path).  Every lane of the wave calls it; loop trip counts are
 // uniform and the loops end early only when every lane has been pruned.
@@ -375,10 +382,11 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
             const double et = decltype(track)::value ? fma(fabs(av), kEtaC, cetaC) : eta1;
             cgt1 += (R > et) ? 1 : 0;
             cge1 += (R >= -et) ? 1 : 0;
+        } else if constexpr (METRIC == ERP) {
+            erp_term(fmax(fmax(LQ[k] - av, av - UQ[k]), 0.0), av - arg, s1);
         } else {
             const double e = fmax(fmax(LQ[k] - av, av - UQ[k]), 0.0);
-            const double xg = av - arg;
-            fast_term<METRIC>(e, xg * xg, arg, s1);
+            fast_term<METRIC>(e, 0.0, arg, s1);
         }
     };
     auto check1 = [&](int ie) {
