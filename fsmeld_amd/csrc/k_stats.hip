@@ -15,27 +15,34 @@ namespace fsm {
 // One lane per macro-segment runs the running-sum recurrence of
 // src/mvmean.cpp:9-46 in the reference's order (its order fixes the bits);
 // only the sums and the non-finite count are loop-carried, so this pass
-// stores them (sum1 in MU, sum2 in SG, count > 0 in INV) and k_window_finish
-// turns every window's sums into mean / std / invalid flag in parallel
+// stores them (sum1 in MU, sum2 in SG; sum1 is replaced by the marker kBadSum
+// while the window holds a non-finite sample) and k_window_finish turns every
+// window's sums into mean / std / invalid flag in parallel
 // (src/mvmean.cpp:30-44, the invalid test of src/fsmeld.cpp:110-129), with
 // the same expressions.  Outputs are segment-major [S][kml].
-constexpr int kStatsU = 16;
+//
+// Long segments leave few lanes (2^24 samples at m = 2048 are 1170
+// segments), so each lane hides its own load latency: the samples of the
+// next kStatsU steps are loaded while the current kStatsU steps run (two
+// register buffers), and the invalid flag rides in sum1's slot, two stores a
+// step instead of three.
+constexpr int kStatsU = 32;
+// a quiet NaN whose payload no arithmetic produces (operations on finite
+// values give the default NaN)
+constexpr unsigned long long kBadSum = 0x7ff8dead0000beefull;
 
 __global__ void __launch_bounds__(64) k_window_sums(const double* __restrict__ T, Geometry G,
                                                     double* __restrict__ MU,
-                                                    double* __restrict__ SG,
-                                                    uint8_t* __restrict__ INV) {
+                                                    double* __restrict__ SG) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= G.S) return;
     const double* a = (s < G.forsize) ? T + s * G.kml : T + (G.N - G.m_k);
     double* s1o = MU + s * G.kml;
     double* s2o = SG + s * G.kml;
-    uint8_t* bo = INV + s * G.kml;
     const int l = G.m;
+    const double badv = __longlong_as_double((long long)kBadSum);
     double sum1 = 0.0, sum2 = 0.0;
     int bad = 0;
-    // loads are issued kStatsU at a time ahead of the chain so that each lane
-    // keeps many in flight instead of waiting on one per sample
     auto first = [&](double v) {
         if (__builtin_isfinite(v)) {
             sum1 += v;
@@ -53,9 +60,8 @@ __global__ void __launch_bounds__(64) k_window_sums(const double* __restrict__ T
         for (int u = 0; u < kStatsU; u++) first(v[u]);
     }
     for (; i0 < l; i0++) first(a[i0]);
-    s1o[0] = sum1;
+    s1o[0] = bad > 0 ? badv : sum1;
     s2o[0] = sum2;
-    bo[0] = bad > 0;
     const int64_t kml = G.kml;
     auto step = [&](int64_t i, double vo, double vi) {
         const bool of = __builtin_isfinite(vo);
@@ -65,20 +71,36 @@ __global__ void __launch_bounds__(64) k_window_sums(const double* __restrict__ T
         bad += (int)!nf - (int)!of;
         sum1 = sum1 - out + in;
         sum2 = sum2 - out * out + in * in;
-        s1o[i] = sum1;
+        s1o[i] = bad > 0 ? badv : sum1;
         s2o[i] = sum2;
-        bo[i] = bad > 0;
     };
-    int64_t i = 1;
-    for (; i + kStatsU <= kml; i += kStatsU) {
-        double vo[kStatsU], vi[kStatsU];
+    // window i drops sample i - 1 and takes sample i + l - 1
+    double ao[kStatsU], ai[kStatsU], bo[kStatsU], bi[kStatsU];
+    auto load = [&](double* vo, double* vi, int64_t base) {
 #pragma unroll
         for (int u = 0; u < kStatsU; u++) {
-            vo[u] = a[i + u - 1];
-            vi[u] = a[i + u + l - 1];
+            vo[u] = a[base + u - 1];
+            vi[u] = a[base + u + l - 1];
         }
+    };
+    auto run = [&](const double* vo, const double* vi, int64_t base) {
 #pragma unroll
-        for (int u = 0; u < kStatsU; u++) step(i + u, vo[u], vi[u]);
+        for (int u = 0; u < kStatsU; u++) step(base + u, vo[u], vi[u]);
+    };
+    int64_t i = 1;
+    if (i + kStatsU <= kml) load(ao, ai, i);
+    // a holds steps [i, i + U) here; b's loads fly while a's steps run, and
+    // a's next loads while b's run
+    while (i + 2 * kStatsU <= kml) {
+        load(bo, bi, i + kStatsU);
+        run(ao, ai, i);
+        if (i + 3 * kStatsU <= kml) load(ao, ai, i + 2 * kStatsU);
+        run(bo, bi, i + kStatsU);
+        i += 2 * kStatsU;
+    }
+    if (i + kStatsU <= kml) {
+        run(ao, ai, i);
+        i += kStatsU;
     }
     for (; i < kml; i++) step(i, a[i - 1], a[i + l - 1]);
 }
@@ -92,7 +114,7 @@ __global__ void __launch_bounds__(256) k_window_finish(int64_t n, int m, double*
     const double nan = __builtin_nan("");
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const bool bad = INV[i] != 0;
+        const bool bad = (unsigned long long)__double_as_longlong(MU[i]) == kBadSum;
         const double mi = bad ? nan : MU[i] / dl;
         const double sg = bad ? nan : sqrt(SG[i] / dl - mi * mi);
         MU[i] = mi;
@@ -105,7 +127,7 @@ hipError_t launch_window_stats(hipStream_t st, const double* T, const Geometry& 
                                double* SG, uint8_t* INV) {
     const int bs = 64;
     const int64_t blocks = (G.S + bs - 1) / bs;
-    hipLaunchKernelGGL(k_window_sums, dim3((unsigned)blocks), dim3(bs), 0, st, T, G, MU, SG, INV);
+    hipLaunchKernelGGL(k_window_sums, dim3((unsigned)blocks), dim3(bs), 0, st, T, G, MU, SG);
     const int64_t n = G.S * G.kml;
     const int64_t fb = std::min<int64_t>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(k_window_finish, dim3((unsigned)fb), dim3(256), 0, st, n, G.m, MU, SG, INV);
