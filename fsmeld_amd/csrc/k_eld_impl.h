@@ -308,6 +308,11 @@ __device__ __forceinline__ void ring_shift(double* Y, double yn) {
     Y[W + 1] = yn;
 }
 
+// Two iterations per pass of the register kernel's loop (the ring moves by
+// two, the loop's control runs once) where the band update is short enough
+// for the per-iteration overhead to matter.
+template <int METRIC> constexpr bool kEldPair = (METRIC == EDR || METRIC == LCSS);
+
 // Per-sample lower bound on the cost of consuming candidate sample y whose
 // band sees the query envelope [lo, hi] (DTW, ERP, MSM); for EDR 1 if y is
 // more than eps from the envelope, for LCSS 1 if it is within eps of it.
@@ -459,7 +464,7 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
     // the query side X[k] = xs[b - 1 + k] is read from LDS (lanes run at
     // their own iteration b); the candidate ring Y stays in registers
     VT A[W + 1], B[W + 1];
-    double Y[W + 2];
+    double Y[W + 4];  // kEldPair: Y[W + 2], Y[W + 3] take the two new samples
 #pragma unroll
     for (int k = 0; k <= W; k++) A[k] = B[k] = S;
 #pragma unroll
@@ -519,18 +524,75 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
         }
         if (!__any(busy)) break;
 
-        ad_step<METRIC, W>(arg, A, B, xs + (b - 1), Y);
-        if constexpr (METRIC == EDR || METRIC == LCSS) {
-            // The slots are materialised here: left to itself the compiler
-            // sinks the integer updates past the rest of the iteration and
-            // keeps all 2W + 1 comparison masks live in scalar registers,
-            // which then spill to vector lanes.
+        // The slots are materialised after each band update: left to itself
+        // the compiler sinks the integer updates (EDR / LCSS) past the rest
+        // of the iteration and keeps all 2W + 1 comparison masks live in
+        // scalar registers, which then spill to vector lanes.
+        auto pin = [&]() {
+            if constexpr (METRIC == EDR || METRIC == LCSS) {
 #pragma unroll
-            for (int k = 0; k <= W; k++) {
-                asm volatile("" : "+v"(A[k]));
-                asm volatile("" : "+v"(B[k]));
+                for (int k = 0; k <= W; k++) {
+                    asm volatile("" : "+v"(A[k]));
+                    asm volatile("" : "+v"(B[k]));
+                }
             }
+        };
+        // next candidate sample of iteration bi (the ring's new entry)
+        auto sample = [&](int bi) {
+            const int jn = bi + W + 1;
+            const int jj = min(max(jn, 0), m - 1);
+            double yv = tp[jj];
+            if (raw) yv = (yv - mu) / sd;
+            return (jn == jj) ? yv : kPad;
+        };
+        if constexpr (kEldPair<METRIC>) {
+            // iterations b and b + 1; a lane whose candidate ends at b keeps
+            // its value (r1) and runs the second half on its stale state
+            // (every index stays inside the padded query, the sample index is
+            // clamped), then idles like any finished lane
+            ad_step<METRIC, W>(arg, A, B, xs + (b - 1), Y);
+            pin();
+            const double y1 = sample(b);
+            if (check) {
+                const int K = b + KOFF;
+                if (K >= 0 && K < m) p += consume_lb<METRIC>(Y[KOFF + 1], lqs[K], uqs[K], arg);
+            }
+            Y[W + 2] = y1;
+            const bool fin1 = busy && b == bend;
+            const VT r1 = (W & 1) ? B[(W - 1) / 2] : A[W / 2];
+            ad_step<METRIC, W>(arg, A, B, xs + b, Y + 1);
+            pin();
+            const double y2 = sample(b + 1);
+            if (check) {
+                const int K = b + 1 + KOFF;
+                if (K >= 0 && K < m) p += consume_lb<METRIC>(Y[KOFF + 2], lqs[K], uqs[K], arg);
+            }
+            Y[W + 3] = y2;
+#pragma unroll
+            for (int k = 0; k < W + 2; k++) Y[k] = Y[k + 2];
+            const bool fin = busy && (fin1 || b + 1 == bend);
+            live_iters += busy ? (fin1 ? 1u : 2u) : 0u;
+            if (busy) b += 2;
+            if (__any(fin)) {
+                const VT rv = fin1 ? r1 : ((W & 1) ? B[(W - 1) / 2] : A[W / 2]);
+                const double d =
+                    eld_result<METRIC>(band_value<METRIC>(rv, 2 * bend + W + (W & 1)), m);
+                if (keep_all) {
+                    if (fin) out_dist[ci] = d;
+                } else {
+                    // src/fsmeld.cpp:846-847
+                    wave_append(fin && __builtin_isfinite(d) && d <= cut, g, d, out_idx,
+                                out_dist, out_count);
+                }
+                busy = busy && !fin;
+            }
+            // every fourth iteration, as below
+            if (check && ((++it) & 1) == 0)
+                busy = busy && ad_alive<METRIC, W>(A, B, b - 1, M, m, thr, total, p);
+            continue;
         }
+        ad_step<METRIC, W>(arg, A, B, xs + (b - 1), Y);
+        pin();
         const int jn = b + W + 1;
         const int jj = min(max(jn, 0), m - 1);
         double yv = tp[jj];
