@@ -20,6 +20,8 @@
 #include <cstring>
 #include <fstream>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <atomic>
 #include <memory>
 #include <mutex>
@@ -208,18 +210,75 @@ double ms_between(hipEvent_t a, hipEvent_t b) {
 
 }  // namespace
 
+// Window statistics for one m and the target envelope for one w depend on
+// the target alone, so all handles of a target share them.  A key computes
+// into the handle's own buffers (reused from query to query, as when nothing
+// repeats) unless it was requested within the last kCacheReuse requests of
+// its kind: then it gets a shared entry, made on the stream of the handle that
+// needs it and published with an event the other handles' streams wait on.
+// (A new entry's allocation costs more than the statistics it saves once, so
+// keys that come back only after many others stay private.)  Beyond the byte cap (half of device memory)
+// a new entry takes over the buffers of the least recently used entry no
+// handle is using, so a cache that is full allocates and frees nothing.
+struct CacheEntry {
+    DevBuf a, b, c;  // statistics: mean, std, invalid flag; envelope: L, U
+    hipEvent_t ready = nullptr;
+    uint64_t bytes = 0, tick = 0;
+    ~CacheEntry() {
+        a.release();
+        b.release();
+        c.release();
+        if (ready) (void)hipEventDestroy(ready);
+    }
+};
+struct TargetCache {
+    std::mutex mu;
+    std::map<int, std::shared_ptr<CacheEntry>> stats, env;  // by m, by w
+    std::map<int, uint64_t> seen_stats, seen_env;            // last request, by key
+    uint64_t req_stats = 0, req_env = 0;                     // requests so far
+    uint64_t tick = 0, bytes = 0, cap = 0;
+    // the least recently used entry of mp no handle is using, taken out of
+    // the cache (null if every entry is in use)
+    std::shared_ptr<CacheEntry> take_idle(std::map<int, std::shared_ptr<CacheEntry>>& mp) {
+        auto best = mp.end();
+        for (auto it = mp.begin(); it != mp.end(); ++it)
+            if (it->second.use_count() == 1 && (best == mp.end() || it->second->tick < best->second->tick))
+                best = it;
+        if (best == mp.end()) return nullptr;
+        std::shared_ptr<CacheEntry> e = std::move(best->second);
+        mp.erase(best);
+        bytes -= e->bytes;
+        return e;
+    }
+    // free every entry no handle is using
+    void drop_idle() {
+        for (auto* mp : {&stats, &env})
+            for (auto it = mp->begin(); it != mp->end();) {
+                if (it->second.use_count() == 1) {
+                    bytes -= it->second->bytes;
+                    it = mp->erase(it);
+                } else {
+                    ++it;
+                }
+            }
+    }
+};
+
 // A handle owns a stream, events and every working buffer; the device copy of
-// the target's samples is shared by all handles made from one another with
-// fsm_target_share (and freed with the last of them).
+// the target's samples and the cache of statistics and envelopes are shared by
+// all handles made from one another with fsm_target_share (and freed with the
+// last of them).
 struct fsm_target {
     int device = 0;
     hipStream_t stream = nullptr;
     int64_t N = 0;
     std::shared_ptr<DevBuf> T;
+    std::shared_ptr<TargetCache> cache;
     int stats_m = -1;
-    DevBuf MU, SG, INV;
+    std::shared_ptr<CacheEntry> st;  // MU, SG, INV of stats_m
     int env_w = -1;
-    DevBuf Lr, Ur;
+    std::shared_ptr<CacheEntry> env;  // Lr, Ur of env_w
+    CacheEntry own_st, own_env;       // this handle's buffers for first requests
     DevBuf lb, list_a, list_b, out_idx, out_dist, slab, rep_lb, rep_idx, tu, tl, qtab,
         scratch, counters, aux;
     hipEvent_t ev[8] = {};
@@ -266,26 +325,92 @@ int upload_query(fsm_target* t, const HostQuery& H, int m, QueryTables* Q) {
     return FSM_OK;
 }
 
+// Key k of map mp for this handle: a shared entry (this stream waits for it),
+// or one made by fill() on this stream, shared when k was requested before
+// and otherwise in the handle's own buffers (own, wrapped without ownership).
+// *made tells whether fill ran.
+constexpr uint64_t kCacheReuse = 64;
+template <class F>
+int cached(fsm_target* t, std::map<int, std::shared_ptr<CacheEntry>> TargetCache::*mp,
+           std::map<int, uint64_t> TargetCache::*seen, uint64_t TargetCache::*req, int k,
+           uint64_t bytes, CacheEntry* own, F fill, std::shared_ptr<CacheEntry>* out,
+           bool* made) {
+    TargetCache& C = *t->cache;
+    std::shared_ptr<CacheEntry> e;
+    *made = false;
+    {
+        std::lock_guard<std::mutex> lk(C.mu);
+        const uint64_t now = ++(C.*req);
+        auto sn = (C.*seen).find(k);
+        const bool recent = sn != (C.*seen).end() && now - sn->second <= kCacheReuse;
+        (C.*seen)[k] = now;
+        auto it = (C.*mp).find(k);
+        if (it != (C.*mp).end()) {
+            e = it->second;
+            e->tick = ++C.tick;
+        } else if (recent) {
+            // made, launched and its event recorded under the lock: whoever
+            // finds it afterwards waits on a recorded event
+            if (C.bytes + bytes > C.cap) e = C.take_idle(C.*mp);
+            if (!e) e = std::make_shared<CacheEntry>();
+            if (!e->ready) HIP_TRY(hipEventCreateWithFlags(&e->ready, hipEventDisableTiming));
+            const int rc = fill(*e);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(e->ready, t->stream));
+            e->bytes = bytes;
+            e->tick = ++C.tick;
+            (C.*mp)[k] = e;
+            C.bytes += bytes;
+            *made = true;
+        }
+    }
+    if (!e) {  // first request: the handle's own buffers
+        const int rc = fill(*own);
+        if (rc) return rc;
+        *out = std::shared_ptr<CacheEntry>(std::shared_ptr<CacheEntry>(), own);
+        *made = true;
+        return FSM_OK;
+    }
+    if (!*made) HIP_TRY(hipStreamWaitEvent(t->stream, e->ready, 0));
+    *out = std::move(e);
+    return FSM_OK;
+}
+
 int ensure_stats(fsm_target* t, const Geometry& G, bool* computed) {
     *computed = false;
-    if (t->stats_m == G.m) return FSM_OK;
+    if (t->st && t->stats_m == G.m) return FSM_OK;
     const size_t n = (size_t)G.S * (size_t)G.kml;
-    HIP_TRY(t->MU.ensure(sizeof(double) * n));
-    HIP_TRY(t->SG.ensure(sizeof(double) * n));
-    HIP_TRY(t->INV.ensure(n));
-    HIP_TRY(launch_window_stats(t->stream, t->T->as<double>(), G, t->MU.as<double>(),
-                                t->SG.as<double>(), t->INV.as<uint8_t>()));
+    auto fill = [&](CacheEntry& e) {
+        HIP_TRY(e.a.ensure(sizeof(double) * n));
+        HIP_TRY(e.b.ensure(sizeof(double) * n));
+        HIP_TRY(e.c.ensure(n));
+        HIP_TRY(launch_window_stats(t->stream, t->T->as<double>(), G, e.a.as<double>(),
+                                    e.b.as<double>(), e.c.as<uint8_t>()));
+        return FSM_OK;
+    };
+    t->st.reset();  // before own buffers may be refilled
+    const int rc = cached(t, &TargetCache::stats, &TargetCache::seen_stats,
+                          &TargetCache::req_stats, G.m, 17 * (uint64_t)n, &t->own_st, fill,
+                          &t->st, computed);
+    if (rc) return rc;
     t->stats_m = G.m;
-    *computed = true;
     return FSM_OK;
 }
 
 int ensure_env(fsm_target* t, int w) {
-    if (t->env_w == w) return FSM_OK;
-    HIP_TRY(t->Lr.ensure(sizeof(double) * (size_t)t->N));
-    HIP_TRY(t->Ur.ensure(sizeof(double) * (size_t)t->N));
-    HIP_TRY(launch_envelope(t->stream, t->T->as<double>(), t->N, w, t->Lr.as<double>(),
-                            t->Ur.as<double>()));
+    if (t->env && t->env_w == w) return FSM_OK;
+    auto fill = [&](CacheEntry& e) {
+        HIP_TRY(e.a.ensure(sizeof(double) * (size_t)t->N));
+        HIP_TRY(e.b.ensure(sizeof(double) * (size_t)t->N));
+        HIP_TRY(launch_envelope(t->stream, t->T->as<double>(), t->N, w, e.a.as<double>(),
+                                e.b.as<double>()));
+        return FSM_OK;
+    };
+    bool made = false;
+    t->env.reset();
+    const int rc = cached(t, &TargetCache::env, &TargetCache::seen_env, &TargetCache::req_env,
+                          w, 16 * (uint64_t)t->N, &t->own_env, fill, &t->env, &made);
+    if (rc) return rc;
     t->env_w = w;
     return FSM_OK;
 }
@@ -302,11 +427,11 @@ int run_lbip(fsm_target* t, const Geometry& G, const QueryTables& Q, const Query
     HIP_TRY(t->slab.ensure(per * (size_t)blocks));
     LbipArgs a{};
     a.T = t->T->as<double>();
-    a.MU = t->MU.as<double>();
-    a.SG = t->SG.as<double>();
-    a.INV = t->INV.as<uint8_t>();
-    a.Lraw = t->Lr.as<double>();
-    a.Uraw = t->Ur.as<double>();
+    a.MU = t->st->a.as<double>();
+    a.SG = t->st->b.as<double>();
+    a.INV = t->st->c.as<uint8_t>();
+    a.Lraw = t->env ? t->env->a.as<double>() : nullptr;
+    a.Uraw = t->env ? t->env->b.as<double>() : nullptr;
     a.q = Q;
     a.p = P;
     a.G = G;
@@ -326,10 +451,10 @@ VerifyArgs verify_args(fsm_target* t, const Geometry& G, const QueryTables& Q,
                        const QueryParams& P) {
     VerifyArgs v{};
     v.T = t->T->as<double>();
-    v.MU = t->MU.as<double>();
-    v.SG = t->SG.as<double>();
-    v.Lraw = t->Lr.as<double>();
-    v.Uraw = t->Ur.as<double>();
+    v.MU = t->st->a.as<double>();
+    v.SG = t->st->b.as<double>();
+    v.Lraw = t->env ? t->env->a.as<double>() : nullptr;
+    v.Uraw = t->env ? t->env->b.as<double>() : nullptr;
     v.q = Q;
     v.p = P;
     v.G = G;
@@ -440,7 +565,7 @@ int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
     HIP_TRY(t->list_b.ensure(sizeof(int64_t) * (size_t)G.ts_n));
     HIP_TRY(t->out_idx.ensure(sizeof(int64_t) * (size_t)G.ts_n));
     HIP_TRY(t->out_dist.ensure(sizeof(double) * (size_t)G.ts_n));
-    HIP_TRY(launch_compact(t->stream, G, P, t->INV.as<uint8_t>(),
+    HIP_TRY(launch_compact(t->stream, G, P, t->st->c.as<uint8_t>(),
                            use_lb ? t->lb.as<double>() : nullptr, use_lb,
                            t->list_a.as<int64_t>(), C));
     HIP_TRY(hipEventRecord(t->ev[3], t->stream));
@@ -498,10 +623,12 @@ const char* fsm_last_error(void) { return g_last_error.c_str(); }
 namespace {
 
 // A handle with its own stream, events and counters over device samples T.
-int new_handle(int64_t n, std::shared_ptr<DevBuf> T, fsm_target** out) {
+int new_handle(int64_t n, std::shared_ptr<DevBuf> T, std::shared_ptr<TargetCache> cache,
+               fsm_target** out) {
     auto* t = new fsm_target();
     t->N = n;
     t->T = std::move(T);
+    t->cache = std::move(cache);
     auto cleanup = [&](int rc) {
         fsm_target_close(t);
         return rc;
@@ -525,10 +652,14 @@ int fsm_target_open(const double* host, int64_t n, fsm_target** out) {
     *out = nullptr;
     if (fsm_device_count() < 1) return fail(FSM_ERR_NODEV, "no HIP device");
     fsm_target* t = nullptr;
+    auto cache = std::make_shared<TargetCache>();
+    size_t mem_free = 0, mem_total = 0;
+    if (hipMemGetInfo(&mem_free, &mem_total) != hipSuccess) mem_total = 0;
+    cache->cap = (uint64_t)mem_total / 2;
     int rc = new_handle(n, std::shared_ptr<DevBuf>(new DevBuf(), [](DevBuf* b) {
                             b->release();
                             delete b;
-                        }), &t);
+                        }), std::move(cache), &t);
     if (rc) return rc;
     if (t->T->ensure(sizeof(double) * (size_t)n) != hipSuccess) {
         fsm_target_close(t);
@@ -546,7 +677,7 @@ int fsm_target_open(const double* host, int64_t n, fsm_target** out) {
 int fsm_target_share(fsm_target* base, fsm_target** out) {
     if (!base || !out) return fail(FSM_ERR_ARG, "null argument");
     *out = nullptr;
-    return new_handle(base->N, base->T, out);
+    return new_handle(base->N, base->T, base->cache, out);
 }
 
 int fsm_target_open_file(const char* path, fsm_target** out) {
@@ -570,7 +701,10 @@ void fsm_target_close(fsm_target* t) {
     for (fsm_target* h : t->pool) fsm_target_close(h);
     t->pool.clear();
     if (t->stream) (void)hipStreamSynchronize(t->stream);
-    DevBuf* bufs[] = {&t->MU, &t->SG, &t->INV, &t->Lr, &t->Ur, &t->lb,
+    t->st.reset();
+    t->env.reset();
+    t->cache.reset();
+    DevBuf* bufs[] = {&t->lb,
                       &t->list_a, &t->list_b, &t->out_idx, &t->out_dist, &t->slab,
                       &t->rep_lb, &t->rep_idx, &t->tu, &t->tl, &t->qtab, &t->scratch,
                       &t->counters, &t->aux};
@@ -637,7 +771,7 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
             unsigned long long* Cd = t->counters.as<unsigned long long>();
             HIP_TRY(hipMemsetAsync(Cd, 0, sizeof(unsigned long long) * 8, t->stream));
             HIP_TRY(t->list_a.ensure(sizeof(int64_t) * (size_t)G.ts_n));
-            HIP_TRY(launch_compact(t->stream, G, P, t->INV.as<uint8_t>(), nullptr, false,
+            HIP_TRY(launch_compact(t->stream, G, P, t->st->c.as<uint8_t>(), nullptr, false,
                                    t->list_a.as<int64_t>(), Cd));
             std::vector<int64_t> reps((size_t)G.S);
             HIP_TRY(hipMemcpyAsync(reps.data(), t->rep_idx.p, sizeof(int64_t) * G.S,
@@ -775,9 +909,9 @@ int fsm_stage_window_stats(fsm_target* t, int m, int64_t xb, int64_t xe, double*
     std::vector<double> hm(n), hs(n);
     std::vector<uint8_t> hi(n);
     HIP_TRY(hipStreamSynchronize(t->stream));
-    HIP_TRY(hipMemcpy(hm.data(), t->MU.p, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hs.data(), t->SG.p, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hi.data(), t->INV.p, n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hm.data(), t->st->a.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hs.data(), t->st->b.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hi.data(), t->st->c.p, n, hipMemcpyDeviceToHost));
     const bool excl = xb >= 0 && xe > xb;
     for (int64_t g = 0; g < G.ts_n; g++) {
         const int64_t s = stat_index(G, g);
@@ -997,10 +1131,21 @@ uint64_t fsm_query_bytes(int64_t n, int metric, int m, int w) {
 void fsm_target_trim(fsm_target* t) {
     if (!t) return;
     (void)hipStreamSynchronize(t->stream);
-    DevBuf* bufs[] = {&t->MU, &t->SG, &t->INV, &t->Lr, &t->Ur, &t->lb, &t->list_a, &t->list_b,
-                      &t->out_idx, &t->out_dist, &t->slab, &t->rep_lb, &t->rep_idx, &t->tu,
-                      &t->tl, &t->qtab, &t->scratch, &t->aux};
+    DevBuf* bufs[] = {&t->lb, &t->list_a, &t->list_b, &t->out_idx, &t->out_dist, &t->slab,
+                      &t->rep_lb, &t->rep_idx, &t->tu, &t->tl, &t->qtab, &t->scratch, &t->aux};
     for (DevBuf* b : bufs) b->release();
+    t->st.reset();
+    t->env.reset();
     t->stats_m = -1;
     t->env_w = -1;
+    t->own_st.a.release();
+    t->own_st.b.release();
+    t->own_st.c.release();
+    t->own_env.a.release();
+    t->own_env.b.release();
+    // and the target's cached statistics / envelopes no handle is using
+    if (t->cache) {
+        std::lock_guard<std::mutex> lk(t->cache->mu);
+        t->cache->drop_idle();
+    }
 }

@@ -123,7 +123,11 @@ int fsm_query_batch(fsm_target* t, int64_t n, const fsm_query_desc* descs,
  * samples (working buffers only; the samples themselves are shared). */
 uint64_t fsm_query_bytes(int64_t n, int metric, int m, int w);
 
-/* Frees a handle's working buffers (they are re-created on its next query). */
+/* Frees a handle's working buffers (they are re-created on its next query),
+ * and the target's shared window statistics / envelopes that no handle is
+ * using.  Those are kept per window length m and radius w for all handles of
+ * a target (fsm_target_share) once a length or radius comes back within 64
+ * requests, up to half of device memory, least recently used first out. */
 void fsm_target_trim(fsm_target* t);
 
 /* ---- serving loop (no reference counterpart) ----
