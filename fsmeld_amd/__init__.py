@@ -260,6 +260,11 @@ class Target:
                 out.append(FsmeldError(status[i], "query failed"))
         return out
 
+    def trim(self):
+        """Frees this handle's working buffers and the target's cached window
+        statistics / envelopes that no handle is using (fsm_target_trim)."""
+        _lib.fsm_target_trim(self._h)
+
     def share(self) -> "Target":
         """Another handle to the same device samples (own stream and buffers)."""
         t = Target.__new__(Target)

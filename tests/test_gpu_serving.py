@@ -79,6 +79,35 @@ def test_batch_reports_per_query_errors(gpu, target):
     assert isinstance(out[1], F.FsmeldError) and out[1].code == -1
 
 
+def test_shared_window_stats_across_handles(gpu):
+    """Window statistics and envelopes are kept per target and shared by its
+    handles once a length / radius comes back: three handles alternating
+    lengths and radii, with trims in between, give exactly the results of a
+    target that shares nothing (a new one per query)."""
+    t = F.synth_target(43, 1 << 15, 0.001, 5.0, 3, 150, 2)
+    T = F.Target(t)
+    H = [T, T.share(), T.share()]
+    ms = [40, 57, 40, 40, 57, 90, 40, 90, 57, 40]
+    for rnd in range(2):
+        for i, m in enumerate(ms):
+            metric, arg = METRICS[(i + rnd) % 5]
+            w = max(1, m // 20) + (i % 2)
+            q, _ = F.synth_query(500 + 37 * rnd + i, t, m, 0.1)
+            got = H[i % 3].query("cutoff", q, m, metric, w, arg, k=5)
+            with F.Target(t) as R:
+                ref = R.query("cutoff", q, m, metric, w, arg, k=5)
+            same_result(got, ref)
+            c = ref.stats["threshold"]
+            got = H[(i + 1) % 3].search(q, m, c, metric, w, arg)
+            with F.Target(t) as R:
+                ref = R.search(q, m, c, metric, w, arg)
+            same_result(got, ref)
+        H[1].trim()
+        H[2].trim()
+    for h in H[1:]:
+        h.close()
+
+
 def test_server_mix_equals_single(gpu, target):
     t, T = target
     specs = mixed_specs(t, 24, 11)
