@@ -123,7 +123,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(the engine is single-GPU; see DESIGN.md for how the path shards).",
+              f"(the engine is single-GPU; a multi-GPU search would give each GPU a range of "
+              f"macro-segments).",
               file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
