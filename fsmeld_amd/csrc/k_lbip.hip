@@ -36,9 +36,11 @@ __host__ __device__ inline int npad_of(int n) {
     return p;
 }
 
-// Workgroup size per segment: 256 threads up to 4096 keys, 1024 beyond, so
-// that long segments spread their per-position work over more waves.
-inline int lbip_threads(int npad) { return npad <= 4096 ? 256 : 1024; }
+// Workgroup size per segment: 128 threads up to 1024 keys (short segments
+// are many and dominated by the workgroup's barriers), 256 up to 4096, 1024
+// beyond, so that long segments spread their per-position work over more
+// waves.
+inline int lbip_threads(int npad) { return npad <= 1024 ? 128 : npad <= 4096 ? 256 : 1024; }
 
 // The segment's sort keys live in LDS up to 8192 of them, 16384 with a
 // 1024-thread workgroup (one per CU: 128 KiB of its 160, and 24 KiB for the
@@ -63,8 +65,9 @@ size_t lbip_slab_bytes(const Geometry& G) {
 // LDS (about 38 KiB) lets four workgroups share a CU instead of three.
 constexpr int kThrBucketsMax = 256;
 __host__ __device__ inline bool thr_small(int npad, int nt) { return nt < 1024 && npad >= 4096; }
+// (one bucket per thread at most: the bucket scans run a thread per bucket)
 __host__ __device__ inline int thr_buckets(int npad, int nt) {
-    return thr_small(npad, nt) ? 128 : kThrBucketsMax;
+    return thr_small(npad, nt) ? 128 : (nt < kThrBucketsMax ? nt : kThrBucketsMax);
 }
 __host__ __device__ inline int thr_cap(int npad, int nt) {
     const int c = thr_small(npad, nt) ? npad / 16 : npad / 8;
@@ -560,10 +563,10 @@ __device__ inline int64_t trans_sum(const uint64_t* gw, const uint64_t* tw, cons
 }
 
 // ------------------------------------------------------------------ kernel ---
-// 256-thread workgroups fit four waves per SIMD.
+// 128- and 256-thread workgroups fit four waves per SIMD.
 template <int NT>
 __global__ void __launch_bounds__(NT)
-    __attribute__((amdgpu_waves_per_eu(NT == 256 ? 4 : 1))) k_lbip(LbipArgs a) {
+    __attribute__((amdgpu_waves_per_eu(NT <= 256 ? 4 : 1))) k_lbip(LbipArgs a) {
     extern __shared__ unsigned char lds_raw[];
     __shared__ ArgBest sh_best[NT / 64];
     __shared__ double sh_sum[NT / 64];
@@ -872,7 +875,9 @@ hipError_t launch_lbip(hipStream_t st, const LbipArgs& a) {
     const size_t lds = lbip_lds_bytes(a.G);
     const int npad = npad_of(a.G.m_k - 2 * a.G.m + 2);
     const dim3 g((unsigned)a.blocks);
-    if (lbip_threads(npad) == 256) hipLaunchKernelGGL(k_lbip<256>, g, dim3(256), lds, st, a);
+    const int nt = lbip_threads(npad);
+    if (nt == 128) hipLaunchKernelGGL(k_lbip<128>, g, dim3(128), lds, st, a);
+    else if (nt == 256) hipLaunchKernelGGL(k_lbip<256>, g, dim3(256), lds, st, a);
     else hipLaunchKernelGGL(k_lbip<1024>, g, dim3(1024), lds, st, a);
     return hipGetLastError();
 }
