@@ -36,11 +36,15 @@ __host__ __device__ inline int npad_of(int n) {
     return p;
 }
 
-// Workgroup size per segment: 128 threads up to 1024 keys (short segments
-// are many and dominated by the workgroup's barriers), 256 up to 4096, 1024
+// Workgroup size per segment: 128 threads up to 1024 keys when there are
+// enough segments to fill the device with two-wave workgroups (short segments
+// are then dominated by the workgroup's barriers), 256 up to 4096, 1024
 // beyond, so that long segments spread their per-position work over more
 // waves.
-inline int lbip_threads(int npad) { return npad <= 1024 ? 128 : npad <= 4096 ? 256 : 1024; }
+inline int lbip_threads(int npad, int64_t segments) {
+    if (npad <= 1024 && segments >= 4096) return 128;
+    return npad <= 4096 ? 256 : 1024;
+}
 
 // The segment's sort keys live in LDS up to 8192 of them, 16384 with a
 // 1024-thread workgroup (one per CU: 128 KiB of its 160, and 24 KiB for the
@@ -55,7 +59,7 @@ size_t lbip_slab_bytes(const Geometry& G) {
     const int nthr = G.m_k - 2 * G.m + 2;
     const int npad = npad_of(nthr);
     size_t b = sizeof(double) * (8 * (size_t)G.kml + 2 * (size_t)G.m_k);
-    if (!keys_in_lds(npad, lbip_threads(npad))) b += sizeof(double) * (size_t)npad;
+    if (!keys_in_lds(npad, lbip_threads(npad, G.S))) b += sizeof(double) * (size_t)npad;
     return (b + 255) & ~(size_t)255;
 }
 
@@ -83,8 +87,9 @@ size_t lbip_lds_bytes(const Geometry& G) {
     const int npad = npad_of(nthr);
     const int nw = (G.m_k + 63) / 64;
     const size_t late = sizeof(int64_t) * 2 * (size_t)(G.m + 1) + sizeof(uint64_t) * 4 * (size_t)nw;
-    const size_t k = keys_in_lds(npad, lbip_threads(npad)) ? sizeof(double) * (size_t)npad : 0;
-    return std::max(late, k + thr_lds_bytes(npad, lbip_threads(npad)));
+    const int nt = lbip_threads(npad, G.S);
+    const size_t k = keys_in_lds(npad, nt) ? sizeof(double) * (size_t)npad : 0;
+    return std::max(late, k + thr_lds_bytes(npad, nt));
 }
 
 // ---------------------------------------------------------------- helpers ---
@@ -875,7 +880,7 @@ hipError_t launch_lbip(hipStream_t st, const LbipArgs& a) {
     const size_t lds = lbip_lds_bytes(a.G);
     const int npad = npad_of(a.G.m_k - 2 * a.G.m + 2);
     const dim3 g((unsigned)a.blocks);
-    const int nt = lbip_threads(npad);
+    const int nt = lbip_threads(npad, a.G.S);
     if (nt == 128) hipLaunchKernelGGL(k_lbip<128>, g, dim3(128), lds, st, a);
     else if (nt == 256) hipLaunchKernelGGL(k_lbip<256>, g, dim3(256), lds, st, a);
     else hipLaunchKernelGGL(k_lbip<1024>, g, dim3(1024), lds, st, a);
