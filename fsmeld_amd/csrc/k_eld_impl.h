@@ -172,6 +172,10 @@ constexpr int kXPad = 40;
 constexpr size_t kEldLdsCap = 64 * 1024;
 constexpr double kPad = 1e300;
 constexpr int kRefillMin = 8;  // idle lanes that trigger a wave's refill
+// queue positions a wave takes per atomic; refills draw on the wave's pool
+// (short-lived candidates would otherwise make one atomic per few lanes on
+// a single counter)
+constexpr unsigned long long kEldChunk = 64;
 
 // Padded boundary.  Query and candidate samples outside [0, m) read as kPad;
 // the two anti-diagonals before the first iteration start as the
@@ -474,20 +478,29 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
     int64_t ci = 0, g = 0;
     int b = 0, it = 0;
     unsigned live_iters = 0;
-    bool busy = false, more = true;
+    bool busy = false, drained = false;
+    // the wave's pool of queue positions [pb, pe) (the same in every lane)
+    unsigned long long pb = 0, pe = 0;
     while (true) {
         const uint64_t idle = __ballot(!busy);
         const int nidle = __builtin_popcountll(idle);
-        if (more && (nidle >= kRefillMin || nidle == 64)) {
-            const int leader = __builtin_ctzll(idle);
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(queue, (unsigned long long)nidle);
-            base = __shfl(base, leader, 64);
-            if (base + (unsigned long long)nidle >= (unsigned long long)n) more = false;
+        if ((pb < pe || !drained) && (nidle >= kRefillMin || nidle == 64)) {
+            const unsigned long long left = pe - pb;
+            const bool take = left < (unsigned long long)nidle && !drained;
+            unsigned long long nb = 0;
+            if (take) {
+                const int leader = __builtin_ctzll(idle);
+                if (lane == leader) nb = atomicAdd(queue, kEldChunk);
+                nb = __shfl(nb, leader, 64);
+                if (nb + kEldChunk >= (unsigned long long)n) drained = true;
+            }
             if (!busy) {
-                const int rank = __builtin_popcountll(idle & ((1ull << lane) - 1));
-                const int64_t c = (int64_t)base + rank;
-                if (c < n) {
+                const unsigned long long rank =
+                    (unsigned long long)__builtin_popcountll(idle & ((1ull << lane) - 1));
+                // the pool's positions first, then the new chunk's
+                const bool have = rank < left || take;
+                const int64_t c = (int64_t)(rank < left ? pb + rank : nb + (rank - left));
+                if (have && c < n) {
                     ci = c;
                     if (raw) {
                         g = in[c];
@@ -520,6 +533,14 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
                     }
                     busy = true;
                 }
+            }
+            if (left >= (unsigned long long)nidle) {
+                pb += (unsigned long long)nidle;
+            } else if (take) {
+                pb = nb + ((unsigned long long)nidle - left);
+                pe = nb + kEldChunk;
+            } else {
+                pb = pe;
             }
         }
         if (!__any(busy)) break;
