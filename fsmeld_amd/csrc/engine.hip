@@ -217,9 +217,10 @@ double ms_between(hipEvent_t a, hipEvent_t b) {
 // its kind: then it gets a shared entry, made on the stream of the handle that
 // needs it and published with an event the other handles' streams wait on.
 // (A new entry's allocation costs more than the statistics it saves once, so
-// keys that come back only after many others stay private.)  Beyond the byte cap (half of device memory)
-// a new entry takes over the buffers of the least recently used entry no
-// handle is using, so a cache that is full allocates and frees nothing.
+// keys that come back only after many others stay private.)  Beyond the byte
+// cap (half of device memory) a new entry takes over the buffers of the least
+// recently used entry no handle is using, so a cache that is full allocates
+// and frees nothing.
 struct CacheEntry {
     DevBuf a, b, c;  // statistics: mean, std, invalid flag; envelope: L, U
     hipEvent_t ready = nullptr;
@@ -242,7 +243,8 @@ struct TargetCache {
     std::shared_ptr<CacheEntry> take_idle(std::map<int, std::shared_ptr<CacheEntry>>& mp) {
         auto best = mp.end();
         for (auto it = mp.begin(); it != mp.end(); ++it)
-            if (it->second.use_count() == 1 && (best == mp.end() || it->second->tick < best->second->tick))
+            if (it->second.use_count() == 1 &&
+                (best == mp.end() || it->second->tick < best->second->tick))
                 best = it;
         if (best == mp.end()) return nullptr;
         std::shared_ptr<CacheEntry> e = std::move(best->second);

@@ -533,7 +533,8 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
 }
 
 template <int METRIC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_glb(VerifyArgs a, const double* __restrict__ Q,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_glb(VerifyArgs a, const double* __restrict__ Q,
                                              const double* __restrict__ LQ,
                                              const double* __restrict__ UQ,
                                              const double* __restrict__ XG,
@@ -658,7 +659,8 @@ hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
     const int stage_len = (a.p.m + 512 <= kGlbStage) ? kGlbStage : 0;
     const size_t lds = sizeof(double) * 3 * (size_t)stage_len;
 #define FSM_GLB(MM) \
-    hipLaunchKernelGGL(k_glb<MM>, g, b, lds, st, a, q.qn, q.LQ, q.UQ, q.XG, q.LQE, q.UQE, in, in_count, out, out_count, out_total, vals, queue, stage_len)
+    hipLaunchKernelGGL(k_glb<MM>, g, b, lds, st, a, q.qn, q.LQ, q.UQ, q.XG, q.LQE, q.UQE, in, \
+                       in_count, out, out_count, out_total, vals, queue, stage_len)
     switch (a.p.metric) {
         case DTW: FSM_GLB(DTW); break;
         case ERP: FSM_GLB(ERP); break;
