@@ -171,7 +171,9 @@ __device__ inline double cell(int i, int j, double up, double left, double diag,
 constexpr int kXPad = 40;
 constexpr size_t kEldLdsCap = 64 * 1024;
 constexpr double kPad = 1e300;
-constexpr int kRefillMin = 8;  // idle lanes that trigger a wave's refill
+// idle lanes that trigger a wave's refill (the integer EDR / LCSS kernels'
+// iterations are cheaper, so their refills are spaced further apart)
+template <int METRIC> constexpr int kRefillMin = (METRIC == EDR || METRIC == LCSS) ? 12 : 8;
 // queue positions a wave takes per atomic; refills draw on the wave's pool
 // (short-lived candidates would otherwise make one atomic per few lanes on
 // a single counter)
@@ -484,7 +486,7 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
     while (true) {
         const uint64_t idle = __ballot(!busy);
         const int nidle = __builtin_popcountll(idle);
-        if ((pb < pe || !drained) && (nidle >= kRefillMin || nidle == 64)) {
+        if ((pb < pe || !drained) && (nidle >= kRefillMin<METRIC> || nidle == 64)) {
             const unsigned long long left = pe - pb;
             const bool take = left < (unsigned long long)nidle && !drained;
             unsigned long long nb = 0;
