@@ -17,6 +17,7 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fsmeld.h"
@@ -38,18 +39,20 @@ void usage() {
                  "No exclusion is applied by default (self-matches are included).\n";
 }
 
-long long integer(const char* text, const char* name) {
-    size_t end = 0;
-    const auto v = std::stoll(text, &end);
-    if (text[end] != '\0') throw std::invalid_argument(std::string("Invalid ") + name);
-    return v;
-}
-
-double number(const char* text, const char* name) {
-    size_t end = 0;
-    const double v = std::stod(text, &end);
-    if (text[end] != '\0' || !std::isfinite(v))
+// A whole argument as a number, with std::stoll / std::stod doing the
+// conversion (their own exceptions report junk and overflow); trailing
+// characters, and for reals a non-finite value, are rejected with the
+// reference's messages.
+template <class T>
+T parse_arg(const char* text, const char* name) {
+    size_t used = 0;
+    const T v = std::is_integral<T>::value ? (T)std::stoll(text, &used) : (T)std::stod(text, &used);
+    const bool rest = text[used] != '\0';
+    if (std::is_integral<T>::value) {
+        if (rest) throw std::invalid_argument(std::string("Invalid ") + name);
+    } else if (rest || !std::isfinite((double)v)) {
         throw std::invalid_argument(std::string("Invalid finite ") + name);
+    }
     return v;
 }
 
@@ -103,16 +106,16 @@ int main(int argc, char** argv) {
         const std::string command = argv[1];
         if (command != "search" && command != "cutoff" && command != "exact")
             throw std::invalid_argument("Unknown command: " + command);
-        const auto length = integer(argv[4], "m");
+        const auto length = parse_arg<long long>(argv[4], "m");
         if (length < 2 || length > std::numeric_limits<int>::max() / 8)
             throw std::invalid_argument("m must be at least 2 and fit the FFT indexing range.");
         const int m = (int)length;
         const std::string metric = argv[6];
         if (!metric_code(metric)) throw std::invalid_argument("Unsupported metric: " + metric);
-        const auto radius = integer(argv[7], "w");
+        const auto radius = parse_arg<long long>(argv[7], "w");
         if (radius < 0 || radius >= m) throw std::invalid_argument("w must satisfy 0 <= w < m.");
         const int w = (int)radius;
-        const double arg = number(argv[8], "metric parameter");
+        const double arg = parse_arg<double>(argv[8], "metric parameter");
         if ((metric == "edr" || metric == "lcss" || metric == "msm") && arg < 0)
             throw std::invalid_argument("Matching tolerance and MSM cost must be non-negative.");
         const auto bytes = std::filesystem::file_size(argv[2]);
@@ -142,8 +145,8 @@ int main(int argc, char** argv) {
         }
         long long begin = -1, end = -1;
         if (argc == 11) {
-            begin = integer(argv[9], "exclude_start");
-            end = integer(argv[10], "exclude_end");
+            begin = parse_arg<long long>(argv[9], "exclude_start");
+            end = parse_arg<long long>(argv[10], "exclude_end");
             if (begin < 0 || end <= begin)
                 throw std::invalid_argument("Exclusion requires 0 <= start < end.");
         }
@@ -155,7 +158,7 @@ int main(int argc, char** argv) {
         d.exclude_begin = begin;
         d.exclude_end = end;
         if (command == "cutoff") {
-            const auto k = integer(argv[5], "k");
+            const auto k = parse_arg<long long>(argv[5], "k");
             if (k < 1 || k > std::numeric_limits<int>::max() ||
                 (unsigned long long)k > bytes / sizeof(double) - m + 1)
                 throw std::invalid_argument(
@@ -163,7 +166,7 @@ int main(int argc, char** argv) {
             d.mode = FSM_MODE_CUTOFF;
             d.k = (int)k;
         } else {
-            const double cutoff = number(argv[5], "cutoff");
+            const double cutoff = parse_arg<double>(argv[5], "cutoff");
             if (cutoff < 0) throw std::invalid_argument("cutoff must be non-negative.");
             d.mode = command == "search" ? FSM_MODE_SEARCH : FSM_MODE_EXACT;
             d.cutoff = cutoff;
@@ -239,7 +242,6 @@ int main(int argc, char** argv) {
             row.lb_time_seconds = lb_seconds;
             row.pruned_candidates = s.valid - s.after_glb;
             row.survivors = s.after_glb;
-            row.has_lbip = true;
             row.lbip_time_seconds = lbip_seconds;
             row.lbip_pruning_rate = lbip_rate;
         } else {

@@ -1,86 +1,118 @@
-// search_result.cpp -- CSV emitter of the CLI.  Column set, number format
-// (17 significant digits), escaping, header-on-new-file and the
-// FSMELD_RESULTS_CSV destination follow src/search_result.cpp:39-101.
+// search_result.cpp -- the experiment CSV record (format in search_result.h).
 #include "search_result.h"
 
-#include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <ctime>
 #include <filesystem>
 #include <fstream>
-#include <iomanip>
 #include <iostream>
-#include <sstream>
+#include <utility>
+#include <vector>
 
 namespace {
 
-std::string escape(const std::string& v) {
-    if (v.find_first_of(",\"\r\n") == std::string::npos) return v;
-    std::string out = "\"";
+using Cells = std::vector<std::pair<const char*, std::string>>;
+
+std::string real(double v) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%.17g", v);
+    return buf;
+}
+
+std::string count(std::uint64_t v) { return std::to_string(v); }
+
+// RFC 4180 style: only cells that need it are quoted
+std::string text(const std::string& v) {
+    bool plain = true;
+    for (char c : v)
+        if (c == ',' || c == '"' || c == '\r' || c == '\n') plain = false;
+    if (plain) return v;
+    std::string q;
+    q.reserve(v.size() + 2);
+    q.push_back('"');
     for (char c : v) {
-        if (c == '"') out += '"';
-        out += c;
+        q.push_back(c);
+        if (c == '"') q.push_back('"');
     }
-    return out + "\"";
+    q.push_back('"');
+    return q;
 }
 
-std::string now_iso() {
-    const std::time_t t = std::chrono::system_clock::to_time_t(std::chrono::system_clock::now());
+std::string local_time_now() {
+    const std::time_t now = std::time(nullptr);
     std::tm tm{};
-    localtime_r(&t, &tm);
-    std::ostringstream s;
-    s << std::put_time(&tm, "%Y-%m-%dT%H:%M:%S");
-    return s.str();
+    localtime_r(&now, &tm);
+    char buf[32];
+    std::strftime(buf, sizeof(buf), "%Y-%m-%dT%H:%M:%S", &tm);
+    return buf;
 }
 
-const char* header() {
-    return "dataset,normalization,metric,m,w,xi,method,lb_time,total_time,valid_candidates,"
-           "invalid_candidates,pruned_candidates,pruning_rate,survivors,matches,"
-           "lbip_time,lbip_pruning_rate,timestamp";
+// The record as (column, cell) pairs in column order.
+Cells cells_of(const ExperimentRow& r) {
+    const double rate = r.valid_candidates
+                            ? (double)r.pruned_candidates / (double)r.valid_candidates
+                            : 0.0;
+    Cells c;
+    c.emplace_back("dataset", text(r.dataset));
+    c.emplace_back("normalization", text(r.normalization));
+    c.emplace_back("metric", text(r.metric));
+    c.emplace_back("m", std::to_string(r.m));
+    c.emplace_back("w", std::to_string(r.w));
+    c.emplace_back("xi", real(r.xi));
+    c.emplace_back("method", text(r.method));
+    c.emplace_back("lb_time", real(r.lb_time_seconds));
+    c.emplace_back("total_time", r.total_time_seconds ? real(*r.total_time_seconds) : "");
+    c.emplace_back("valid_candidates", count(r.valid_candidates));
+    c.emplace_back("invalid_candidates", count(r.invalid_candidates));
+    c.emplace_back("pruned_candidates", count(r.pruned_candidates));
+    c.emplace_back("pruning_rate", real(rate));
+    c.emplace_back("survivors", count(r.survivors));
+    c.emplace_back("matches", r.matches ? count(*r.matches) : "");
+    c.emplace_back("lbip_time", r.lbip_time_seconds ? real(*r.lbip_time_seconds) : "");
+    c.emplace_back("lbip_pruning_rate", r.lbip_time_seconds ? real(r.lbip_pruning_rate) : "");
+    c.emplace_back("timestamp", local_time_now());
+    return c;
 }
 
-std::string row_text(const ExperimentRow& r) {
-    const double rate = r.valid_candidates == 0
-                            ? 0.0
-                            : (double)r.pruned_candidates / (double)r.valid_candidates;
-    std::ostringstream o;
-    o << std::setprecision(17) << escape(r.dataset) << ',' << escape(r.normalization) << ','
-      << escape(r.metric) << ',' << r.m << ',' << r.w << ',' << r.xi << ',' << escape(r.method)
-      << ',' << r.lb_time_seconds << ',' << r.total_time_seconds << ',' << r.valid_candidates
-      << ',' << r.invalid_candidates << ',' << r.pruned_candidates << ',' << rate << ','
-      << r.survivors << ',' << r.matches << ',';
-    if (r.has_lbip) o << r.lbip_time_seconds;
-    o << ',';
-    if (r.has_lbip) o << r.lbip_pruning_rate;
-    o << ',' << now_iso();
-    return o.str();
+std::string join(const Cells& c, bool names) {
+    std::string s;
+    for (size_t i = 0; i < c.size(); i++) {
+        if (i) s.push_back(',');
+        s += names ? std::string(c[i].first) : c[i].second;
+    }
+    return s;
 }
 
 }  // namespace
 
 std::string dataset_name_of(const char* target_path) {
     if (!target_path || !*target_path) return "unknown";
-    std::filesystem::path p(target_path);
-    const auto parent = p.parent_path();
-    return parent.empty() ? p.stem().string() : parent.filename().string();
+    const std::filesystem::path p(target_path);
+    if (p.has_parent_path()) return p.parent_path().filename().string();
+    return p.stem().string();
 }
 
+std::string experiment_csv_header() { return join(cells_of(ExperimentRow{}), true); }
+
+std::string experiment_csv_row(const ExperimentRow& r) { return join(cells_of(r), false); }
+
 void emit_experiment_row(const ExperimentRow& r) {
-    const std::string row = row_text(r);
-    std::cout << "EXPERIMENT_CSV_HEADER=" << header() << '\n';
-    std::cout << "EXPERIMENT_CSV=" << row << std::endl;
+    const Cells cells = cells_of(r);
+    const std::string header = join(cells, true), row = join(cells, false);
+    std::cout << "EXPERIMENT_CSV_HEADER=" << header << '\n'
+              << "EXPERIMENT_CSV=" << row << std::endl;
     const char* env = std::getenv("FSMELD_RESULTS_CSV");
-    const std::filesystem::path path = (env && *env) ? std::filesystem::path(env)
-                                                     : std::filesystem::path("experiment_results.csv");
+    const std::filesystem::path path = (env && *env) ? env : "experiment_results.csv";
     std::error_code ec;
-    const bool need_header =
-        !std::filesystem::exists(path, ec) || std::filesystem::file_size(path, ec) == 0;
+    const auto size = std::filesystem::file_size(path, ec);
+    const bool fresh = ec || size == 0;  // missing, unreadable or empty
     std::ofstream out(path, std::ios::app);
     if (!out) {
         std::cerr << "[Warning] Cannot append experiment CSV: " << path.string() << std::endl;
         return;
     }
-    if (need_header) out << header() << '\n';
+    if (fresh) out << header << '\n';
     out << row << '\n';
     std::cout << "Experiment CSV saved to: " << path.string() << std::endl;
 }

@@ -350,20 +350,31 @@ int cached(fsm_target* t, std::map<int, std::shared_ptr<CacheEntry>> TargetCache
         if (it != (C.*mp).end()) {
             e = it->second;
             e->tick = ++C.tick;
-        } else if (recent) {
-            // made, launched and its event recorded under the lock: whoever
-            // finds it afterwards waits on a recorded event
-            if (C.bytes + bytes > C.cap) e = C.take_idle(C.*mp);
-            if (!e) e = std::make_shared<CacheEntry>();
-            if (!e->ready) HIP_TRY(hipEventCreateWithFlags(&e->ready, hipEventDisableTiming));
-            const int rc = fill(*e);
-            if (rc) return rc;
-            HIP_TRY(hipEventRecord(e->ready, t->stream));
-            e->bytes = bytes;
-            e->tick = ++C.tick;
-            (C.*mp)[k] = e;
-            C.bytes += bytes;
-            *made = true;
+        } else if (recent && bytes <= C.cap) {
+            // Idle entries make room, least recently used first (one of the
+            // same kind is recycled, the others freed).  With every entry in
+            // use the key stays private, so the cache never exceeds its cap.
+            std::shared_ptr<CacheEntry> reuse;
+            while (C.bytes + bytes > C.cap) {
+                std::shared_ptr<CacheEntry> x = C.take_idle(C.*mp);
+                if (!x) x = C.take_idle(mp == &TargetCache::stats ? C.env : C.stats);
+                if (!x) break;
+                if (!reuse && x->ready) reuse = std::move(x);
+            }
+            if (C.bytes + bytes <= C.cap) {
+                // made, launched and its event recorded under the lock:
+                // whoever finds it afterwards waits on a recorded event
+                e = reuse ? std::move(reuse) : std::make_shared<CacheEntry>();
+                if (!e->ready) HIP_TRY(hipEventCreateWithFlags(&e->ready, hipEventDisableTiming));
+                const int rc = fill(*e);
+                if (rc) return rc;
+                HIP_TRY(hipEventRecord(e->ready, t->stream));
+                e->bytes = bytes;
+                e->tick = ++C.tick;
+                (C.*mp)[k] = e;
+                C.bytes += bytes;
+                *made = true;
+            }
         }
     }
     if (!e) {  // first request: the handle's own buffers
@@ -375,6 +386,11 @@ int cached(fsm_target* t, std::map<int, std::shared_ptr<CacheEntry>> TargetCache
     }
     if (!*made) HIP_TRY(hipStreamWaitEvent(t->stream, e->ready, 0));
     *out = std::move(e);
+    // this handle's own buffers are not needed while it uses a shared entry
+    // (fsm_query_bytes counts one set per handle)
+    own->a.release();
+    own->b.release();
+    own->c.release();
     return FSM_OK;
 }
 
@@ -814,7 +830,11 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
                     if (std::isfinite(x)) fin.push_back(x);
                 if ((int64_t)fin.size() >= d->k) {
                     std::nth_element(fin.begin(), fin.begin() + (d->k - 1), fin.end());
-                    xi0 = fin[d->k - 1];
+                    // widened as xi1 is (tighten_cutoff): the window whose
+                    // distance is the k-th must survive LB_IP and GLB, whose
+                    // rounded sums can exceed an equal exact distance by an
+                    // ulp (w = 0); the answer is cut to k afterwards
+                    xi0 = fin[d->k - 1] * (1.0 + 1e-9) + 1e-300;
                 }
             }
             P.cutoff = xi0;
@@ -1128,6 +1148,18 @@ uint64_t fsm_query_bytes(int64_t n, int metric, int m, int w) {
     if (eld_uses_scratch(metric, m, w))
         b += sizeof(double) * 2 * (uint64_t)(2 * w + 2) * (uint64_t)eld_scratch_threads();
     return b;
+}
+
+int fsm_target_cache_limit(fsm_target* t, uint64_t bytes) {
+    if (!t || !t->cache) return fail(FSM_ERR_ARG, "null target");
+    std::lock_guard<std::mutex> lk(t->cache->mu);
+    t->cache->cap = bytes;
+    // entries beyond the new cap go as soon as no handle is using them
+    while (t->cache->bytes > bytes) {
+        if (t->cache->take_idle(t->cache->stats)) continue;
+        if (!t->cache->take_idle(t->cache->env)) break;
+    }
+    return FSM_OK;
 }
 
 void fsm_target_trim(fsm_target* t) {

@@ -189,10 +189,105 @@ __global__ void __launch_bounds__(kEnvThreads) k_envelope(const double* __restri
     }
 }
 
+// Wide radii (2w + 1 > kEnvTile): the windows of the kEnvTile outputs
+// [p0, p0 + T) of a block all contain the middle run C = [p0 + T - 1 - w,
+// p0 + w].  Window p0 + d is C plus a suffix [p0 - w + d, p0 + T - 2 - w] of
+// the left run and a prefix [p0 + w + 1, p0 + d + w] of the right run (T - 1
+// samples each), so a block reduces C once (2w + 2 - T samples, coalesced)
+// and scans the two runs in LDS.  Samples outside [0, N) are the identity.
+__global__ void __launch_bounds__(kEnvThreads) k_envelope_wide(const double* __restrict__ T,
+                                                               int64_t N, int w,
+                                                               double* __restrict__ L,
+                                                               double* __restrict__ U) {
+    constexpr int TT = kEnvTile;
+    constexpr int PER = TT / kEnvThreads;
+    // [0]: left run, slot d = sample p0 - w + d (slot T - 1: identity),
+    //      scanned into suffixes; [1]: right run, slot d = sample p0 + w + d
+    //      (slot 0: identity), scanned into prefixes
+    __shared__ double smin[2][TT], smax[2][TT];
+    __shared__ double rmin[kEnvThreads / 64], rmax[kEnvThreads / 64];
+    const int64_t p0 = (int64_t)blockIdx.x * TT;
+    const double inf = __builtin_inf();
+    const int tid = threadIdx.x;
+    for (int d = tid; d < TT; d += kEnvThreads) {
+        const int64_t sl = p0 - w + d, sr = p0 + w + d;
+        const bool inl = d < TT - 1 && sl >= 0 && sl < N;
+        const bool inr = d > 0 && sr >= 0 && sr < N;
+        const double vl = inl ? T[sl] : 0.0, vr = inr ? T[sr] : 0.0;
+        smin[0][d] = inl ? vl : inf;
+        smax[0][d] = inl ? vl : -inf;
+        smin[1][d] = inr ? vr : inf;
+        smax[1][d] = inr ? vr : -inf;
+    }
+    // middle run, clipped to the target
+    double cmin = inf, cmax = -inf;
+    {
+        int64_t c0 = p0 + TT - 1 - w, c1 = p0 + w;  // inclusive
+        if (c0 < 0) c0 = 0;
+        if (c1 > N - 1) c1 = N - 1;
+        for (int64_t c = c0 + tid; c <= c1; c += kEnvThreads) {
+            const double v = T[c];
+            cmin = nmin(cmin, v);
+            cmax = nmax(cmax, v);
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+            cmin = nmin(cmin, __shfl_xor(cmin, off));
+            cmax = nmax(cmax, __shfl_xor(cmax, off));
+        }
+        if ((tid & 63) == 0) {
+            rmin[tid >> 6] = cmin;
+            rmax[tid >> 6] = cmax;
+        }
+    }
+    __syncthreads();
+    cmin = rmin[0];
+    cmax = rmax[0];
+    for (int k = 1; k < kEnvThreads / 64; k++) {
+        cmin = nmin(cmin, rmin[k]);
+        cmax = nmax(cmax, rmax[k]);
+    }
+    // Hillis-Steele scans: suffixes of the left run, prefixes of the right
+    for (int off = 1; off < TT; off <<= 1) {
+        double a[PER], b[PER], c[PER], e[PER];
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int d = tid + u * kEnvThreads;
+            const bool hs = d + off < TT, hp = d - off >= 0;
+            a[u] = hs ? nmin(smin[0][d], smin[0][d + off]) : smin[0][d];
+            b[u] = hs ? nmax(smax[0][d], smax[0][d + off]) : smax[0][d];
+            c[u] = hp ? nmin(smin[1][d - off], smin[1][d]) : smin[1][d];
+            e[u] = hp ? nmax(smax[1][d - off], smax[1][d]) : smax[1][d];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int d = tid + u * kEnvThreads;
+            smin[0][d] = a[u];
+            smax[0][d] = b[u];
+            smin[1][d] = c[u];
+            smax[1][d] = e[u];
+        }
+        __syncthreads();
+    }
+    for (int d = tid; d < TT; d += kEnvThreads) {
+        const int64_t pos = p0 + d;
+        if (pos >= N) break;
+        L[pos] = nmin(nmin(smin[0][d], cmin), smin[1][d]);
+        U[pos] = nmax(nmax(smax[0][d], cmax), smax[1][d]);
+    }
+}
+
 hipError_t launch_envelope(hipStream_t st, const double* T, int64_t N, int w, double* L,
                            double* U) {
+    if (w < 0) return hipErrorInvalidValue;
     const int span = kEnvTile + 2 * w;
-    if (span > 8 * kEnvThreads) return hipErrorInvalidValue;
+    if (span > 8 * kEnvThreads) {
+        // here 2w + 1 > kEnvTile, so every block's windows share a middle run
+        const int64_t blocks = (N + kEnvTile - 1) / kEnvTile;
+        hipLaunchKernelGGL(k_envelope_wide, dim3((unsigned)blocks), dim3(kEnvThreads), 0, st, T,
+                           N, w, L, U);
+        return hipGetLastError();
+    }
     int K = 0;
     while ((2 << K) <= 2 * w + 1) K++;  // 2^K <= 2w+1 < 2^(K+1)
     const int64_t blocks = (N + kEnvTile - 1) / kEnvTile;

@@ -2,11 +2,16 @@
 // the public C ABI only: each worker thread drives its own handle
 // (fsm_target_share), so queries run concurrently on separate HIP streams.
 //
-// Scheduling: first come, first served.  A free worker takes the oldest
-// queued query when the memory held by every worker (the largest estimate it
-// has served since it last released its buffers) plus this query's estimate
-// fits the budget; otherwise idle workers release their cached buffers
-// (fsm_target_trim) and the query waits for running ones to finish.
+// Memory: a quarter of the budget is the target's shared statistics /
+// envelope cache (its cap is set to that); the workers' buffers share the
+// rest.  Scheduling: first come, first served.  A free worker takes the
+// oldest queued query when the memory held by every worker (the largest
+// estimate it has served since it last released its buffers) plus this
+// query's estimate fits the workers' share; otherwise idle workers release
+// their cached buffers (fsm_target_trim) and the query waits for running
+// ones to finish.  A worker that holds nothing and finds the oldest query
+// too large wakes the others, so an idle worker whose buffers block it
+// releases them.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -40,7 +45,7 @@ struct Request {
 
 struct fsm_server {
     fsm_target* base = nullptr;
-    uint64_t budget = 0;
+    uint64_t budget = 0;  // the workers' share of mem_budget
     std::vector<fsm_target*> handles;
     std::vector<uint64_t> held;  // per worker: bytes its handle may hold
     std::vector<bool> busy;
@@ -106,6 +111,13 @@ struct fsm_server {
                     work_cv.notify_all();
                     continue;
                 }
+                // nothing of its own to release: wake the idle workers that
+                // hold buffers (they release them, or fit the query)
+                for (size_t k = 0; k < held.size(); k++)
+                    if (k != j && !busy[k] && held[k] > 0) {
+                        work_cv.notify_all();
+                        break;
+                    }
             }
             work_cv.wait(lk);
         }
@@ -117,7 +129,10 @@ int fsm_server_create(fsm_target* t, uint64_t mem_budget, int max_workers, fsm_s
     *out = nullptr;
     auto s = std::make_unique<fsm_server>();
     s->base = t;
-    s->budget = mem_budget;
+    const uint64_t cache_share = mem_budget / 4;
+    s->budget = mem_budget - cache_share;
+    const int lrc = fsm_target_cache_limit(t, cache_share);
+    if (lrc) return lrc;
     for (int j = 0; j < max_workers; j++) {
         fsm_target* h = nullptr;
         const int rc = fsm_target_share(t, &h);
@@ -153,7 +168,7 @@ int fsm_server_submit(fsm_server* s, const fsm_query_desc* d, const double* q, i
     s->reqs.emplace(id, std::move(r));
     s->queue.push_back(id);
     *ticket = id;
-    s->work_cv.notify_one();
+    s->work_cv.notify_all();
     return FSM_OK;
 }
 

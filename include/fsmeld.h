@@ -130,20 +130,29 @@ uint64_t fsm_query_bytes(int64_t n, int metric, int m, int w);
  * requests, up to half of device memory, least recently used first out. */
 void fsm_target_trim(fsm_target* t);
 
+/* Caps the device bytes of that shared cache (default: half of device
+ * memory); a new length or radius that would exceed the cap stays in the
+ * requesting handle's own buffers.  Idle entries beyond a lowered cap are
+ * freed at once; entries still in use are freed by later requests that need
+ * the room, or by fsm_target_trim, once no handle uses them. */
+int fsm_target_cache_limit(fsm_target* t, uint64_t bytes);
+
 /* ---- serving loop (no reference counterpart) ----
  * A pool of worker threads, each driving its own handle that shares t's
- * device samples, serves submitted queries first come, first served.  A free
- * worker starts the oldest queued query as soon as the working memory of the
- * running and cached queries plus its own estimate (fsm_query_bytes) fits
- * mem_budget; idle workers release cached buffers while the oldest query
- * does not fit.  Queries that arrive while others run are admitted as
+ * device samples, serves submitted queries first come, first served.  A
+ * quarter of mem_budget is the target's shared statistics / envelope cache
+ * (fsm_target_cache_limit is set to it); a free worker starts the oldest
+ * queued query as soon as the working memory of the running and cached
+ * queries plus its own estimate (fsm_query_bytes) fits the other three
+ * quarters; idle workers release cached buffers while the oldest query does
+ * not fit.  Queries that arrive while others run are admitted as
  * workers and memory free up, and their kernels overlap on the device. */
 #define FSM_PENDING 1             /* fsm_server_poll: not finished yet */
 #define FSM_ERR_CANCELLED (-7)    /* the query was cancelled */
 typedef struct fsm_server fsm_server;
 int fsm_server_create(fsm_target* t, uint64_t mem_budget, int max_workers, fsm_server** out);
 /* Copies the query; *ticket identifies it.  FSM_ERR_OOM if its estimate alone
- * exceeds the budget.  fsm_query's own errors (argument validation included)
+ * exceeds the workers' three quarters of the budget.  fsm_query's own errors (argument validation included)
  * arrive as the *status of fsm_server_poll. */
 int fsm_server_submit(fsm_server* s, const fsm_query_desc* desc, const double* query_raw,
                       int64_t query_len, uint64_t* ticket);

@@ -61,6 +61,7 @@ _sig("orc_thresholds", None, [_I, _D, _D, _I, _D, _D, _I, ctypes.c_double, _D, _
 _sig("orc_q_to_h", None, [_I, _D, _I, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D])
 _sig("orc_s_to_g", None, [_D, _D, _I, ctypes.c_double, ctypes.c_double, _D, _D])
 _sig("orc_sip_direct", None, [_D, _I, _D, _I, _D])
+_sig("orc_trans", ctypes.c_double, [_I, ctypes.c_double, ctypes.c_double, _I])
 _sig("orc_lbip_exceeds_cutoff", _I, [ctypes.c_double, ctypes.c_double, _I])
 _sig("orc_glb", ctypes.c_double, [_I, _D, _D, _I, ctypes.c_double, _D, _D, _I, ctypes.c_double])
 _sig("orc_eld", ctypes.c_double, [_I, _D, _D, _I, _I, ctypes.c_double])
@@ -69,7 +70,7 @@ _sig("orc_search", _I, [_D, _L, _D, _I, ctypes.c_double, _I, _I, ctypes.c_double
                         _L, _I, ctypes.POINTER(Stats), _I64, _L, _I64])
 _sig("orc_exact", _I, [_D, _L, _D, _I, ctypes.c_double, _I, _I, ctypes.c_double, _L, _L, _I,
                        ctypes.POINTER(Stats), _I64, _D, _L, _I64])
-_sig("orc_cutoff", _I, [_D, _L, _D, _I, _I, _I, _I, ctypes.c_double, _L, _L,
+_sig("orc_cutoff", _I, [_D, _L, _D, _I, _I, _I, _I, ctypes.c_double, _L, _L, _I,
                         ctypes.POINTER(Stats), _I64, _D])
 _sig("orc_window_stats", _I, [_D, _L, _I, _L, _L, _D, _D, _U8])
 _sig("orc_lbip_all", _I, [_D, _L, _D, _I, _I, _I, ctypes.c_double, _D, _D, _D])
@@ -156,6 +157,17 @@ def sip_direct(g, h):
     return z
 
 
+def trans(metric, zl, zu, m):
+    return _lib.orc_trans(_metric(metric), float(zl), float(zu), int(m))
+
+
+def s_to_g(Ln, Un, tl, tu):
+    Ln, Un = _a(Ln), _a(Un)
+    gl, gu = np.empty(Ln.size), np.empty(Un.size)
+    _lib.orc_s_to_g(_p(Ln), _p(Un), Ln.size, float(tl), float(tu), _p(gl), _p(gu))
+    return gl, gu
+
+
 def lbip_exceeds_cutoff(bound, cutoff, metric):
     return bool(_lib.orc_lbip_exceeds_cutoff(float(bound), float(cutoff), _metric(metric)))
 
@@ -215,16 +227,17 @@ def exact(T, q_raw, m, cutoff, metric, w, arg, exclude=None, threads=1):
     return out[: n.value].copy(), dist[: n.value].copy(), _stats(st)
 
 
-def cutoff(T, q_raw, m, k, metric, w, arg, exclude=None):
+def cutoff(T, q_raw, m, k, metric, w, arg, exclude=None, threads=1):
     """Returns (threshold, ranks, rank distances, stats); raises LookupError
-    when fewer than k candidates exist (src/fsmeld.cpp:519-520)."""
+    when fewer than k candidates exist (src/fsmeld.cpp:519-520).  threads > 1
+    merges the heaps of parallel segment ranges (same ranks and threshold)."""
     T, q = _a(T), _a(q_raw)
     st = Stats()
     xb, xe = _excl(exclude)
     ranks = np.empty(k, np.int64)
     rd = np.empty(k)
     rc = _lib.orc_cutoff(_p(T), T.size, _p(q), m, k, _metric(metric), w, float(arg), xb, xe,
-                         ctypes.byref(st), _p(ranks, _I64), _p(rd))
+                         int(threads), ctypes.byref(st), _p(ranks, _I64), _p(rd))
     if rc == -4:
         raise LookupError("Fewer valid, non-excluded candidates than requested k.")
     if rc:

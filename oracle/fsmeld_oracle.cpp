@@ -1090,6 +1090,7 @@ void orc_s_to_g(const double* Ln, const double* Un, int n, double tl, double tu,
 void orc_sip_direct(const double* g, int n, const double* h, int m, double* z) {
     sip_direct(g, n, h, m, z);
 }
+double orc_trans(int metric, double zl, double zu, int m) { return trans1(metric, zl, zu, m); }
 int orc_lbip_exceeds_cutoff(double bound, double cutoff, int metric) {
     return lbip_exceeds_cutoff(bound, cutoff, metric) ? 1 : 0;
 }
@@ -1218,23 +1219,29 @@ int orc_exact(const double* T, int64_t N, const double* q_raw, int m, double xi,
     return 0;
 }
 
-// src/fsmeld.cpp:185-569 (cutoff, filter_by_initial_epsilon = false), sequential.
-int orc_cutoff(const double* T, int64_t N, const double* q_raw, int m, int k, int metric,
-               int w, double arg, int64_t xb, int64_t xe, orc_stats* st, int64_t* ranks,
-               double* rank_dists) {
-    if (check_args(N, m, metric, w) || k < 1) return -1;
-    Geometry G;
-    if (!geometry(N, m, &G)) return -1;
-    Query Q;
-    prepare_query(q_raw, m, w, &Q);
+}  // extern "C"
+
+namespace {
+
+// src/fsmeld.cpp:185-569 (cutoff, filter_by_initial_epsilon = false) over
+// segments [s0, s1): the reference's max-heap of (distance, index) pairs with
+// strict-< replacement, scanned in index order, LB_IP at DBL_MAX until the
+// heap is full and GLB only after that.  The heap it leaves holds the k
+// (distance, index)-smallest windows of the range.
+struct CutoffRange {
+    std::vector<std::pair<double, long long>> best;  // ascending (distance, index)
+    uint64_t kk1 = 0, lbp = 0, glbp = 0, ex = 0;
+};
+
+void cutoff_range(const double* T, int64_t N, const Geometry& G, const Query& Q, int m, int k,
+                  int metric, int w, double arg, int64_t xb, int64_t xe, int64_t s0, int64_t s1,
+                  CutoffRange& R) {
     SegmentWork W;
     W.init(G.m_k, G.k_m_l, m);
     std::priority_queue<std::pair<double, long long>> heap;
     double threshold = std::numeric_limits<double>::max();  // src/main.cpp:95
-    uint64_t kk1 = 0, lbp = 0, glbp = 0, ex = 0;
     std::vector<double> tt(m);
-    const int64_t S = G.forsize + (G.tail > 0 ? 1 : 0);
-    for (int64_t seg = 0; seg < S; seg++) {
+    for (int64_t seg = s0; seg < s1; seg++) {
         const bool is_tail = seg == G.forsize;
         const double* t = is_tail ? T + (N - G.m_k) : T + seg * G.k_m_l;
         const int first = is_tail ? G.tail_first : 0;
@@ -1244,11 +1251,11 @@ int orc_cutoff(const double* T, int64_t N, const double* q_raw, int m, int k, in
         segment_lbip(t, G, Q, metric, w, arg, W, nullptr, nullptr);
         for (int j = first; j < first + count; j++) {
             if (W.mask[j]) continue;
-            kk1++;
+            R.kk1++;
             const bool full = heap.size() >= static_cast<size_t>(k);
             const double active = full ? threshold : DBL_MAX;
             if (lbip_exceeds_cutoff(W.z[j], active, metric)) {
-                lbp++;
+                R.lbp++;
                 continue;
             }
             normalize_window(t, j, m, W.mu[j], W.si[j], tt.data());
@@ -1256,12 +1263,12 @@ int orc_cutoff(const double* T, int64_t N, const double* q_raw, int m, int k, in
                 const double gd = glb(metric, Q.qn.data(), m, tt.data(), m, arg, Q.LQ.data(),
                                       Q.UQ.data(), w, threshold);
                 if (gd > threshold) {
-                    glbp++;
+                    R.glbp++;
                     continue;
                 }
             }
             const double d = eld(metric, Q.qn.data(), tt.data(), m, w, arg);
-            ex++;
+            R.ex++;
             if (!std::isfinite(d)) continue;
             const long long idx = base + (j - first);
             if (heap.size() < static_cast<size_t>(k)) {
@@ -1274,20 +1281,60 @@ int orc_cutoff(const double* T, int64_t N, const double* q_raw, int m, int k, in
             }
         }
     }
+    R.best.clear();
+    while (!heap.empty()) {
+        R.best.push_back(heap.top());
+        heap.pop();
+    }
+    std::reverse(R.best.begin(), R.best.end());
+}
+
+}  // namespace
+
+extern "C" {
+
+// threads <= 1: the reference's sequential scan over all segments.
+// threads > 1: contiguous segment ranges scanned as above in parallel; the k
+// (distance, index)-smallest of the ranges' heaps are the sequential heap's
+// contents (each range's heap holds its k smallest), so ranks and threshold
+// are the same; only the pruning counters describe the split schedule.
+int orc_cutoff(const double* T, int64_t N, const double* q_raw, int m, int k, int metric,
+               int w, double arg, int64_t xb, int64_t xe, int threads, orc_stats* st,
+               int64_t* ranks, double* rank_dists) {
+    if (check_args(N, m, metric, w) || k < 1) return -1;
+    Geometry G;
+    if (!geometry(N, m, &G)) return -1;
+    Query Q;
+    prepare_query(q_raw, m, w, &Q);
+    const int64_t S = G.forsize + (G.tail > 0 ? 1 : 0);
+    const int nt = threads > 1 ? threads : 1;
+    const int64_t parts = nt > 1 ? std::min<int64_t>(S, 4 * (int64_t)nt) : 1;
+    std::vector<CutoffRange> res(parts);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+#endif
+    for (int64_t p = 0; p < parts; p++)
+        cutoff_range(T, N, G, Q, m, k, metric, w, arg, xb, xe, S * p / parts,
+                     S * (p + 1) / parts, res[p]);
+    std::vector<std::pair<double, long long>> v;
+    uint64_t kk1 = 0, lbp = 0, glbp = 0, ex = 0;
+    for (auto& r : res) {
+        v.insert(v.end(), r.best.begin(), r.best.end());
+        kk1 += r.kk1;
+        lbp += r.lbp;
+        glbp += r.glbp;
+        ex += r.ex;
+    }
+    std::sort(v.begin(), v.end());
+    if (v.size() > static_cast<size_t>(k)) v.resize(k);
     std::memset(st, 0, sizeof(*st));
     fill_stats(G, st);
     st->valid = kk1;
     st->after_lbip = kk1 - lbp;
     st->after_glb = kk1 - lbp - glbp;
     st->exact_evals = ex;
-    if (heap.size() < static_cast<size_t>(k)) return -4;  // src/fsmeld.cpp:519-520
-    st->threshold = threshold;
-    std::vector<std::pair<double, long long>> v;
-    while (!heap.empty()) {
-        v.push_back(heap.top());
-        heap.pop();
-    }
-    std::reverse(v.begin(), v.end());
+    if (v.size() < static_cast<size_t>(k)) return -4;  // src/fsmeld.cpp:519-520
+    st->threshold = v.back().first;
     for (size_t i = 0; i < v.size(); i++) {
         ranks[i] = v[i].second;
         if (rank_dists) rank_dists[i] = v[i].first;

@@ -37,6 +37,8 @@ void orc_s_to_g(const double* L_n, const double* U_n, int n, double t_l, double 
                 double* g_l, double* g_u);
 /* direct sliding inner product: z[j+m-1] = sum_{s<m} g[j+s] h[s], j in [0, n-m] */
 void orc_sip_direct(const double* g, int n, const double* h, int m, double* z);
+/* LB_IP value of one window from its two sliding sums (trans, src/fsmeld.cpp:84-106) */
+double orc_trans(int metric, double z_l, double z_u, int m);
 int orc_lbip_exceeds_cutoff(double bound, double cutoff, int metric);
 double orc_glb(int metric, const double* Q, const double* T, int m, double arg,
                const double* L_Q, const double* U_Q, int w, double bsf);
@@ -66,9 +68,11 @@ int orc_exact(const double* T, int64_t N, const double* q_raw, int m, double cut
               int metric, int w, double arg, int64_t excl_b, int64_t excl_e,
               int threads, orc_stats* st, int64_t* matches, double* dists,
               int64_t cap, int64_t* n_out);
+/* threads > 1 scans contiguous segment ranges in parallel and merges their
+ * heaps: the same ranks and threshold, counters of the split schedule. */
 int orc_cutoff(const double* T, int64_t N, const double* q_raw, int m, int k,
                int metric, int w, double arg, int64_t excl_b, int64_t excl_e,
-               orc_stats* st, int64_t* ranks, double* rank_dists);
+               int threads, orc_stats* st, int64_t* ranks, double* rank_dists);
 
 /* Per-window stage outputs in global window order (ts_n entries), for
  * stage-level parity: mu/sigma/invalid of the owning segment, the LB_IP
