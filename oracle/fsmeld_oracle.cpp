@@ -734,25 +734,51 @@ double glb(int metric, const double* Q, int lenQ, const double* T, int lenT, dou
 // reference's arrays hold there.
 inline double min3(double a, double b, double c) { return std::min({a, b, c}); }
 
+// Band rows: slot k = j - i + w of a row holds column j; a row carries one
+// padding slot on each side (index -1 and W) that keeps the out-of-band value,
+// so neighbours outside the band read it without a bounds test.  Buffers are
+// per thread.
+struct BandRows {
+    std::vector<double> a, b;
+    double* prev;
+    double* cur;
+    int W;
+    void init(int W_, double pad) {
+        W = W_;
+        a.assign((size_t)W + 2, pad);
+        b.assign((size_t)W + 2, pad);
+        prev = a.data() + 1;
+        cur = b.data() + 1;
+    }
+    void clear_cur(double v) { std::fill(cur, cur + W, v); }
+    void swap() { std::swap(prev, cur); }
+};
+thread_local BandRows g_rows;
+
 // src/ElasticMeasures.cpp:24-61 (A = query, B = candidate, n = m)
 double eld_dtw(const double* A, const double* B, int m, int r) {
-    std::vector<double> prev(2 * r + 1, kEldSentinel), cur(2 * r + 1, kEldSentinel);
+    BandRows& R = g_rows;
+    R.init(2 * r + 1, kEldSentinel);
     for (int i = 0; i < m; i++) {
-        std::fill(cur.begin(), cur.end(), kEldSentinel);
-        int k = std::max(0, r - i);
-        for (int j = std::max(0, i - r); j <= std::min(m - 1, i + r); j++, k++) {
+        R.clear_cur(kEldSentinel);
+        double* C = R.cur;
+        const double* P = R.prev;
+        const int j0 = std::max(0, i - r), j1 = std::min(m - 1, i + r);
+        for (int j = j0, k = j0 - i + r; j <= j1; j++, k++) {
             if (i == 0 && j == 0) {
-                cur[k] = sq(A[0], B[0]);
+                C[k] = sq(A[0], B[0]);
                 continue;
             }
-            const double y = (j - 1 < 0 || k - 1 < 0) ? kEldSentinel : cur[k - 1];
-            const double x = (i - 1 < 0 || k + 1 > 2 * r) ? kEldSentinel : prev[k + 1];
-            const double z = (i - 1 < 0 || j - 1 < 0) ? kEldSentinel : prev[k];
-            cur[k] = min3(x, y, z) + sq(A[i], B[j]);
+            // column j - 1 of this row, column j and j - 1 of the last row;
+            // the unwritten slots (j < 0, i < 0, outside the band) hold 1e18
+            const double y = C[k - 1];
+            const double x = P[k + 1];
+            const double z = P[k];
+            C[k] = min3(x, y, z) + sq(A[i], B[j]);
         }
-        std::swap(cur, prev);
+        R.swap();
     }
-    const double f = prev[r];
+    const double f = R.prev[r];
     if (!std::isfinite(f)) return kEldSentinel;
     return std::sqrt(std::max(0.0, f));
 }
@@ -760,93 +786,80 @@ double eld_dtw(const double* A, const double* B, int m, int r) {
 // src/ElasticMeasures.cpp:63-93 (Q = query, A = candidate); out-of-band = 0.
 double eld_lcss(const double* Q, const double* A, int n, int w, double eps) {
     const int m = n;
-    const int W = 2 * w + 1;
-    std::vector<double> prev(W, 0.0), cur(W, 0.0);
-    auto at = [&](const std::vector<double>& row, int i, int j) -> double {
-        const int k = j - i + w;
-        if (j < 0 || j >= m || k < 0 || k >= W) return 0.0;
-        return row[k];
-    };
+    BandRows& R = g_rows;
+    R.init(2 * w + 1, 0.0);
     for (int i = 0; i < n; ++i) {
         const int wmin = std::max(0, i - w), wmax = std::min(m, i + w + 1);
-        std::fill(cur.begin(), cur.end(), 0.0);
-        for (int j = wmin; j < wmax; ++j) {
-            const int k = j - i + w;
+        R.clear_cur(0.0);
+        double* C = R.cur;
+        const double* P = R.prev;
+        for (int j = wmin, k = wmin - i + w; j < wmax; ++j, ++k) {
             const bool match = std::abs(Q[i] - A[j]) <= eps;
             double v;
             if (i + j == 0) {
                 v = match ? 1 : 0;
             } else if (i == 0) {
-                v = match ? 1 : ((j > wmin) ? cur[k - 1] : 0);
+                v = match ? 1 : ((j > wmin) ? C[k - 1] : 0);
             } else if (j == 0) {
-                v = match ? 1 : at(prev, i - 1, j);
+                v = match ? 1 : P[k + 1];
             } else {
-                const double v1 = at(prev, i - 1, j - 1);
-                const double v2 = at(prev, i - 1, j);
-                const double v3 = (j > wmin) ? cur[k - 1] : 0;
+                const double v1 = P[k];
+                const double v2 = P[k + 1];
+                const double v3 = (j > wmin) ? C[k - 1] : 0;
                 v = match ? std::max({v1 + 1, v2, v3}) : std::max({v1, v2, v3});
             }
-            cur[k] = v;
+            C[k] = v;
         }
-        std::swap(cur, prev);
+        R.swap();
     }
-    const double res = at(prev, n - 1, m - 1);
+    const double res = R.prev[(m - 1) - (n - 1) + w];
     return 1 - res / ((n < m) ? n : m);
 }
 
 // src/ElasticMeasures.cpp:95-130 (x = query, y = candidate); rows 0..m.
 double eld_erp(const double* x, const double* y, int m, double g, int w) {
-    const int W = 2 * w + 1;  // band index k = j - i + w
-    std::vector<double> prev(W, kEldSentinel), cur(W, kEldSentinel);
-    auto at = [&](const std::vector<double>& row, int i, int j) -> double {
-        const int k = j - i + w;
-        if (j < 0 || j > m || k < 0 || k >= W) return kEldSentinel;
-        return row[k];
-    };
-    prev[w] = 0.0;  // (0,0)
-    for (int j = 1; j <= std::min(m, w); ++j) prev[j + w] = prev[j - 1 + w] + sq(y[j - 1], g);
+    BandRows& R = g_rows;
+    R.init(2 * w + 1, kEldSentinel);  // band index k = j - i + w
+    R.prev[w] = 0.0;                  // (0,0)
+    for (int j = 1; j <= std::min(m, w); ++j) R.prev[j + w] = R.prev[j - 1 + w] + sq(y[j - 1], g);
     for (int i = 1; i <= m; ++i) {
-        std::fill(cur.begin(), cur.end(), kEldSentinel);
-        if (i <= w) cur[w - i] = at(prev, i - 1, 0) + sq(x[i - 1], g);
+        R.clear_cur(kEldSentinel);
+        double* C = R.cur;
+        const double* P = R.prev;
+        if (i <= w) C[w - i] = P[w - i + 1] + sq(x[i - 1], g);
         const int minw = std::max(1, i - w), maxw = std::min(m, i + w);
-        for (int j = minw; j <= maxw; ++j) {
-            const int k = j - i + w;
-            const double match = at(prev, i - 1, j - 1) + sq(x[i - 1], y[j - 1]);
-            const double ins = at(cur, i, j - 1) + sq(y[j - 1], g);
-            const double del = at(prev, i - 1, j) + sq(x[i - 1], g);
-            cur[k] = min3(match, ins, del);
+        for (int j = minw, k = minw - i + w; j <= maxw; ++j, ++k) {
+            const double match = P[k] + sq(x[i - 1], y[j - 1]);
+            const double ins = C[k - 1] + sq(y[j - 1], g);
+            const double del = P[k + 1] + sq(x[i - 1], g);
+            C[k] = min3(match, ins, del);
         }
-        std::swap(cur, prev);
+        R.swap();
     }
-    const double res = at(prev, m, m);
+    const double res = R.prev[w];
     if (!std::isfinite(res)) return kEldSentinel;
     return std::sqrt(std::max(0.0, res));
 }
 
 // src/ElasticMeasures.cpp:132-165 (x = query, y = candidate).
 double eld_edr(const double* x, const double* y, int m, double eps, int w) {
-    const int W = 2 * w + 1;
-    std::vector<double> prev(W, kEldSentinel), cur(W, kEldSentinel);
-    auto at = [&](const std::vector<double>& row, int i, int j) -> double {
-        const int k = j - i + w;
-        if (j < 0 || j > m || k < 0 || k >= W) return kEldSentinel;
-        return row[k];
-    };
-    prev[w] = 0.0;
-    for (int j = 1; j <= std::min(m, w); ++j) prev[j + w] = static_cast<double>(j);
+    BandRows& R = g_rows;
+    R.init(2 * w + 1, kEldSentinel);
+    R.prev[w] = 0.0;
+    for (int j = 1; j <= std::min(m, w); ++j) R.prev[j + w] = static_cast<double>(j);
     for (int i = 1; i <= m; ++i) {
-        std::fill(cur.begin(), cur.end(), kEldSentinel);
-        if (i <= w) cur[w - i] = static_cast<double>(i);
+        R.clear_cur(kEldSentinel);
+        double* C = R.cur;
+        const double* P = R.prev;
+        if (i <= w) C[w - i] = static_cast<double>(i);
         const int minw = std::max(1, i - w), maxw = std::min(m, i + w);
-        for (int j = minw; j <= maxw; ++j) {
-            const int k = j - i + w;
+        for (int j = minw, k = minw - i + w; j <= maxw; ++j, ++k) {
             const double sub = (std::abs(x[i - 1] - y[j - 1]) <= eps) ? 0.0 : 1.0;
-            cur[k] = min3(at(prev, i - 1, j) + 1.0, at(cur, i, j - 1) + 1.0,
-                          at(prev, i - 1, j - 1) + sub);
+            C[k] = min3(P[k + 1] + 1.0, C[k - 1] + 1.0, P[k] + sub);
         }
-        std::swap(cur, prev);
+        R.swap();
     }
-    return at(prev, m, m);
+    return R.prev[w];
 }
 
 // src/ElasticMeasures.cpp:167-198 (x = query, y = candidate).
@@ -855,33 +868,29 @@ inline double msm_cost(double np, double x, double y, double c) {
     return c + std::min(std::abs(np - x), std::abs(np - y));
 }
 double eld_msm(const double* x, const double* y, int m, double c, int w) {
-    const int W = 2 * w + 1;
-    std::vector<double> prev(W, kEldSentinel), cur(W, kEldSentinel);
-    auto at = [&](const std::vector<double>& row, int i, int j) -> double {
-        const int k = j - i + w;
-        if (j < 0 || j >= m || k < 0 || k >= W) return kEldSentinel;
-        return row[k];
-    };
+    BandRows& R = g_rows;
+    R.init(2 * w + 1, kEldSentinel);
     // row 0
-    std::fill(cur.begin(), cur.end(), kEldSentinel);
-    cur[w] = std::abs(x[0] - y[0]);
+    R.clear_cur(kEldSentinel);
+    R.cur[w] = std::abs(x[0] - y[0]);
     for (int j = 1; j < m && j <= w; ++j)
-        cur[j + w] = cur[j - 1 + w] + msm_cost(y[j], x[0], y[j - 1], c);
-    std::swap(cur, prev);
+        R.cur[j + w] = R.cur[j - 1 + w] + msm_cost(y[j], x[0], y[j - 1], c);
+    R.swap();
     for (int i = 1; i < m; ++i) {
-        std::fill(cur.begin(), cur.end(), kEldSentinel);
-        if (i <= w) cur[w - i] = at(prev, i - 1, 0) + msm_cost(x[i], x[i - 1], y[0], c);
+        R.clear_cur(kEldSentinel);
+        double* C = R.cur;
+        const double* P = R.prev;
+        if (i <= w) C[w - i] = P[w - i + 1] + msm_cost(x[i], x[i - 1], y[0], c);
         const int js = std::max(1, i - w), je = std::min(m, i + w + 1);
-        for (int j = js; j < je; ++j) {
-            const int k = j - i + w;
-            const double c1 = at(prev, i - 1, j - 1) + std::abs(x[i] - y[j]);
-            const double c2 = at(prev, i - 1, j) + msm_cost(x[i], x[i - 1], y[j], c);
-            const double c3 = at(cur, i, j - 1) + msm_cost(y[j], x[i], y[j - 1], c);
-            cur[k] = min3(c1, c2, c3);
+        for (int j = js, k = js - i + w; j < je; ++j, ++k) {
+            const double c1 = P[k] + std::abs(x[i] - y[j]);
+            const double c2 = P[k + 1] + msm_cost(x[i], x[i - 1], y[j], c);
+            const double c3 = C[k - 1] + msm_cost(y[j], x[i], y[j - 1], c);
+            C[k] = min3(c1, c2, c3);
         }
-        std::swap(cur, prev);
+        R.swap();
     }
-    return at(prev, m - 1, m - 1);
+    return R.prev[w];
 }
 
 // src/fsmeld.cpp:170-183
@@ -997,24 +1006,39 @@ void normalize_window(const double* t, int j, int m, double mu, double si, doubl
     for (int s = 0; s < m; s++) out[s] = (t[s + j] - mu) / si;  // src/fsmeld.cpp:822-826
 }
 
-struct SegResult {
-    uint64_t valid = 0, invalid = 0, k2 = 0, k3 = 0;
-    std::vector<int64_t> matches;
+// A window that passed the filters, to be verified: its global index, its
+// start j inside segment seg and the statistics that normalise it.
+struct Survivor {
+    int64_t index, seg;
+    int j;
+    double mu, si;
 };
 
-// One search segment: src/fsmeld.cpp:812-851 (full) / :926-968 (tail).
+struct SegResult {
+    uint64_t valid = 0, invalid = 0, k2 = 0, k3 = 0;
+    std::vector<Survivor> surv;
+};
+
+const double* segment_base(const double* T, int64_t N, const Geometry& G, int64_t seg) {
+    return seg == G.forsize ? T + (N - G.m_k) : T + seg * G.k_m_l;
+}
+
+// One search segment's filters: src/fsmeld.cpp:812-851 (full) / :926-968
+// (tail).  The reference buffers the GLB survivors of a block and verifies
+// them afterwards; here they are verified by the caller, in parallel over all
+// survivors of the target (the verdict of each is independent of the rest).
 void search_segment(const double* T, int64_t N, const Geometry& G, const Query& Q,
                     int metric, int w, double arg, double xi, int64_t xb, int64_t xe,
                     int64_t seg, SegmentWork& W, SegResult& R) {
     const int m = Q.m;
     const bool is_tail = seg == G.forsize;
-    const double* t = is_tail ? T + (N - G.m_k) : T + seg * G.k_m_l;
+    const double* t = segment_base(T, N, G, seg);
     const int first = is_tail ? G.tail_first : 0;
     const int count = is_tail ? static_cast<int>(G.tail) : static_cast<int>(G.k_m_l);
     const int64_t base = seg * G.k_m_l;
     segment_stats(t, G, m, base, first, count, xb, xe, W);
     segment_lbip(t, G, Q, metric, w, arg, W, nullptr, nullptr);
-    std::vector<std::pair<int64_t, std::vector<double>>> surv;
+    std::vector<double> tt(m);
     for (int j = first; j < first + count; j++) {
         if (W.mask[j]) {
             R.invalid++;
@@ -1023,18 +1047,36 @@ void search_segment(const double* T, int64_t N, const Geometry& G, const Query& 
         R.valid++;
         if (lbip_exceeds_cutoff(W.z[j], xi, metric)) continue;
         R.k2++;
-        std::vector<double> tt(m);
         normalize_window(t, j, m, W.mu[j], W.si[j], tt.data());
         const double gd = glb(metric, Q.qn.data(), m, tt.data(), m, arg, Q.LQ.data(),
                               Q.UQ.data(), w, xi);
         if (gd <= xi) {
             R.k3++;
-            surv.emplace_back(base + (j - first), std::move(tt));
+            R.surv.push_back({base + (j - first), seg, j, W.mu[j], W.si[j]});
         }
     }
-    for (auto& s : surv) {
-        const double d = eld(metric, Q.qn.data(), s.second.data(), m, w, arg);
-        if (std::isfinite(d) && d <= xi) R.matches.push_back(s.first);
+}
+
+// Exact distances of the survivors, in parallel; dist[i] belongs to v[i].
+void verify_all(const double* T, int64_t N, const Geometry& G, const Query& Q, int metric,
+                int w, double arg, const std::vector<Survivor>& v, int threads,
+                std::vector<double>& dist) {
+    const int m = Q.m;
+    dist.assign(v.size(), 0.0);
+#ifdef _OPENMP
+    const int nt = threads > 0 ? threads : 1;
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        std::vector<double> tt(m);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (int64_t i = 0; i < (int64_t)v.size(); i++) {
+            normalize_window(segment_base(T, N, G, v[i].seg), v[i].j, m, v[i].mu, v[i].si,
+                             tt.data());
+            dist[i] = eld(metric, Q.qn.data(), tt.data(), m, w, arg);
+        }
     }
 }
 
@@ -1124,8 +1166,8 @@ int orc_search(const double* T, int64_t N, const double* q_raw, int m, double xi
     if (seg_begin < 0) seg_begin = 0;
     const int64_t nseg = seg_end > seg_begin ? seg_end - seg_begin : 0;
     std::vector<SegResult> res(nseg);
-#ifdef _OPENMP
     const int nt = threads > 0 ? threads : 1;
+#ifdef _OPENMP
 #pragma omp parallel num_threads(nt)
 #endif
     {
@@ -1139,16 +1181,22 @@ int orc_search(const double* T, int64_t N, const double* q_raw, int m, double xi
     }
     std::memset(st, 0, sizeof(*st));
     fill_stats(G, st);
-    int64_t n = 0;
+    std::vector<Survivor> all;
     for (auto& r : res) {
         st->valid += r.valid;
         st->invalid += r.invalid;
         st->after_lbip += r.k2;
         st->after_glb += r.k3;
-        for (int64_t idx : r.matches) {
-            if (n < cap) out[n] = idx;
-            n++;
-        }
+        all.insert(all.end(), r.surv.begin(), r.surv.end());
+        std::vector<Survivor>().swap(r.surv);
+    }
+    std::vector<double> dist;
+    verify_all(T, N, G, Q, metric, w, arg, all, nt, dist);
+    int64_t n = 0;
+    for (size_t i = 0; i < all.size(); i++) {  // survivors are in index order
+        if (!(std::isfinite(dist[i]) && dist[i] <= xi)) continue;
+        if (n < cap) out[n] = all[i].index;
+        n++;
     }
     st->exact_evals = st->after_glb;
     st->matches = static_cast<uint64_t>(n);
@@ -1166,22 +1214,21 @@ int orc_exact(const double* T, int64_t N, const double* q_raw, int m, double xi,
     Query Q;
     prepare_query(q_raw, m, w, &Q);
     const int64_t S = G.forsize + (G.tail > 0 ? 1 : 0);
-    struct R { uint64_t valid = 0, invalid = 0; std::vector<std::pair<int64_t, double>> hit; };
+    struct R { uint64_t valid = 0, invalid = 0; std::vector<Survivor> win; };
     std::vector<R> res(S);
-#ifdef _OPENMP
     const int nt = threads > 0 ? threads : 1;
+#ifdef _OPENMP
 #pragma omp parallel num_threads(nt)
 #endif
     {
         SegmentWork W;
         W.init(G.m_k, G.k_m_l, m);
-        std::vector<double> tt(m);
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 4)
 #endif
         for (int64_t seg = 0; seg < S; seg++) {
             const bool is_tail = seg == G.forsize;
-            const double* t = is_tail ? T + (N - G.m_k) : T + seg * G.k_m_l;
+            const double* t = segment_base(T, N, G, seg);
             const int first = is_tail ? G.tail_first : 0;
             const int count = is_tail ? static_cast<int>(G.tail) : static_cast<int>(G.k_m_l);
             const int64_t base = seg * G.k_m_l;
@@ -1192,26 +1239,29 @@ int orc_exact(const double* T, int64_t N, const double* q_raw, int m, double xi,
                     continue;
                 }
                 res[seg].valid++;
-                normalize_window(t, j, m, W.mu[j], W.si[j], tt.data());
-                const double d = eld(metric, Q.qn.data(), tt.data(), m, w, arg);
-                if (!std::isfinite(d)) continue;
-                if (d <= xi) res[seg].hit.emplace_back(base + (j - first), d);
+                res[seg].win.push_back({base + (j - first), seg, j, W.mu[j], W.si[j]});
             }
         }
     }
     std::memset(st, 0, sizeof(*st));
     fill_stats(G, st);
-    int64_t n = 0;
+    std::vector<Survivor> all;
     for (auto& r : res) {
         st->valid += r.valid;
         st->invalid += r.invalid;
-        for (auto& h : r.hit) {
-            if (n < cap) {
-                out[n] = h.first;
-                if (dists) dists[n] = h.second;
-            }
-            n++;
+        all.insert(all.end(), r.win.begin(), r.win.end());
+        std::vector<Survivor>().swap(r.win);
+    }
+    std::vector<double> dist;
+    verify_all(T, N, G, Q, metric, w, arg, all, nt, dist);
+    int64_t n = 0;
+    for (size_t i = 0; i < all.size(); i++) {
+        if (!(std::isfinite(dist[i]) && dist[i] <= xi)) continue;
+        if (n < cap) {
+            out[n] = all[i].index;
+            if (dists) dists[n] = dist[i];
         }
+        n++;
     }
     st->exact_evals = st->valid;
     st->matches = static_cast<uint64_t>(n);

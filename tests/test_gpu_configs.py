@@ -1,0 +1,210 @@
+"""GPU parity at the benchmark configurations' shapes, against the oracle.
+
+* plain large (2^26 random walk + spikes + constant runs, m = 512, w = 25,
+  five metrics, cutoffs from k = 100): the first 2^21 samples of that target;
+  cutoff threshold bits, ranking and distance bits against the oracle's
+  cutoff, the search's windows against the oracle's search, the exact scan's
+  windows and every distance against the oracle's verifier, and the LB_IP /
+  GLB survivor counts against the oracle's;
+* batching (random walk, m log-uniform in [64, 2048], w = 5% of m, mixed
+  metrics, k = 10) through fsm_query_batch, and serving (m in [64, 1024],
+  mixed modes) through the server, every result against the oracle;
+* radii above 512 (the wide envelope path) with search, cutoff and exact.
+
+A GPU cutoff is checked either against the oracle's cutoff or, where the
+oracle's heap-ordered scan is too slow, against the k (distance,
+index)-smallest windows of the oracle's search just above the GPU's
+threshold: that set holds the true k best whenever the GPU's threshold is at
+or above the true one, and has fewer than k windows, or other ones, when it
+is below, so either way a wrong answer fails."""
+import os
+
+import numpy as np
+import pytest
+
+import fsmeld_amd as F
+import oracle as O
+from fsmeld_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+THREADS = int(os.environ.get("FSMELD_TEST_THREADS", "16"))
+
+
+def bits(x):
+    return np.asarray(x, dtype=np.float64).view(np.uint64)
+
+
+def oracle_dist(t, q, m, metric, w, arg, windows, stats=None):
+    """Exact distances of the given windows by the oracle's verifier, each
+    window normalised with its macro-segment's mean / std."""
+    mu, sg, _ = stats if stats is not None else O.window_stats(t, m)
+    qn = O.normalize_query(q, m)
+    return np.array([O.eld(metric, qn, (t[g:g + m] - mu[g]) / sg[g], w, arg) for g in windows])
+
+
+def oracle_topk(t, q, m, k, metric, w, arg, bound, stats=None):
+    """The k (distance, index)-smallest windows among the oracle search's
+    matches at `bound` (the reference's cutoff order, src/fsmeld.cpp:376-418)."""
+    ids, _ = O.search(t, q, m, bound, metric, w, arg, threads=THREADS)
+    d = oracle_dist(t, q, m, metric, w, arg, ids, stats)
+    order = np.lexsort((ids, d))[:k]
+    return ids[order], d[order]
+
+
+def check_cutoff_by_search(t, q, m, k, metric, w, arg, g, stats=None):
+    bound = g.threshold * (1 + 1e-6) + 1e-12
+    ids, d = oracle_topk(t, q, m, k, metric, w, arg, bound, stats)
+    assert ids.size == k, "fewer than k windows at or below the GPU threshold"
+    assert np.array_equal(g.indices, ids)
+    assert np.array_equal(bits(g.distances), bits(d))
+    assert bits(g.threshold) == bits(d[-1])
+
+
+def check_search(t, q, m, cut, metric, w, arg, s, stats=None):
+    ids, ost = O.search(t, q, m, cut, metric, w, arg, threads=THREADS)
+    assert np.array_equal(s.indices, ids)
+    assert np.array_equal(bits(s.distances), bits(oracle_dist(t, q, m, metric, w, arg, ids,
+                                                                stats)))
+    assert s.stats["valid"] == ost["valid"] and s.stats["invalid"] == ost["invalid"]
+    return ost
+
+
+def check_exact(t, q, m, cut, metric, w, arg, e, search_ids, stats=None):
+    """The exact scan returns the search's windows plus, at most, windows
+    whose distance equals the cutoff (the reference's GLB can drop those by an
+    ulp); every distance equals the oracle verifier's."""
+    assert np.setdiff1d(search_ids, e.indices).size == 0
+    d = oracle_dist(t, q, m, metric, w, arg, e.indices, stats)
+    assert np.array_equal(bits(e.distances), bits(d))
+    extra = ~np.isin(e.indices, search_ids)
+    assert np.all(d[extra] == cut)
+    assert np.all(d <= cut)
+
+
+# ------------------------------------------------------------ plain large ---
+@pytest.fixture(scope="module")
+def plain_large_prefix():
+    c = workloads.plain_large()
+    t = c["target"][: 1 << 21].copy()
+    q = c["query"]
+    del c
+    return t, q, F.Target(t), O.window_stats(t, 512)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("metric,arg", workloads.CONFIG2_METRICS)
+def test_plain_large_prefix(gpu, plain_large_prefix, metric, arg):
+    t, q, T, stats = plain_large_prefix
+    m, w, k = 512, 25, 100
+    thr, ranks, rd, _ = O.cutoff(t, q, m, k, metric, w, arg, threads=THREADS)
+    g = T.cutoff(q, m, k, metric, w, arg)
+    assert bits(g.threshold) == bits(thr)
+    assert np.array_equal(g.indices, ranks)
+    assert np.array_equal(bits(g.distances), bits(rd))
+    s = T.search(q, m, thr, metric, w, arg)
+    ost = check_search(t, q, m, thr, metric, w, arg, s, stats)
+    e = T.exact(q, m, thr, metric, w, arg)
+    check_exact(t, q, m, thr, metric, w, arg, e, s.indices, stats)
+    # the LB_IP and GLB stages prune as the oracle's restatement does (its
+    # thresholds can differ in a few segments where near-ties resolve
+    # differently, which moves counters, never matches)
+    v = ost["valid"]
+    print(f"\n{metric}: valid {v} after_lbip gpu {s.stats['after_lbip']} oracle "
+          f"{ost['after_lbip']}, after_glb gpu {s.stats['after_glb']} oracle {ost['after_glb']}, "
+          f"matches {s.indices.size}")
+    assert abs(int(s.stats["after_lbip"]) - int(ost["after_lbip"])) <= 0.02 * v
+    assert abs(int(s.stats["after_glb"]) - int(ost["after_glb"])) <= 0.02 * v
+
+
+# --------------------------------------------------------------- batching ---
+@pytest.fixture(scope="module")
+def batch_case():
+    t = F.synth_target(4, 1 << 20)
+    qs = workloads.batch_queries(t, 24, seed=7)   # m in [64, 2048], w = 5% of m
+    T = F.Target(t)
+    specs = [dict(mode="cutoff", query=x["query"], m=x["m"], metric=x["metric"], w=x["w"],
+                  arg=x["arg"], k=10) for x in qs]
+    cuts = T.batch(specs, concurrency=4)
+    return t, qs, T, cuts
+
+
+@pytest.mark.timeout(600)
+def test_batch_against_oracle(gpu, batch_case):
+    t, qs, T, cuts = batch_case
+    specs = []
+    for x, c in zip(qs, cuts):
+        assert isinstance(c, F.QueryResult), c
+        specs.append(dict(mode="search", query=x["query"], m=x["m"], metric=x["metric"],
+                          w=x["w"], arg=x["arg"], cutoff=c.threshold))
+    found = T.batch(specs, concurrency=4)
+    stats = {}
+    for x, c, s in zip(qs, cuts, found):
+        m = x["m"]
+        if m not in stats:
+            stats[m] = O.window_stats(t, m)
+        args = (t, x["query"], m)
+        check_cutoff_by_search(*args, 10, x["metric"], x["w"], x["arg"], c, stats[m])
+        check_search(*args, c.threshold, x["metric"], x["w"], x["arg"], s, stats[m])
+
+
+@pytest.mark.timeout(600)
+def test_server_against_oracle(gpu):
+    """Serving shape: spikes and constant runs, m log-uniform in [64, 1024],
+    mixed metrics and modes submitted together to a 4-worker server."""
+    t = F.synth_target(5, 1 << 21, 0.001, 5.0, 30, 300, 0)
+    qs = workloads.batch_queries(t, 15, seed=11, m_lo=64, m_hi=1024)
+    T = F.Target(t)
+    specs = []
+    for i, x in enumerate(qs):
+        mode = ["cutoff", "search", "exact"][i % 3]
+        sp = dict(mode=mode, query=x["query"], m=x["m"], metric=x["metric"], w=x["w"],
+                  arg=x["arg"], k=10)
+        if mode != "cutoff":
+            sp["cutoff"] = T.cutoff(x["query"], x["m"], 10, x["metric"], x["w"],
+                                    x["arg"]).threshold
+        specs.append(sp)
+    with F.Server(T, mem_budget=64 << 30, max_workers=4) as S:
+        tickets = [S.submit(sp) for sp in specs]
+        results = []
+        for tk in tickets:
+            state, r, _ = S.poll(tk, wait_ms=120000)
+            assert state == "done" and isinstance(r, F.QueryResult), (state, r)
+            results.append(r)
+    for sp, r in zip(specs, results):
+        m, metric, w, arg = sp["m"], sp["metric"], sp["w"], sp["arg"]
+        st = O.window_stats(t, m)
+        if sp["mode"] == "cutoff":
+            check_cutoff_by_search(t, sp["query"], m, 10, metric, w, arg, r, st)
+        elif sp["mode"] == "search":
+            check_search(t, sp["query"], m, sp["cutoff"], metric, w, arg, r, st)
+        else:
+            ids, _ = O.search(t, sp["query"], m, sp["cutoff"], metric, w, arg, threads=THREADS)
+            check_exact(t, sp["query"], m, sp["cutoff"], metric, w, arg, r, ids, st)
+
+
+# ------------------------------------------------------------ wide radius ---
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("m,w,n,metrics", [
+    (1100, 513, 6000, [("dtw", 0.0), ("msm", 0.5), ("lcss", 0.25)]),
+    (2048, 1000, 8000, [("dtw", 0.0), ("edr", 0.25)]),
+    (700, 520, 1 << 15, [("erp", 0.0), ("msm", 0.5)]),
+])
+def test_wide_radius_commands(gpu, m, w, n, metrics):
+    """Radii above 512 (target envelope by the middle-run kernel); the last
+    case spans several macro-segments and a tail."""
+    t = F.synth_target(60 + m % 11, n, 0.001, 5.0, 1, 200, 1)
+    q, _ = F.synth_query(61, t, m, 0.1)
+    T = F.Target(t)
+    stats = O.window_stats(t, m)
+    for metric, arg in metrics:
+        c = T.cutoff(q, m, 3, metric, w, arg)
+        check_cutoff_by_search(t, q, m, 3, metric, w, arg, c, stats)
+        s = T.search(q, m, c.threshold, metric, w, arg)
+        check_search(t, q, m, c.threshold, metric, w, arg, s, stats)
+        e = T.exact(q, m, c.threshold, metric, w, arg)
+        check_exact(t, q, m, c.threshold, metric, w, arg, e, s.indices, stats)
+        lb, _, _ = T.lbip(q, m, metric, w, arg)
+        win = np.flatnonzero(~stats[2])[::97]
+        d = oracle_dist(t, q, m, metric, w, arg, win, stats)
+        assert np.all(~(lb[win] > d + 1e-9 * np.maximum(1.0, d)))

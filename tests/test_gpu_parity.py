@@ -356,3 +356,17 @@ def test_eld_matches_reference_golden_vectors(gpu):
         q = np.array(c["q"])
         got = F.eld_pairs(c["metric"], q, np.array(c["t"])[None, :], c["w"], c["arg"])[0]
         assert float(got).hex() == c["d"], c
+
+
+@pytest.mark.parametrize("metric", ["dtw", "erp"])
+def test_cutoff_k1_w0_first_bound_is_widened(gpu, metric):
+    """k = 1, w = 0: the best window's distance is the first bound itself and
+    GLB's reordered sum can exceed it by an ulp; the bound is widened so that
+    window survives (on the oracle, seeds 9 and 24 of DTW lose it otherwise)."""
+    for seed in (0, 3, 9, 17, 24, 31):
+        t = make_target(seed, 1 << 13)
+        q, _ = F.synth_query(seed, t, 40, 0.1)
+        thr, ranks, rd, _ = O.cutoff(t, q, 40, 1, metric, 0, 0.0)
+        g = F.Target(t).cutoff(q, 40, 1, metric, 0, 0.0)
+        assert bits(g.threshold) == bits(thr), seed
+        assert np.array_equal(g.indices, ranks), seed

@@ -132,7 +132,9 @@ def test_server_cancel_and_budget(gpu, target):
     sp = dict(mode="exact", query=F.synth_query(5, t, 128, 0.1)[0], m=128, metric="msm", w=6,
               arg=0.5, cutoff=1e9)
     need = F.query_bytes(len(t), "msm", 128, 6)
-    with F.Server(T, mem_budget=need, max_workers=2) as S:
+    # the workers get three quarters of the budget (the rest is the shared
+    # statistics cache): room for exactly one such query at a time
+    with F.Server(T, mem_budget=need * 4 // 3 + 4, max_workers=2) as S:
         tickets = [S.submit(sp) for _ in range(6)]
         S.cancel(tickets[-1])   # still queued behind the others: dropped
         states = [S.poll(tk, wait_ms=-1)[0] for tk in tickets]
