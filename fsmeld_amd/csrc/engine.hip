@@ -628,6 +628,8 @@ int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
 
 }  // namespace
 
+int fsm::report_error(int code, const char* msg) { return fail(code, msg); }
+
 extern "C" {
 
 int fsm_device_count(void) {

@@ -101,6 +101,9 @@ __device__ inline void block_append(bool keep, int64_t value, double dval, int64
     __syncthreads();
 }
 
+// Records msg as the calling thread's fsm_last_error() and returns code.
+int report_error(int code, const char* msg);
+
 // ------------------------------------------------------------ launchers ---
 hipError_t launch_window_stats(hipStream_t st, const double* T, const Geometry& G, double* MU,
                                double* SG, uint8_t* INV);

@@ -209,6 +209,22 @@ int fsm_stage_eld(fsm_target* t, const fsm_query_desc* desc, const double* query
 int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const double* cands,
                   int64_t n, double* out);
 
+/* ---- the same without device allocation (no reference counterpart) ----
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that forbid hipMalloc): fsm_eld_pairs with q[m],
+ * cands[n][m] and out[n] in DEVICE memory, the kernels enqueued on `stream`
+ * (a hipStream_t; NULL: the null stream) and no synchronisation, so out is
+ * ready when the stream reaches that point.  No hipMalloc, hipFree or other
+ * allocation: the gap costs and the work counter live in a static device
+ * array of 256 + 8 * FSM_NOALLOC_MAX_M bytes per GPU, shared by every call on
+ * that GPU, so calls that overlap on one GPU race.  Limits: m <=
+ * FSM_NOALLOC_MAX_M and (m, w) on the register / wave band kernels (w <= 127,
+ * m <= 4094); FSM_ERR_ARG beyond them.  Values must be finite (they are on
+ * the device and not checked; non-finite ones give unspecified distances). */
+#define FSM_NOALLOC_MAX_M 4096
+int fsm_eld_pairs_noalloc(int metric, int m, int w, double arg, const double* q,
+                          const double* cands, int64_t n, double* out, void* stream);
+
 /* ---- seeded synthetic workloads (host only; no GPU needed) ----
  * SplitMix64-seeded xoshiro256** with Box-Muller normals.
  * Target: random walk of N(0,1) increments; spike_frac of positions get an
