@@ -63,7 +63,8 @@ class _Stats(ctypes.Structure):
                 ("m_k", ctypes.c_int), ("ms_stats", ctypes.c_double),
                 ("ms_lbip", ctypes.c_double), ("ms_glb", ctypes.c_double),
                 ("ms_eld", ctypes.c_double), ("ms_total", ctypes.c_double),
-                ("eld_cells", ctypes.c_uint64)]
+                ("eld_cells", ctypes.c_uint64), ("ms_stats_finish", ctypes.c_double),
+                ("ms_env", ctypes.c_double)]
 
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -103,6 +104,8 @@ _sig("fsm_eld_pairs", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, c
                                      _D, _D, ctypes.c_int64, _D])
 _sig("fsm_target_share", ctypes.c_int, [_VP, ctypes.POINTER(_VP)])
 _sig("fsm_target_trim", None, [_VP])
+_sig("fsm_target_reuse_stats", ctypes.c_int, [_VP, ctypes.c_int])
+_sig("fsm_target_cache_limit", ctypes.c_int, [_VP, ctypes.c_uint64])
 _sig("fsm_query_bytes", ctypes.c_uint64, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int])
 _sig("fsm_query_batch", ctypes.c_int, [_VP, ctypes.c_int64, ctypes.POINTER(_Desc),
                                        ctypes.POINTER(_D), _I64, ctypes.c_int,
@@ -264,6 +267,15 @@ class Target:
         """Frees this handle's working buffers and the target's cached window
         statistics / envelopes that no handle is using (fsm_target_trim)."""
         _lib.fsm_target_trim(self._h)
+
+    def reuse_stats(self, enable: bool):
+        """False: every query recomputes its window statistics and envelope,
+        as the reference does per query (fsm_target_reuse_stats)."""
+        _check(_lib.fsm_target_reuse_stats(self._h, 1 if enable else 0))
+
+    def cache_limit(self, nbytes: int):
+        """Caps the target's shared statistics / envelope cache (bytes)."""
+        _check(_lib.fsm_target_cache_limit(self._h, int(nbytes)))
 
     def share(self) -> "Target":
         """Another handle to the same device samples (own stream and buffers)."""

@@ -284,6 +284,9 @@ struct fsm_target {
     DevBuf lb, list_a, list_b, out_idx, out_dist, slab, rep_lb, rep_idx, tu, tl, qtab,
         scratch, counters, aux;
     hipEvent_t ev[8] = {};
+    // false: every query recomputes its window statistics and envelope into
+    // this handle's own buffers (fsm_target_reuse_stats)
+    bool reuse = true;
     // fsm_query_batch's handles, kept between calls
     std::mutex pool_mu;
     std::vector<fsm_target*> pool;
@@ -394,29 +397,38 @@ int cached(fsm_target* t, std::map<int, std::shared_ptr<CacheEntry>> TargetCache
     return FSM_OK;
 }
 
+// ev[6] marks the end of the statistics' first kernel when they are computed.
 int ensure_stats(fsm_target* t, const Geometry& G, bool* computed) {
     *computed = false;
-    if (t->st && t->stats_m == G.m) return FSM_OK;
+    if (t->reuse && t->st && t->stats_m == G.m) return FSM_OK;
     const size_t n = (size_t)G.S * (size_t)G.kml;
     auto fill = [&](CacheEntry& e) {
         HIP_TRY(e.a.ensure(sizeof(double) * n));
         HIP_TRY(e.b.ensure(sizeof(double) * n));
         HIP_TRY(e.c.ensure(n));
         HIP_TRY(launch_window_stats(t->stream, t->T->as<double>(), G, e.a.as<double>(),
-                                    e.b.as<double>(), e.c.as<uint8_t>()));
+                                    e.b.as<double>(), e.c.as<uint8_t>(), t->ev[6]));
         return FSM_OK;
     };
     t->st.reset();  // before own buffers may be refilled
-    const int rc = cached(t, &TargetCache::stats, &TargetCache::seen_stats,
-                          &TargetCache::req_stats, G.m, 17 * (uint64_t)n, &t->own_st, fill,
-                          &t->st, computed);
-    if (rc) return rc;
+    if (!t->reuse) {
+        const int rc = fill(t->own_st);
+        if (rc) return rc;
+        t->st = std::shared_ptr<CacheEntry>(std::shared_ptr<CacheEntry>(), &t->own_st);
+        *computed = true;
+    } else {
+        const int rc = cached(t, &TargetCache::stats, &TargetCache::seen_stats,
+                              &TargetCache::req_stats, G.m, 17 * (uint64_t)n, &t->own_st, fill,
+                              &t->st, computed);
+        if (rc) return rc;
+    }
     t->stats_m = G.m;
     return FSM_OK;
 }
 
-int ensure_env(fsm_target* t, int w) {
-    if (t->env && t->env_w == w) return FSM_OK;
+int ensure_env(fsm_target* t, int w, bool* computed = nullptr) {
+    if (computed) *computed = false;
+    if (t->reuse && t->env && t->env_w == w) return FSM_OK;
     auto fill = [&](CacheEntry& e) {
         HIP_TRY(e.a.ensure(sizeof(double) * (size_t)t->N));
         HIP_TRY(e.b.ensure(sizeof(double) * (size_t)t->N));
@@ -426,10 +438,19 @@ int ensure_env(fsm_target* t, int w) {
     };
     bool made = false;
     t->env.reset();
-    const int rc = cached(t, &TargetCache::env, &TargetCache::seen_env, &TargetCache::req_env,
-                          w, 16 * (uint64_t)t->N, &t->own_env, fill, &t->env, &made);
-    if (rc) return rc;
+    if (!t->reuse) {
+        const int rc = fill(t->own_env);
+        if (rc) return rc;
+        t->env = std::shared_ptr<CacheEntry>(std::shared_ptr<CacheEntry>(), &t->own_env);
+        made = true;
+    } else {
+        const int rc = cached(t, &TargetCache::env, &TargetCache::seen_env,
+                              &TargetCache::req_env, w, 16 * (uint64_t)t->N, &t->own_env, fill,
+                              &t->env, &made);
+        if (rc) return rc;
+    }
     t->env_w = w;
+    if (computed) *computed = made;
     return FSM_OK;
 }
 
@@ -763,7 +784,7 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
     unsigned long long C[16] = {};
 
     HIP_TRY(hipEventRecord(t->ev[0], t->stream));
-    bool stats_new = false;
+    bool stats_new = false, env_new = false;
     rc = ensure_stats(t, G, &stats_new);
     if (rc) return bail(rc);
     HIP_TRY(hipEventRecord(t->ev[1], t->stream));
@@ -780,8 +801,9 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
         st.after_glb = C[0];
         st.exact_evals = C[0];
     } else {
-        rc = ensure_env(t, d->w);
+        rc = ensure_env(t, d->w, &env_new);
         if (rc) return bail(rc);
+        HIP_TRY(hipEventRecord(t->ev[7], t->stream));
         rc = run_lbip(t, G, Q, P, d->mode == FSM_MODE_CUTOFF);
         if (rc) return bail(rc);
         HIP_TRY(hipEventRecord(t->ev[2], t->stream));
@@ -896,7 +918,9 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
     st.matches = R->idx.size();
     st.eld_cells = (uint64_t)C[9] * (uint64_t)(2 * d->w + 1);
     st.ms_stats = stats_new ? ms_between(t->ev[0], t->ev[1]) : 0.0;
-    st.ms_lbip = d->mode == FSM_MODE_EXACT ? 0.0 : ms_between(t->ev[1], t->ev[2]);
+    st.ms_stats_finish = stats_new ? ms_between(t->ev[6], t->ev[1]) : 0.0;
+    st.ms_env = env_new ? ms_between(t->ev[1], t->ev[7]) : 0.0;
+    st.ms_lbip = d->mode == FSM_MODE_EXACT ? 0.0 : ms_between(t->ev[7], t->ev[2]);
     if (d->mode == FSM_MODE_EXACT) {
         st.ms_glb = 0.0;
         st.ms_eld = ms_between(t->ev[2], t->ev[5]);
@@ -1150,6 +1174,14 @@ uint64_t fsm_query_bytes(int64_t n, int metric, int m, int w) {
     if (eld_uses_scratch(metric, m, w))
         b += sizeof(double) * 2 * (uint64_t)(2 * w + 2) * (uint64_t)eld_scratch_threads();
     return b;
+}
+
+int fsm_target_reuse_stats(fsm_target* t, int enable) {
+    if (!t) return fail(FSM_ERR_ARG, "null target");
+    t->reuse = enable != 0;
+    t->stats_m = -1;
+    t->env_w = -1;
+    return FSM_OK;
 }
 
 int fsm_target_cache_limit(fsm_target* t, uint64_t bytes) {

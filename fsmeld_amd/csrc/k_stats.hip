@@ -124,10 +124,14 @@ __global__ void __launch_bounds__(256) k_window_finish(int64_t n, int m, double*
 }
 
 hipError_t launch_window_stats(hipStream_t st, const double* T, const Geometry& G, double* MU,
-                               double* SG, uint8_t* INV) {
+                               double* SG, uint8_t* INV, hipEvent_t between) {
     const int bs = 64;
     const int64_t blocks = (G.S + bs - 1) / bs;
     hipLaunchKernelGGL(k_window_sums, dim3((unsigned)blocks), dim3(bs), 0, st, T, G, MU, SG);
+    if (between) {
+        const hipError_t e = hipEventRecord(between, st);
+        if (e != hipSuccess) return e;
+    }
     const int64_t n = G.S * G.kml;
     const int64_t fb = std::min<int64_t>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(k_window_finish, dim3((unsigned)fb), dim3(256), 0, st, n, G.m, MU, SG, INV);

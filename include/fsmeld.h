@@ -75,6 +75,10 @@ typedef struct {
     /* band cells the ELD kernel evaluated for live candidates (iterations of
      * the register-band kernel x (2w + 1)); 0 when the row-band kernel ran */
     uint64_t eld_cells;
+    /* of ms_stats: the second statistics kernel (mean / std / invalid flag
+     * from the running sums); the target envelope, when this query computed
+     * it (ms_lbip no longer includes it) */
+    double ms_stats_finish, ms_env;
 } fsm_stats;
 
 /* ---- device and errors ---- */
@@ -129,6 +133,12 @@ uint64_t fsm_query_bytes(int64_t n, int metric, int m, int w);
  * a target (fsm_target_share) once a length or radius comes back within 64
  * requests, up to half of device memory, least recently used first out. */
 void fsm_target_trim(fsm_target* t);
+
+/* enable = 0: every query on this handle recomputes its window statistics
+ * and target envelope into the handle's own buffers, as the reference
+ * computes them for every query (src/fsmeld.cpp:750-773); enable = 1 (the
+ * default): they are reused from an earlier query of the same m / w. */
+int fsm_target_reuse_stats(fsm_target* t, int enable);
 
 /* Caps the device bytes of that shared cache (default: half of device
  * memory); a new length or radius that would exceed the cap stays in the
