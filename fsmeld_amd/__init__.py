@@ -28,7 +28,7 @@ __all__ = [
 ]
 
 METRICS = {"dtw": 1, "edr": 2, "erp": 3, "lcss": 4, "msm": 5}
-MODES = {"search": 0, "cutoff": 1, "exact": 2}
+MODES = {"search": 0, "cutoff": 1, "exact": 2, "exact_topk": 3}
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 lib_path = os.path.join(_HERE, "lib", "libfsmeld_hip.so")
@@ -291,6 +291,11 @@ class Target:
 
     def cutoff(self, query, m, k, metric, w, arg, exclude=None) -> QueryResult:
         return self.query("cutoff", query, m, metric, w, arg, k=k, exclude=exclude)
+
+    def exact_topk(self, query, m, k, metric, w, arg, exclude=None) -> QueryResult:
+        """FSMELD_exact with top_k = k: every window at or below the k-th
+        smallest exact distance (stats["threshold"]), ascending, by brute force."""
+        return self.query("exact_topk", query, m, metric, w, arg, 0.0, k, exclude)
 
     def exact(self, query, m, cutoff_value, metric, w, arg, exclude=None) -> QueryResult:
         return self.query("exact", query, m, metric, w, arg, cutoff_value=cutoff_value,

@@ -163,7 +163,7 @@ void prepare_query(const double* q_raw, int m, int w, double arg, HostQuery* H) 
 // Argument checks of src/main.cpp:46-99 that concern the engine.
 int validate(const fsm_query_desc* d, int64_t N, const double* q, int64_t qlen, bool need_mode) {
     if (!d) return fail(FSM_ERR_ARG, "null query descriptor");
-    if (need_mode && (d->mode < FSM_MODE_SEARCH || d->mode > FSM_MODE_EXACT))
+    if (need_mode && (d->mode < FSM_MODE_SEARCH || d->mode > FSM_MODE_EXACT_TOPK))
         return fail(FSM_ERR_ARG, "Unknown command");
     if (d->m < 2 || d->m > (1 << 27))
         return fail(FSM_ERR_ARG, "m must be at least 2 and fit the FFT indexing range.");
@@ -192,7 +192,7 @@ int validate(const fsm_query_desc* d, int64_t N, const double* q, int64_t qlen, 
     const bool excl = !(d->exclude_begin == -1 && d->exclude_end == -1);
     if (excl && (d->exclude_begin < 0 || d->exclude_end <= d->exclude_begin))
         return fail(FSM_ERR_ARG, "Exclusion requires 0 <= start < end.");
-    if (need_mode && d->mode == FSM_MODE_CUTOFF) {
+    if (need_mode && (d->mode == FSM_MODE_CUTOFF || d->mode == FSM_MODE_EXACT_TOPK)) {
         if (d->k < 1 || (int64_t)d->k > N - d->m + 1)
             return fail(FSM_ERR_ARG, "k exceeds the candidate count or supported integer range.");
     } else if (need_mode) {
@@ -202,9 +202,15 @@ int validate(const fsm_query_desc* d, int64_t N, const double* q, int64_t qlen, 
     return FSM_OK;
 }
 
+// Elapsed time between two recorded events.  A failure (an event this query
+// did not record) would stay behind as the thread's last HIP error and be
+// returned by a later launch's hipGetLastError, so it is cleared here.
 double ms_between(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0;
+    }
     return (double)ms;
 }
 
@@ -282,7 +288,7 @@ struct fsm_target {
     std::shared_ptr<CacheEntry> env;  // Lr, Ur of env_w
     CacheEntry own_st, own_env;       // this handle's buffers for first requests
     DevBuf lb, list_a, list_b, out_idx, out_dist, slab, rep_lb, rep_idx, tu, tl, qtab,
-        scratch, counters, aux;
+        scratch, counters, aux, hist;
     hipEvent_t ev[8] = {};
     // false: every query recomputes its window statistics and envelope into
     // this handle's own buffers (fsm_target_reuse_stats)
@@ -647,6 +653,39 @@ int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
     return FSM_OK;
 }
 
+// Exact top-k (src/fsmeld.cpp:1097-1107, :1273-1290): every valid window's
+// exact distance (aux, in the order of the valid list in list_a), the k-th
+// smallest finite one as *kth, and the windows at or below it appended to
+// out_idx / out_dist with their count in counters[3].  *n_finite < k means
+// fewer than k finite distances (the reference then reports none).
+int run_exact_topk(fsm_target* t, const Geometry& G, const QueryTables& Q, const QueryParams& P,
+                   int k, double* kth, int64_t* n_finite) {
+    unsigned long long* C = t->counters.as<unsigned long long>();
+    HIP_TRY(hipMemsetAsync(C, 0, sizeof(unsigned long long) * 16, t->stream));
+    HIP_TRY(t->list_a.ensure(sizeof(int64_t) * (size_t)G.ts_n));
+    HIP_TRY(t->aux.ensure(sizeof(double) * (size_t)G.ts_n));
+    HIP_TRY(t->out_idx.ensure(sizeof(int64_t) * (size_t)G.ts_n));
+    HIP_TRY(t->out_dist.ensure(sizeof(double) * (size_t)G.ts_n));
+    HIP_TRY(t->hist.ensure(sizeof(unsigned long long) * 2048));
+    HIP_TRY(launch_compact(t->stream, G, P, t->st->c.as<uint8_t>(), nullptr, false,
+                           t->list_a.as<int64_t>(), C));
+    HIP_TRY(hipEventRecord(t->ev[3], t->stream));
+    int rc = ensure_eld_scratch(t, P.metric, P.m, P.w);
+    if (rc) return rc;
+    VerifyArgs v = verify_args(t, G, Q, P);
+    HIP_TRY(launch_eld(t->stream, v, t->list_a.as<int64_t>(), nullptr, C, true, nullptr,
+                       t->aux.as<double>(), nullptr, C + 8, nullptr, t->scratch.as<double>(),
+                       eld_scratch_threads()));
+    HIP_TRY(hipEventRecord(t->ev[4], t->stream));
+    HIP_TRY(kth_smallest(t->stream, t->aux.as<double>(), C, k, t->hist.as<unsigned long long>(),
+                         kth, n_finite));
+    if (*n_finite < k) return FSM_OK;
+    HIP_TRY(launch_select_le_kv(t->stream, t->list_a.as<int64_t>(), t->aux.as<double>(), C, *kth,
+                                t->out_idx.as<int64_t>(), t->out_dist.as<double>(), C + 3));
+    HIP_TRY(hipEventRecord(t->ev[5], t->stream));
+    return FSM_OK;
+}
+
 }  // namespace
 
 int fsm::report_error(int code, const char* msg) { return fail(code, msg); }
@@ -748,7 +787,7 @@ void fsm_target_close(fsm_target* t) {
     DevBuf* bufs[] = {&t->lb,
                       &t->list_a, &t->list_b, &t->out_idx, &t->out_dist, &t->slab,
                       &t->rep_lb, &t->rep_idx, &t->tu, &t->tl, &t->qtab, &t->scratch,
-                      &t->counters, &t->aux};
+                      &t->counters, &t->aux, &t->hist};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : t->ev)
         if (e) (void)hipEventDestroy(e);
@@ -789,7 +828,23 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
     if (rc) return bail(rc);
     HIP_TRY(hipEventRecord(t->ev[1], t->stream));
 
-    if (d->mode == FSM_MODE_EXACT) {
+    double topk_kth = 0.0;
+    if (d->mode == FSM_MODE_EXACT_TOPK) {
+        HIP_TRY(hipEventRecord(t->ev[2], t->stream));
+        int64_t nfin = 0;
+        rc = run_exact_topk(t, G, Q, P, d->k, &topk_kth, &nfin);
+        if (rc) return bail(rc);
+        if (nfin < d->k)
+            return bail(fail(FSM_ERR_FEWER_THAN_K,
+                             "Fewer finite exact distances than requested k."));
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        HIP_TRY(hipMemcpy(C, t->counters.p, sizeof(C), hipMemcpyDeviceToHost));
+        st.valid = C[1];
+        st.invalid = (uint64_t)G.ts_n - C[1];
+        st.after_lbip = C[0];
+        st.after_glb = C[0];
+        st.exact_evals = C[0];
+    } else if (d->mode == FSM_MODE_EXACT) {
         HIP_TRY(hipEventRecord(t->ev[2], t->stream));
         rc = run_filter_verify(t, G, Q, P, false, false);
         if (rc) return bail(rc);
@@ -914,14 +969,20 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
             return bail(fail(FSM_ERR_FEWER_THAN_K,
                              "Fewer valid, non-excluded candidates than requested k."));
         st.threshold = R->dist.back();
+    } else if (d->mode == FSM_MODE_EXACT_TOPK) {
+        st.threshold = topk_kth;
     }
     st.matches = R->idx.size();
     st.eld_cells = (uint64_t)C[9] * (uint64_t)(2 * d->w + 1);
     st.ms_stats = stats_new ? ms_between(t->ev[0], t->ev[1]) : 0.0;
     st.ms_stats_finish = stats_new ? ms_between(t->ev[6], t->ev[1]) : 0.0;
     st.ms_env = env_new ? ms_between(t->ev[1], t->ev[7]) : 0.0;
-    st.ms_lbip = d->mode == FSM_MODE_EXACT ? 0.0 : ms_between(t->ev[7], t->ev[2]);
-    if (d->mode == FSM_MODE_EXACT) {
+    const bool cascade = d->mode == FSM_MODE_SEARCH || d->mode == FSM_MODE_CUTOFF;
+    st.ms_lbip = cascade ? ms_between(t->ev[7], t->ev[2]) : 0.0;
+    if (d->mode == FSM_MODE_EXACT_TOPK) {
+        st.ms_glb = 0.0;
+        st.ms_eld = ms_between(t->ev[3], t->ev[4]);
+    } else if (d->mode == FSM_MODE_EXACT) {
         st.ms_glb = 0.0;
         st.ms_eld = ms_between(t->ev[2], t->ev[5]);
     } else {
@@ -1200,7 +1261,8 @@ void fsm_target_trim(fsm_target* t) {
     if (!t) return;
     (void)hipStreamSynchronize(t->stream);
     DevBuf* bufs[] = {&t->lb, &t->list_a, &t->list_b, &t->out_idx, &t->out_dist, &t->slab,
-                      &t->rep_lb, &t->rep_idx, &t->tu, &t->tl, &t->qtab, &t->scratch, &t->aux};
+                      &t->rep_lb, &t->rep_idx, &t->tu, &t->tl, &t->qtab, &t->scratch, &t->aux,
+                      &t->hist};
     for (DevBuf* b : bufs) b->release();
     t->st.reset();
     t->env.reset();

@@ -150,6 +150,18 @@ hipError_t launch_select_le(hipStream_t st, const int64_t* list, const double* k
                             const unsigned long long* count, double sign, double tau,
                             int64_t* out, unsigned long long* out_count);
 
+// The k-th smallest finite value of keys[0 .. *count) (radix select on the
+// device, one host round trip per digit; hist: 2048 device counters).
+// *n_finite receives the number of finite keys; *kth is left alone when
+// that is below k.
+hipError_t kth_smallest(hipStream_t st, const double* keys, const unsigned long long* count,
+                        int64_t k, unsigned long long* hist, double* kth, int64_t* n_finite);
+// (list[i], keys[i]) for i < *count with keys[i] <= tau, appended to out /
+// out_keys (unordered).
+hipError_t launch_select_le_kv(hipStream_t st, const int64_t* list, const double* keys,
+                               const unsigned long long* count, double tau, int64_t* out,
+                               double* out_keys, unsigned long long* out_count);
+
 struct VerifyArgs {
     const double* T;
     const double* MU;

@@ -43,6 +43,11 @@ extern "C" {
 #define FSM_MODE_SEARCH 0 /* FSMELD()                 include/funcs.h:67-68 */
 #define FSM_MODE_CUTOFF 1 /* FSMELD_find_threshold()  include/funcs.h:64-65 */
 #define FSM_MODE_EXACT 2  /* FSMELD_exact(top_k = 0)  include/funcs.h:70-72 */
+/* FSMELD_exact(top_k = k), src/fsmeld.cpp:1097-1107, :1273-1290: the exact
+ * distance of every valid window (no filter), the k-th smallest of them as
+ * the threshold (stats.threshold), and every window at or below it (ties
+ * included), ascending by index */
+#define FSM_MODE_EXACT_TOPK 3
 
 typedef struct fsm_target fsm_target;
 typedef struct fsm_result fsm_result;
@@ -54,7 +59,7 @@ typedef struct {
     int w;                 /* Sakoe-Chiba radius, 0 <= w < m */
     double arg;            /* ERP gap g, EDR/LCSS epsilon, MSM cost c; ignored by DTW */
     double cutoff;         /* search / exact: distance cutoff xi >= 0 */
-    int k;                 /* cutoff: rank k >= 1 */
+    int k;                 /* cutoff / exact top-k: rank k >= 1 */
     int64_t exclude_begin; /* candidate exclusion [begin, end); -1 = none */
     int64_t exclude_end;
 } fsm_query_desc;

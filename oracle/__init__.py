@@ -72,6 +72,8 @@ _sig("orc_exact", _I, [_D, _L, _D, _I, ctypes.c_double, _I, _I, ctypes.c_double,
                        ctypes.POINTER(Stats), _I64, _D, _L, _I64])
 _sig("orc_cutoff", _I, [_D, _L, _D, _I, _I, _I, _I, ctypes.c_double, _L, _L, _I,
                         ctypes.POINTER(Stats), _I64, _D])
+_sig("orc_exact_topk", _I, [_D, _L, _D, _I, _I, _I, _I, ctypes.c_double, _L, _L, _I,
+                            ctypes.POINTER(Stats), _I64, _D, _L, _I64, _D])
 _sig("orc_window_stats", _I, [_D, _L, _I, _L, _L, _D, _D, _U8])
 _sig("orc_lbip_all", _I, [_D, _L, _D, _I, _I, _I, ctypes.c_double, _D, _D, _D])
 
@@ -243,6 +245,27 @@ def cutoff(T, q_raw, m, k, metric, w, arg, exclude=None, threads=1):
     if rc:
         raise ValueError(f"orc_cutoff rc={rc}")
     return st.threshold, ranks, rd, _stats(st)
+
+
+def exact_topk(T, q_raw, m, k, metric, w, arg, exclude=None, threads=1):
+    """(threshold, indices, distances, stats) of FSMELD_exact with top_k = k;
+    raises LookupError below k finite distances."""
+    T, q = _a(T), _a(q_raw)
+    st = Stats()
+    xb, xe = _excl(exclude)
+    cap = 1 << 22
+    out = np.empty(cap, np.int64)
+    dist = np.empty(cap)
+    n = ctypes.c_int64()
+    kth = ctypes.c_double()
+    rc = _lib.orc_exact_topk(_p(T), T.size, _p(q), m, k, _metric(metric), w, float(arg), xb, xe,
+                             threads, ctypes.byref(st), _p(out, _I64), _p(dist), cap,
+                             ctypes.byref(n), ctypes.byref(kth))
+    if rc == -4:
+        raise LookupError("fewer finite exact distances than k")
+    if rc:
+        raise ValueError(f"orc_exact_topk rc={rc}")
+    return kth.value, out[: n.value].copy(), dist[: n.value].copy(), _stats(st)
 
 
 def window_stats(T, m, exclude=None):

@@ -1393,6 +1393,47 @@ int orc_cutoff(const double* T, int64_t N, const double* q_raw, int m, int k, in
     return 0;
 }
 
+// src/fsmeld.cpp:1097-1107, :1273-1290 (FSMELD_exact with top_k = k): the
+// k smallest finite distances in a max-heap of values, the threshold its top,
+// and every window whose distance is at or below it, ascending by index.
+int orc_exact_topk(const double* T, int64_t N, const double* q_raw, int m, int k, int metric,
+                   int w, double arg, int64_t xb, int64_t xe, int threads, orc_stats* st,
+                   int64_t* out, double* dists, int64_t cap, int64_t* n_out, double* kth) {
+    if (k < 1) return -1;
+    const int64_t ts_n = N - m + 1;
+    if (ts_n < 1) return -1;
+    std::vector<int64_t> idx((size_t)ts_n);
+    std::vector<double> d((size_t)ts_n);
+    int64_t n = 0;
+    const int rc = orc_exact(T, N, q_raw, m, std::numeric_limits<double>::infinity(), metric, w,
+                             arg, xb, xe, threads, st, idx.data(), d.data(), ts_n, &n);
+    if (rc) return rc;
+    std::priority_queue<double> heap;
+    for (int64_t i = 0; i < n; i++) {  // every finite distance, in index order
+        if (heap.size() < static_cast<size_t>(k)) heap.push(d[i]);
+        else if (d[i] < heap.top()) {
+            heap.pop();
+            heap.push(d[i]);
+        }
+    }
+    if (heap.size() < static_cast<size_t>(k)) return -4;
+    const double theta = heap.top();
+    int64_t cnt = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (d[i] > theta) continue;
+        if (cnt < cap) {
+            out[cnt] = idx[i];
+            if (dists) dists[cnt] = d[i];
+        }
+        cnt++;
+    }
+    st->threshold = theta;
+    st->matches = static_cast<uint64_t>(cnt);
+    *n_out = cnt;
+    *kth = theta;
+    return 0;
+}
+
 int orc_window_stats(const double* T, int64_t N, int m, int64_t xb, int64_t xe, double* mu,
                      double* sigma, uint8_t* invalid) {
     if (m < 2 || N < 2LL * m - 1) return -1;

@@ -74,6 +74,13 @@ int orc_cutoff(const double* T, int64_t N, const double* q_raw, int m, int k,
                int metric, int w, double arg, int64_t excl_b, int64_t excl_e,
                int threads, orc_stats* st, int64_t* ranks, double* rank_dists);
 
+/* FSMELD_exact with top_k = k: every window whose exact distance is at or
+ * below the k-th smallest finite one (*kth), ascending; -4 below k. */
+int orc_exact_topk(const double* T, int64_t N, const double* q_raw, int m, int k, int metric,
+                   int w, double arg, int64_t excl_b, int64_t excl_e, int threads,
+                   orc_stats* st, int64_t* matches, double* dists, int64_t cap,
+                   int64_t* n_out, double* kth);
+
 /* Per-window stage outputs in global window order (ts_n entries), for
  * stage-level parity: mu/sigma/invalid of the owning segment, the LB_IP
  * value (NaN for invalid windows) and the per-segment thresholds. */

@@ -370,3 +370,24 @@ def test_cutoff_k1_w0_first_bound_is_widened(gpu, metric):
         g = F.Target(t).cutoff(q, 40, 1, metric, 0, 0.0)
         assert bits(g.threshold) == bits(thr), seed
         assert np.array_equal(g.indices, ranks), seed
+
+
+@pytest.mark.parametrize("metric,arg", METRICS)
+def test_exact_topk_matches_oracle(gpu, metric, arg):
+    """FSMELD_exact with top_k = k by brute force on the GPU (every valid
+    window's distance, a device radix select of the k-th): threshold bits,
+    the windows at or below it (ties included) and their distance bits equal
+    the oracle's heap restatement; also with an exclusion and a wide band."""
+    t = make_target(81, 1 << 14)
+    T = F.Target(t)
+    for m, w, k, excl in [(48, 3, 9, None), (64, 40, 4, (2000, 6000)), (32, 0, 25, None)]:
+        q, _ = F.synth_query(81 + m, t, m, 0.2)
+        thr, ids, d, _ = O.exact_topk(t, q, m, k, metric, w, arg, exclude=excl, threads=8)
+        g = T.exact_topk(q, m, k, metric, w, arg, exclude=excl)
+        assert bits(g.threshold) == bits(thr), (m, w)
+        assert np.array_equal(g.indices, ids), (m, w)
+        assert same_bits(g.distances, d), (m, w)
+        assert g.indices.size >= k
+    with pytest.raises(F.FsmeldError) as e:
+        F.Target(t[:200]).exact_topk(q[:32], 32, 170, metric, 3, arg)   # 169 windows
+    assert e.value.code in (-1, -4)

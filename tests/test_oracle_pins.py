@@ -250,3 +250,28 @@ def test_parallel_cutoff_equals_sequential(metric, arg):
     thr, ranks, rd, _ = O.cutoff(t, q, 48, 9, metric, 3, arg)
     thr2, ranks2, rd2, _ = O.cutoff(t, q, 48, 9, metric, 3, arg, threads=4)
     assert thr == thr2 and np.array_equal(ranks, ranks2) and np.array_equal(rd, rd2)
+
+
+# ------------------------------------------------------------ exact top-k ---
+@pytest.mark.parametrize("metric,arg", [("dtw", 0.0), ("edr", 0.25), ("lcss", 0.25),
+                                        ("msm", 0.5)])
+def test_exact_topk_is_the_kth_distance_with_ties(metric, arg):
+    """FSMELD_exact with top_k = k (src/fsmeld.cpp:1097-1107, :1273-1290): the
+    threshold is the k-th smallest of all exact distances (a plain sort of
+    every distance the brute-force scan finds) and the result is every window
+    at or below it -- more than k when distances tie (EDR / LCSS counts)."""
+    import fsmeld_amd as F
+    t = F.synth_target(23, 3000, 0.001, 5.0, 2, 150, 1)
+    q, _ = F.synth_query(23, t, 32, 0.3)
+    k = 7
+    thr, ids, d, _ = O.exact_topk(t, q, 32, k, metric, 3, arg)
+    all_ids, all_d, _ = O.exact(t, q, 32, 1e300, metric, 3, arg)
+    assert thr == np.sort(all_d)[k - 1]
+    assert np.array_equal(ids, all_ids[all_d <= thr]) and np.array_equal(d, all_d[all_d <= thr])
+    assert ids.size >= k
+    cthr, ranks, _, _ = O.cutoff(t, q, 32, k, metric, 3, arg)
+    assert cthr == thr and np.all(np.isin(ranks, ids))
+    if metric in ("edr", "lcss"):
+        assert ids.size > k   # integer-valued distances tie at the threshold
+    with pytest.raises(LookupError):
+        O.exact_topk(t[:80], q, 32, 60, metric, 3, arg)
