@@ -51,7 +51,7 @@ class _Desc(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("metric", ctypes.c_int), ("m", ctypes.c_int),
                 ("w", ctypes.c_int), ("arg", ctypes.c_double), ("cutoff", ctypes.c_double),
                 ("k", ctypes.c_int), ("exclude_begin", ctypes.c_int64),
-                ("exclude_end", ctypes.c_int64)]
+                ("exclude_end", ctypes.c_int64), ("raw", ctypes.c_int)]
 
 
 class _Stats(ctypes.Structure):
@@ -176,9 +176,10 @@ def _take_result(r) -> QueryResult:
 
 
 def _spec_desc(spec) -> _Desc:
-    """Descriptor of a query spec: dict(mode, m, metric, w, arg, cutoff | k, exclude)."""
+    """Descriptor of a query spec: dict(mode, m, metric, w, arg, cutoff | k, exclude, raw)."""
     return _desc(spec["mode"], spec["metric"], spec["m"], spec["w"], spec.get("arg", 0.0),
-                 spec.get("cutoff", 0.0), spec.get("k", 0), spec.get("exclude"))
+                 spec.get("cutoff", 0.0), spec.get("k", 0), spec.get("exclude"),
+                 spec.get("raw", False))
 
 
 def query_bytes(n, metric, m, w) -> int:
@@ -187,8 +188,9 @@ def query_bytes(n, metric, m, w) -> int:
                                     else int(metric), int(m), int(w)))
 
 
-def _desc(mode, metric, m, w, arg, cutoff_value=0.0, k=0, exclude=None) -> _Desc:
+def _desc(mode, metric, m, w, arg, cutoff_value=0.0, k=0, exclude=None, raw=False) -> _Desc:
     d = _Desc()
+    d.raw = 1 if raw else 0
     d.mode = MODES[mode]
     d.metric = METRICS[metric] if isinstance(metric, str) else int(metric)
     d.m, d.w, d.arg = int(m), int(w), float(arg)
@@ -235,9 +237,11 @@ class Target:
 
     # ---- query-level ----
     def query(self, mode, query, m, metric, w, arg, cutoff_value=0.0, k=0,
-              exclude=None) -> QueryResult:
+              exclude=None, raw=False) -> QueryResult:
+        """raw=True: raw (non-normalised) matching, the reference's
+        NormalizationMode::Raw (paper App. D)."""
         q = _f64(query)
-        d = _desc(mode, metric, m, w, arg, cutoff_value, k, exclude)
+        d = _desc(mode, metric, m, w, arg, cutoff_value, k, exclude, raw)
         r = _VP()
         _check(_lib.fsm_query(self._h, ctypes.byref(d), _ptr(q), int(q.size), ctypes.byref(r)))
         return _take_result(r)
@@ -285,21 +289,23 @@ class Target:
         _check(_lib.fsm_target_share(self._h, ctypes.byref(t._h)))
         return t
 
-    def search(self, query, m, cutoff_value, metric, w, arg, exclude=None) -> QueryResult:
+    def search(self, query, m, cutoff_value, metric, w, arg, exclude=None,
+               raw=False) -> QueryResult:
         return self.query("search", query, m, metric, w, arg, cutoff_value=cutoff_value,
-                          exclude=exclude)
+                          exclude=exclude, raw=raw)
 
-    def cutoff(self, query, m, k, metric, w, arg, exclude=None) -> QueryResult:
-        return self.query("cutoff", query, m, metric, w, arg, k=k, exclude=exclude)
+    def cutoff(self, query, m, k, metric, w, arg, exclude=None, raw=False) -> QueryResult:
+        return self.query("cutoff", query, m, metric, w, arg, k=k, exclude=exclude, raw=raw)
 
-    def exact_topk(self, query, m, k, metric, w, arg, exclude=None) -> QueryResult:
+    def exact_topk(self, query, m, k, metric, w, arg, exclude=None, raw=False) -> QueryResult:
         """FSMELD_exact with top_k = k: every window at or below the k-th
         smallest exact distance (stats["threshold"]), ascending, by brute force."""
-        return self.query("exact_topk", query, m, metric, w, arg, 0.0, k, exclude)
+        return self.query("exact_topk", query, m, metric, w, arg, 0.0, k, exclude, raw)
 
-    def exact(self, query, m, cutoff_value, metric, w, arg, exclude=None) -> QueryResult:
+    def exact(self, query, m, cutoff_value, metric, w, arg, exclude=None,
+              raw=False) -> QueryResult:
         return self.query("exact", query, m, metric, w, arg, cutoff_value=cutoff_value,
-                          exclude=exclude)
+                          exclude=exclude, raw=raw)
 
     # ---- stage-level (parity tests) ----
     def window_stats(self, m, exclude=None):
@@ -311,9 +317,9 @@ class Target:
                                            _ptr(sg), _ptr(inv, _U8)))
         return mu, sg, inv.astype(bool)
 
-    def lbip(self, query, m, metric, w, arg):
+    def lbip(self, query, m, metric, w, arg, raw=False):
         q = _f64(query)
-        d = _desc("search", metric, m, w, arg)
+        d = _desc("search", metric, m, w, arg, raw=raw)
         n = len(self) - m + 1
         lb = np.empty(n)
         seg = _geometry(len(self), m)["S"]

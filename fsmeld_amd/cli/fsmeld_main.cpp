@@ -65,6 +65,14 @@ int metric_code(const std::string& s) {
     return 0;
 }
 
+// FSMELD_NORMALIZATION=raw selects raw matching (the reference's
+// NormalizationMode::Raw, include/funcs.h:20-28, which its CLI never sets);
+// anything else, or unset, z-normalised matching
+bool raw_matching() {
+    const char* e = std::getenv("FSMELD_NORMALIZATION");
+    return e && std::string(e) == "raw";
+}
+
 bool quiet_matches() {  // src/fsmeld.cpp:998-1001
     const char* e = std::getenv("FSMELD_QUIET_MATCHES");
     return e && std::string(e) != "0" && std::string(e) != "false";
@@ -139,7 +147,7 @@ int main(int argc, char** argv) {
             }
             const double mean = sum / m;
             const double sigma = std::sqrt(squares / m - mean * mean);
-            if (!std::isfinite(sigma) || sigma <= 1e-12)
+            if (!raw_matching() && (!std::isfinite(sigma) || sigma <= 1e-12))
                 throw std::invalid_argument(
                     "Query cannot be z-normalized: non-finite or near-zero standard deviation.");
         }
@@ -157,6 +165,7 @@ int main(int argc, char** argv) {
         d.arg = arg;
         d.exclude_begin = begin;
         d.exclude_end = end;
+        d.raw = raw_matching() ? 1 : 0;
         if (command == "cutoff") {
             const auto k = parse_arg<long long>(argv[5], "k");
             if (k < 1 || k > std::numeric_limits<int>::max() ||
@@ -203,6 +212,7 @@ int main(int argc, char** argv) {
         std::printf("forsize=%lld\n", (long long)s.forsize);
         ExperimentRow row;
         row.dataset = dataset_name_of(argv[2]);
+        row.normalization = d.raw ? "raw" : "zscore";
         row.metric = metric;
         row.m = m;
         row.w = w;

@@ -123,17 +123,23 @@ struct HostQuery {
 };
 
 // src/fsmeld.cpp:622-643 (normalisation and sorted copy), :721 (envelope).
-void prepare_query(const double* q_raw, int m, int w, double arg, HostQuery* H) {
-    double ex = 0.0, ex2 = 0.0;
-    for (int i = 0; i < m; i++) {
-        ex += q_raw[i];
-        ex2 += q_raw[i] * q_raw[i];
-    }
-    const double mean = ex / m;
-    double sd = ex2 / m;
-    sd = std::sqrt(sd - mean * mean);
+// Raw matching (paper App. D) takes the query as it is.
+void prepare_query(const double* q_raw, int m, int w, double arg, HostQuery* H,
+                   bool raw = false) {
     H->qn.resize(m);
-    for (int i = 0; i < m; i++) H->qn[i] = (q_raw[i] - mean) / sd;
+    if (raw) {
+        for (int i = 0; i < m; i++) H->qn[i] = q_raw[i];
+    } else {
+        double ex = 0.0, ex2 = 0.0;
+        for (int i = 0; i < m; i++) {
+            ex += q_raw[i];
+            ex2 += q_raw[i] * q_raw[i];
+        }
+        const double mean = ex / m;
+        double sd = ex2 / m;
+        sd = std::sqrt(sd - mean * mean);
+        for (int i = 0; i < m; i++) H->qn[i] = (q_raw[i] - mean) / sd;
+    }
     host_envelope(H->qn, w, H->LQ, H->UQ);
     H->XG.resize(m);
     for (int i = 0; i < m; i++) H->XG[i] = (H->qn[i] - arg) * (H->qn[i] - arg);  // dist(x, g)
@@ -186,7 +192,8 @@ int validate(const fsm_query_desc* d, int64_t N, const double* q, int64_t qlen, 
     }
     const double mean = sum / d->m;
     const double sigma = std::sqrt(sq / d->m - mean * mean);
-    if (!std::isfinite(sigma) || sigma <= 1e-12)
+    if (d->raw != 0 && d->raw != 1) return fail(FSM_ERR_ARG, "Unknown normalization mode");
+    if (!d->raw && (!std::isfinite(sigma) || sigma <= 1e-12))
         return fail(FSM_ERR_ARG,
                     "Query cannot be z-normalized: non-finite or near-zero standard deviation.");
     const bool excl = !(d->exclude_begin == -1 && d->exclude_end == -1);
@@ -282,7 +289,7 @@ struct fsm_target {
     int64_t N = 0;
     std::shared_ptr<DevBuf> T;
     std::shared_ptr<TargetCache> cache;
-    int stats_m = -1;
+    int stats_m = -1;                // m, or -m for raw matching's statistics
     std::shared_ptr<CacheEntry> st;  // MU, SG, INV of stats_m
     int env_w = -1;
     std::shared_ptr<CacheEntry> env;  // Lr, Ur of env_w
@@ -404,16 +411,19 @@ int cached(fsm_target* t, std::map<int, std::shared_ptr<CacheEntry>> TargetCache
 }
 
 // ev[6] marks the end of the statistics' first kernel when they are computed.
-int ensure_stats(fsm_target* t, const Geometry& G, bool* computed) {
+// Raw matching's statistics (mean 0, std 1, invalid = a non-finite sample)
+// are cached under the key -m.
+int ensure_stats(fsm_target* t, const Geometry& G, bool* computed, bool raw = false) {
     *computed = false;
-    if (t->reuse && t->st && t->stats_m == G.m) return FSM_OK;
+    const int key = raw ? -G.m : G.m;
+    if (t->reuse && t->st && t->stats_m == key) return FSM_OK;
     const size_t n = (size_t)G.S * (size_t)G.kml;
     auto fill = [&](CacheEntry& e) {
         HIP_TRY(e.a.ensure(sizeof(double) * n));
         HIP_TRY(e.b.ensure(sizeof(double) * n));
         HIP_TRY(e.c.ensure(n));
         HIP_TRY(launch_window_stats(t->stream, t->T->as<double>(), G, e.a.as<double>(),
-                                    e.b.as<double>(), e.c.as<uint8_t>(), t->ev[6]));
+                                    e.b.as<double>(), e.c.as<uint8_t>(), t->ev[6], raw));
         return FSM_OK;
     };
     t->st.reset();  // before own buffers may be refilled
@@ -424,11 +434,11 @@ int ensure_stats(fsm_target* t, const Geometry& G, bool* computed) {
         *computed = true;
     } else {
         const int rc = cached(t, &TargetCache::stats, &TargetCache::seen_stats,
-                              &TargetCache::req_stats, G.m, 17 * (uint64_t)n, &t->own_st, fill,
+                              &TargetCache::req_stats, key, 17 * (uint64_t)n, &t->own_st, fill,
                               &t->st, computed);
         if (rc) return rc;
     }
-    t->stats_m = G.m;
+    t->stats_m = key;
     return FSM_OK;
 }
 
@@ -807,7 +817,7 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
     if (!make_geometry(t->N, d->m, &G))
         return fail(FSM_ERR_ARG, "Macro segment exceeds the supported FFT length.");
     HostQuery H;
-    prepare_query(q, d->m, d->w, d->arg, &H);
+    prepare_query(q, d->m, d->w, d->arg, &H, d->raw != 0);
     QueryTables Q{};
     rc = upload_query(t, H, d->m, &Q);
     if (rc) return rc;
@@ -824,7 +834,7 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
 
     HIP_TRY(hipEventRecord(t->ev[0], t->stream));
     bool stats_new = false, env_new = false;
-    rc = ensure_stats(t, G, &stats_new);
+    rc = ensure_stats(t, G, &stats_new, d->raw != 0);
     if (rc) return bail(rc);
     HIP_TRY(hipEventRecord(t->ev[1], t->stream));
 
@@ -1039,12 +1049,12 @@ int fsm_stage_lbip(fsm_target* t, const fsm_query_desc* d, const double* q, doub
     Geometry G;
     if (!make_geometry(t->N, d->m, &G)) return fail(FSM_ERR_ARG, "invalid m");
     HostQuery H;
-    prepare_query(q, d->m, d->w, d->arg, &H);
+    prepare_query(q, d->m, d->w, d->arg, &H, d->raw != 0);
     QueryTables Q{};
     if ((rc = upload_query(t, H, d->m, &Q))) return rc;
     QueryParams P = params_of(d);
     bool fresh;
-    if ((rc = ensure_stats(t, G, &fresh))) return rc;
+    if ((rc = ensure_stats(t, G, &fresh, d->raw != 0))) return rc;
     if ((rc = ensure_env(t, d->w))) return rc;
     if ((rc = run_lbip(t, G, Q, P, false))) return rc;
     HIP_TRY(hipStreamSynchronize(t->stream));
@@ -1068,12 +1078,12 @@ static int stage_verify(fsm_target* t, const fsm_query_desc* d, const double* q,
         if (win[i] < 0 || win[i] >= G.ts_n) return fail(FSM_ERR_ARG, "window index out of range");
     if (n == 0) return FSM_OK;
     HostQuery H;
-    prepare_query(q, d->m, d->w, d->arg, &H);
+    prepare_query(q, d->m, d->w, d->arg, &H, d->raw != 0);
     QueryTables Q{};
     if ((rc = upload_query(t, H, d->m, &Q))) return rc;
     QueryParams P = params_of(d);
     bool fresh;
-    if ((rc = ensure_stats(t, G, &fresh))) return rc;
+    if ((rc = ensure_stats(t, G, &fresh, d->raw != 0))) return rc;
     if (glb && (rc = ensure_env(t, d->w))) return rc;
     if ((rc = ensure_eld_scratch(t, d->metric, d->m, d->w))) return rc;
     HIP_TRY(t->list_a.ensure(sizeof(int64_t) * (size_t)n));

@@ -105,10 +105,11 @@ __device__ inline void block_append(bool keep, int64_t value, double dval, int64
 int report_error(int code, const char* msg);
 
 // ------------------------------------------------------------ launchers ---
-// k_window_sums then k_window_finish; `between` (or null) is recorded
-// between the two
+// k_window_sums then k_window_finish (raw: mean 0, std 1 and only the
+// non-finite test); `between` (or null) is recorded between the two
 hipError_t launch_window_stats(hipStream_t st, const double* T, const Geometry& G, double* MU,
-                               double* SG, uint8_t* INV, hipEvent_t between = nullptr);
+                               double* SG, uint8_t* INV, hipEvent_t between = nullptr,
+                               bool raw = false);
 hipError_t launch_envelope(hipStream_t st, const double* T, int64_t N, int w, double* Lraw,
                            double* Uraw);
 

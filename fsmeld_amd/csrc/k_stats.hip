@@ -123,8 +123,23 @@ __global__ void __launch_bounds__(256) k_window_finish(int64_t n, int m, double*
     }
 }
 
+// Raw matching (paper App. D): no mean / std; a window is invalid only when
+// it holds a non-finite sample.  Mean 0 and std 1 make every later stage's
+// normalisation (x - 0) / 1 = x exact, so the cascade runs on raw values.
+__global__ void __launch_bounds__(256) k_window_finish_raw(int64_t n, double* __restrict__ MU,
+                                                           double* __restrict__ SG,
+                                                           uint8_t* __restrict__ INV) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const bool bad = (unsigned long long)__double_as_longlong(MU[i]) == kBadSum;
+        MU[i] = 0.0;
+        SG[i] = 1.0;
+        INV[i] = bad;
+    }
+}
+
 hipError_t launch_window_stats(hipStream_t st, const double* T, const Geometry& G, double* MU,
-                               double* SG, uint8_t* INV, hipEvent_t between) {
+                               double* SG, uint8_t* INV, hipEvent_t between, bool raw) {
     const int bs = 64;
     const int64_t blocks = (G.S + bs - 1) / bs;
     hipLaunchKernelGGL(k_window_sums, dim3((unsigned)blocks), dim3(bs), 0, st, T, G, MU, SG);
@@ -134,7 +149,12 @@ hipError_t launch_window_stats(hipStream_t st, const double* T, const Geometry& 
     }
     const int64_t n = G.S * G.kml;
     const int64_t fb = std::min<int64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_window_finish, dim3((unsigned)fb), dim3(256), 0, st, n, G.m, MU, SG, INV);
+    if (raw)
+        hipLaunchKernelGGL(k_window_finish_raw, dim3((unsigned)fb), dim3(256), 0, st, n, MU, SG,
+                           INV);
+    else
+        hipLaunchKernelGGL(k_window_finish, dim3((unsigned)fb), dim3(256), 0, st, n, G.m, MU, SG,
+                           INV);
     return hipGetLastError();
 }
 

@@ -62,6 +62,11 @@ typedef struct {
     int k;                 /* cutoff / exact top-k: rank k >= 1 */
     int64_t exclude_begin; /* candidate exclusion [begin, end); -1 = none */
     int64_t exclude_end;
+    int raw;               /* 0: z-normalised matching (the reference's only mode,
+                              NormalizationMode::ZScore, include/funcs.h:20-28);
+                              1: raw matching (NormalizationMode::Raw, paper App. D):
+                              query and windows as they are, a window is invalid only
+                              when it holds a non-finite sample */
 } fsm_query_desc;
 
 typedef struct {

@@ -61,6 +61,7 @@ _sig("orc_thresholds", None, [_I, _D, _D, _I, _D, _D, _I, ctypes.c_double, _D, _
 _sig("orc_q_to_h", None, [_I, _D, _I, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D])
 _sig("orc_s_to_g", None, [_D, _D, _I, ctypes.c_double, ctypes.c_double, _D, _D])
 _sig("orc_sip_direct", None, [_D, _I, _D, _I, _D])
+_sig("orc_set_normalization_mode", None, [_I])
 _sig("orc_trans", ctypes.c_double, [_I, ctypes.c_double, ctypes.c_double, _I])
 _sig("orc_lbip_exceeds_cutoff", _I, [ctypes.c_double, ctypes.c_double, _I])
 _sig("orc_glb", ctypes.c_double, [_I, _D, _D, _I, ctypes.c_double, _D, _D, _I, ctypes.c_double])
@@ -190,6 +191,18 @@ def normalize_query(q_raw, m):
     out = np.empty(m)
     _lib.orc_normalize_query(_p(q), m, _p(out))
     return out
+
+
+class raw_mode:
+    """Context manager: the drivers run raw (non-normalised) matching inside
+    it (orc_set_normalization_mode(1)), z-normalised matching outside."""
+
+    def __enter__(self):
+        _lib.orc_set_normalization_mode(1)
+        return self
+
+    def __exit__(self, *exc):
+        _lib.orc_set_normalization_mode(0)
 
 
 # ---- drivers ----

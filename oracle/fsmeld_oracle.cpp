@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <atomic>
 #include <limits>
 #include <queue>
 #include <utility>
@@ -933,8 +934,22 @@ struct Query {
     std::vector<double> qn, qs, LQ, UQ;
 };
 
+// Matching mode, as the reference's set_normalization_mode
+// (src/normalization_mode.cpp): 0 z-normalised, 1 raw (paper App. D).
+std::atomic<int> g_raw{0};
+
 // src/fsmeld.cpp:622-643 (read order), :721 (query envelope)
 void prepare_query(const double* q_raw, int m, int w, Query* Q) {
+    if (g_raw.load()) {  // raw matching: the query as it is
+        Q->m = m;
+        Q->qn.assign(q_raw, q_raw + m);
+        Q->qs = Q->qn;
+        std::sort(Q->qs.begin(), Q->qs.end());
+        Q->LQ.resize(m);
+        Q->UQ.resize(m);
+        lower_upper_lemire(Q->qn.data(), m, w, Q->LQ.data(), Q->UQ.data());
+        return;
+    }
     double ex = 0, ex2 = 0;
     for (int i = 0; i < m; i++) {
         ex += q_raw[i];
@@ -971,11 +986,25 @@ struct SegmentWork {
 bool excluded(int64_t g, int64_t b, int64_t e) { return b >= 0 && e > b && g >= b && g < e; }
 
 // Stats + mask for one segment: src/fsmeld.cpp:753-757 (full), :861-866 (tail).
+// Raw matching (paper App. D) estimates no mean or std: every window is
+// taken as it is (mean 0, std 1 leave it unchanged under the normalisation
+// and its envelope bounds) and only a non-finite sample invalidates it.
 void segment_stats(const double* t, const Geometry& G, int m, int64_t block_start,
                    int first, int count, int64_t xb, int64_t xe, SegmentWork& W) {
-    mvmean(t, G.m_k, m, W.mu.data(), W.si.data());
     std::fill(W.mask.begin(), W.mask.end(), 0);
-    invalid_mask(t, W.si.data(), G.m_k, m, W.mask.data());
+    if (g_raw.load()) {
+        std::fill(W.mu.begin(), W.mu.end(), 0.0);
+        std::fill(W.si.begin(), W.si.end(), 1.0);
+        int bad = 0;
+        for (int k = 0; k < m; k++) bad += !std::isfinite(t[k]);
+        for (int64_t j = 0; j < G.k_m_l; j++) {
+            if (j) bad += (int)!std::isfinite(t[j + m - 1]) - (int)!std::isfinite(t[j - 1]);
+            W.mask[j] = bad > 0;
+        }
+    } else {
+        mvmean(t, G.m_k, m, W.mu.data(), W.si.data());
+        invalid_mask(t, W.si.data(), G.m_k, m, W.mask.data());
+    }
     for (int j = first; j < first + count; j++)
         if (excluded(block_start + (j - first), xb, xe)) W.mask[j] = 1;
 }
@@ -1133,6 +1162,7 @@ void orc_sip_direct(const double* g, int n, const double* h, int m, double* z) {
     sip_direct(g, n, h, m, z);
 }
 double orc_trans(int metric, double zl, double zu, int m) { return trans1(metric, zl, zu, m); }
+void orc_set_normalization_mode(int raw) { g_raw.store(raw ? 1 : 0); }
 int orc_lbip_exceeds_cutoff(double bound, double cutoff, int metric) {
     return lbip_exceeds_cutoff(bound, cutoff, metric) ? 1 : 0;
 }

@@ -37,6 +37,10 @@ void orc_s_to_g(const double* L_n, const double* U_n, int n, double t_l, double 
                 double* g_l, double* g_u);
 /* direct sliding inner product: z[j+m-1] = sum_{s<m} g[j+s] h[s], j in [0, n-m] */
 void orc_sip_direct(const double* g, int n, const double* h, int m, double* z);
+/* 0: z-normalised matching (default); 1: raw matching (paper App. D: query
+ * and windows as they are, a window invalid only when it holds a non-finite
+ * sample).  Process-wide, like the reference's set_normalization_mode. */
+void orc_set_normalization_mode(int raw);
 /* LB_IP value of one window from its two sliding sums (trans, src/fsmeld.cpp:84-106) */
 double orc_trans(int metric, double z_l, double z_u, int m);
 int orc_lbip_exceeds_cutoff(double bound, double cutoff, int metric);

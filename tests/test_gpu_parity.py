@@ -391,3 +391,43 @@ def test_exact_topk_matches_oracle(gpu, metric, arg):
     with pytest.raises(F.FsmeldError) as e:
         F.Target(t[:200]).exact_topk(q[:32], 32, 170, metric, 3, arg)   # 169 windows
     assert e.value.code in (-1, -4)
+
+
+@pytest.mark.parametrize("metric,arg", METRICS)
+def test_raw_mode_matches_oracle(gpu, metric, arg):
+    """Raw (non-normalised) matching, paper App. D: cutoff threshold bits and
+    ranking, search and exact windows, distance bits, exact top-k and the
+    LB_IP bounds equal the oracle's raw mode; a constant query, which
+    z-normalised matching rejects, is accepted."""
+    t = make_target(91, 1 << 14, nans=3, runs=4)
+    T = F.Target(t)
+    m, w, k = 48, 4, 6
+    q, _ = F.synth_query(91, t, m, 0.05)   # a finite window of the target, plus noise
+    with O.raw_mode():
+        thr, ranks, rd, _ = O.cutoff(t, q, m, k, metric, w, arg)
+        os_ids, ost = O.search(t, q, m, thr, metric, w, arg)
+        oe_ids, oe_d, _ = O.exact(t, q, m, thr, metric, w, arg)
+        kth, tk_ids, tk_d, _ = O.exact_topk(t, q, m, k, metric, w, arg)
+        olb, _, _ = O.lbip_all(t, q, m, metric, w, arg)
+    g = T.cutoff(q, m, k, metric, w, arg, raw=True)
+    assert bits(g.threshold) == bits(thr)
+    assert np.array_equal(g.indices, ranks) and same_bits(g.distances, rd)
+    s = T.search(q, m, thr, metric, w, arg, raw=True)
+    e = T.exact(q, m, thr, metric, w, arg, raw=True)
+    assert np.array_equal(s.indices, os_ids)
+    assert np.array_equal(e.indices, oe_ids) and same_bits(e.distances, oe_d)
+    assert s.stats["valid"] == ost["valid"] and s.stats["invalid"] == ost["invalid"]
+    tk = T.exact_topk(q, m, k, metric, w, arg, raw=True)
+    assert bits(tk.threshold) == bits(kth) and np.array_equal(tk.indices, tk_ids)
+    lb, _, _ = T.lbip(q, m, metric, w, arg, raw=True)
+    ok = np.isfinite(olb)
+    assert np.array_equal(np.isfinite(lb), ok)
+    # equal wherever both chose the same thresholds (near-ties in the
+    # threshold search can resolve differently, as in z-normalised matching)
+    close = np.abs(lb[ok] - olb[ok]) <= 1e-9 * np.maximum(1.0, np.abs(olb[ok]))
+    assert close.mean() >= 0.9
+    flat = np.full(m, 0.5)
+    with O.raw_mode():
+        fthr, franks, _, _ = O.cutoff(t, flat, m, 3, metric, w, arg)
+    fg = T.cutoff(flat, m, 3, metric, w, arg, raw=True)
+    assert bits(fg.threshold) == bits(fthr) and np.array_equal(fg.indices, franks)

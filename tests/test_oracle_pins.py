@@ -275,3 +275,38 @@ def test_exact_topk_is_the_kth_distance_with_ties(metric, arg):
         assert ids.size > k   # integer-valued distances tie at the threshold
     with pytest.raises(LookupError):
         O.exact_topk(t[:80], q, 32, 60, metric, 3, arg)
+
+
+# --------------------------------------------------------------- raw mode ---
+@pytest.mark.parametrize("metric,arg", [("dtw", 0.0), ("erp", 0.5), ("edr", 0.3),
+                                        ("lcss", 0.3), ("msm", 0.5)])
+def test_raw_mode_is_the_cascade_on_unnormalised_values(metric, arg):
+    """Raw matching (NormalizationMode::Raw, paper App. D): every distance is
+    the verifier's on the raw query and raw window, a window is invalid only
+    when it holds a non-finite sample (a constant run stays valid), search
+    returns exactly the exact scan's windows, cutoff is the k-th of them, and
+    the LB_IP bounds stay below the distances."""
+    import fsmeld_amd as F
+    t = F.synth_target(29, 4000, 0.001, 5.0, 2, 120, 2)
+    q = t[700:740] + np.random.default_rng(1).normal(size=40) * 0.05
+    m, w = 40, 3
+    with O.raw_mode():
+        mu, sg, inv = O.window_stats(t, m)
+        ids, d, st = O.exact(t, q, m, 1e300, metric, w, arg, threads=4)
+        thr, ranks, _, _ = O.cutoff(t, q, m, 6, metric, w, arg)
+        s_ids, _ = O.search(t, q, m, thr, metric, w, arg)
+        e_ids, _, _ = O.exact(t, q, m, thr, metric, w, arg)
+        lb, _, _ = O.lbip_all(t, q, m, metric, w, arg)
+    win = np.lib.stride_tricks.sliding_window_view(t, m)
+    assert np.all(mu == 0.0) and np.all(sg == 1.0)
+    assert np.array_equal(inv, ~np.isfinite(win).all(axis=1))
+    assert not inv[np.flatnonzero(np.all(np.diff(win, axis=1) == 0, axis=1))].any()
+    assert np.array_equal(ids, np.flatnonzero(~inv))
+    want = np.array([O.eld(metric, q, t[g:g + m], w, arg) for g in ids])
+    assert np.array_equal(d, want)
+    assert thr == np.sort(d)[5]
+    assert np.array_equal(s_ids, e_ids) and np.all(np.isin(ranks, s_ids))
+    assert np.all(~(lb[ids] > d + 1e-9 * np.maximum(1.0, d)))
+    # the mode matters: z-normalised distances are not these
+    zd = O.exact(t, q, m, 1e300, metric, w, arg)[1]
+    assert not np.array_equal(zd[: d.size], d[: zd.size])
