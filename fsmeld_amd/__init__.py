@@ -24,7 +24,7 @@ import numpy as np
 
 __all__ = [
     "METRICS", "Target", "QueryResult", "FsmeldError", "search", "cutoff", "exact",
-    "synth_target", "synth_query", "device_count", "lib_path",
+    "synth_target", "synth_query", "device_count", "lib_path", "query_file",
 ]
 
 METRICS = {"dtw": 1, "edr": 2, "erp": 3, "lcss": 4, "msm": 5}
@@ -104,6 +104,8 @@ _sig("fsm_eld_pairs", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, c
                                      _D, _D, ctypes.c_int64, _D])
 _sig("fsm_target_share", ctypes.c_int, [_VP, ctypes.POINTER(_VP)])
 _sig("fsm_target_trim", None, [_VP])
+_sig("fsm_query_file", ctypes.c_int, [ctypes.c_char_p, _VP, _D, ctypes.c_int64, ctypes.c_uint64,
+                                      ctypes.POINTER(_VP)])
 _sig("fsm_target_reuse_stats", ctypes.c_int, [_VP, ctypes.c_int])
 _sig("fsm_target_cache_limit", ctypes.c_int, [_VP, ctypes.c_uint64])
 _sig("fsm_query_bytes", ctypes.c_uint64, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int])
@@ -385,6 +387,19 @@ geometry = _geometry
 
 
 # ---- module-level wrappers named after the reference's entry points ----
+def query_file(path, mode, query, m, metric, w, arg, cutoff_value=0.0, k=0, exclude=None,
+               raw=False, device_budget=8 << 30) -> QueryResult:
+    """A query over a target FILE with at most device_budget bytes of device
+    memory (fsm_query_file): the file streams through the GPU in chunks of
+    whole macro-segments; the result equals the whole-target query's."""
+    q = _f64(query)
+    d = _desc(mode, metric, m, w, arg, cutoff_value, k, exclude, raw)
+    r = _VP()
+    _check(_lib.fsm_query_file(os.fsencode(path), ctypes.byref(d), _ptr(q), int(q.size),
+                               ctypes.c_uint64(int(device_budget)), ctypes.byref(r)))
+    return _take_result(r)
+
+
 def search(target, query, m, cutoff_value, metric, w, arg, exclude=None) -> QueryResult:
     """FSMELD (include/funcs.h:67-68)."""
     t = target if isinstance(target, Target) else Target(target)

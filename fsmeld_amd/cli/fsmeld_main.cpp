@@ -7,6 +7,7 @@
 // at once), and the cutoff report's pruning counters describe the GPU's
 // representative-then-search schedule rather than the reference's
 // sequential dynamic threshold.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -182,8 +183,19 @@ int main(int argc, char** argv) {
         }
 
         Run run;
-        check(fsm_target_open_file(argv[2], &run.target));
-        check(fsm_query(run.target, &d, query.data(), (int64_t)query.size(), &run.result));
+        // FSMELD_DEVICE_BUDGET=<bytes>: stream the target file through the
+        // GPU with at most that much device memory (fsm_query_file), as the
+        // reference reads its target segment by segment; unset: the whole
+        // target is made device-resident
+        const char* budget = std::getenv("FSMELD_DEVICE_BUDGET");
+        if (budget && *budget) {
+            const auto bytes_allowed = parse_arg<long long>(budget, "FSMELD_DEVICE_BUDGET");
+            check(fsm_query_file(argv[2], &d, query.data(), (int64_t)query.size(),
+                                 (uint64_t)std::max<long long>(bytes_allowed, 0), &run.result));
+        } else {
+            check(fsm_target_open_file(argv[2], &run.target));
+            check(fsm_query(run.target, &d, query.data(), (int64_t)query.size(), &run.result));
+        }
         fsm_stats s{};
         check(fsm_result_stats(run.result, &s));
         const int64_t n = fsm_result_count(run.result);

@@ -1289,3 +1289,273 @@ void fsm_target_trim(fsm_target* t) {
         t->cache->drop_idle();
     }
 }
+
+// ------------------------------------------------------ out-of-core query ---
+// fsm_query_file: the reference's file streaming (src/fsmeld.cpp:740-746,
+// read_tail_segment :159-168) for targets larger than the device budget.
+// The target is cut into chunks of whole macro-segments: a chunk of segments
+// [s0, s1) is the sample range [s0 k_m_l, (s1 - 1) k_m_l + m_k), and the last
+// chunk runs to the end of the file, so its tail segment is the reference's
+// EOF-aligned one.  Run as a target of its own, a chunk has exactly the
+// global segmentation (no tail except in the last chunk), so every window
+// gets the same statistics, bounds and distance as in a whole-target query;
+// its indices are shifted by s0 k_m_l.  Two handles alternate: while one
+// chunk's cascade runs, a reader thread fills a pinned buffer with the next
+// chunk and it is uploaded on a copy stream into the other handle.
+namespace {
+
+struct PinnedBuf {
+    double* p = nullptr;
+    size_t cap = 0;  // doubles
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipHostMalloc((void**)&p, sizeof(double) * n, hipHostMallocDefault);
+        if (e == hipSuccess) cap = n;
+        return e;
+    }
+};
+
+struct ChunkPlan {
+    int64_t off, n;  // sample range [off, off + n)
+};
+
+// Chunks of `per` full segments each; the last also takes the tail segment
+// and at least one full segment (a chunk shorter than m_k would change the
+// segment length).
+std::vector<ChunkPlan> plan_chunks(const Geometry& G, int64_t per) {
+    std::vector<ChunkPlan> v;
+    if (G.forsize <= 0) {  // a single (tail) segment covers the target
+        v.push_back({0, G.N});
+        return v;
+    }
+    for (int64_t s0 = 0; s0 < G.forsize; s0 += per) {
+        int64_t s1 = std::min(G.forsize, s0 + per);
+        if (G.forsize - s1 < 1 && s1 < G.forsize) s1 = G.forsize;
+        const bool last = s1 >= G.forsize;
+        const int64_t off = s0 * G.kml;
+        const int64_t end = last ? G.N : (s1 - 1) * G.kml + G.m_k;
+        v.push_back({off, end - off});
+        if (last) break;
+    }
+    return v;
+}
+
+// device bytes of one chunk handle: its samples and its query's buffers
+uint64_t chunk_bytes(int64_t n, const fsm_query_desc* d) {
+    return 8 * (uint64_t)n + fsm_query_bytes(n, d->metric, d->m, d->w);
+}
+
+struct Hit {
+    double d;
+    int64_t g;
+};
+
+}  // namespace
+
+extern "C" int fsm_query_file(const char* path, const fsm_query_desc* d, const double* q,
+                              int64_t qlen, uint64_t budget, fsm_result** out) {
+    const auto wall0 = std::chrono::steady_clock::now();
+    if (!path || !out || !d) return fail(FSM_ERR_ARG, "null argument");
+    *out = nullptr;
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) return fail(FSM_ERR_IO, "Cannot open target file.");
+    const std::streamoff bytes = f.tellg();
+    if (bytes <= 0 || bytes % (std::streamoff)sizeof(double))
+        return fail(FSM_ERR_ARG, "Target must contain at least 2*m-1 complete double values.");
+    const int64_t N = (int64_t)(bytes / (std::streamoff)sizeof(double));
+    int rc = validate(d, N, q, qlen, true);
+    if (rc) return rc;
+    Geometry G;
+    if (!make_geometry(N, d->m, &G))
+        return fail(FSM_ERR_ARG, "Macro segment exceeds the supported FFT length.");
+    if (fsm_device_count() < 1) return fail(FSM_ERR_NODEV, "no HIP device");
+
+    // the most full segments per chunk whose two handles fit the budget
+    int64_t per = std::max<int64_t>(G.forsize, 1);
+    auto fits = [&](int64_t p) {
+        uint64_t mx = 0;
+        for (const ChunkPlan& c : plan_chunks(G, p)) mx = std::max(mx, chunk_bytes(c.n, d));
+        return 2 * mx <= budget;
+    };
+    if (!fits(per)) {
+        int64_t lo = 1, hi = per;
+        if (!fits(lo))
+            return fail(FSM_ERR_OOM, "the device budget holds no chunk of one macro-segment");
+        while (hi - lo > 1) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            (fits(mid) ? lo : hi) = mid;
+        }
+        per = lo;
+    }
+    const std::vector<ChunkPlan> chunks = plan_chunks(G, per);
+    size_t cap = 0;
+    for (const ChunkPlan& c : chunks) cap = std::max(cap, (size_t)c.n);
+
+    // two handles (own samples, no shared cache) and two pinned buffers
+    fsm_target* h[2] = {nullptr, nullptr};
+    PinnedBuf pin[2];
+    hipStream_t copy = nullptr;
+    hipEvent_t up[2] = {nullptr, nullptr};
+    auto cleanup = [&](int code) {
+        if (copy) (void)hipStreamSynchronize(copy);
+        for (auto& e : up)
+            if (e) (void)hipEventDestroy(e);
+        if (copy) (void)hipStreamDestroy(copy);
+        for (auto* x : h)
+            if (x) fsm_target_close(x);
+        return code;
+    };
+    for (int s = 0; s < 2 && s < (int)chunks.size(); s++) {
+        auto cache = std::make_shared<TargetCache>();  // cap 0: nothing shared
+        rc = new_handle(0, std::shared_ptr<DevBuf>(new DevBuf(), [](DevBuf* b) {
+                            b->release();
+                            delete b;
+                        }), std::move(cache), &h[s]);
+        if (rc) return cleanup(rc);
+        h[s]->reuse = false;
+        if (h[s]->T->ensure(sizeof(double) * cap) != hipSuccess)
+            return cleanup(fail(FSM_ERR_OOM, "device allocation of a target chunk failed"));
+        if (pin[s].ensure(cap) != hipSuccess)
+            return cleanup(fail(FSM_ERR_OOM, "pinned host allocation failed"));
+        if (hipEventCreateWithFlags(&up[s], hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(FSM_ERR_HIP, "hipEventCreate failed"));
+    }
+    if (hipStreamCreateWithFlags(&copy, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(FSM_ERR_HIP, "hipStreamCreate failed"));
+
+    // reading chunk c into pin[c % 2] (after that buffer's last upload)
+    bool read_ok[2] = {true, true}, recorded[2] = {false, false};
+    auto read_chunk = [&](size_t c) {
+        const int s = (int)(c % 2);
+        if (recorded[s]) (void)hipEventSynchronize(up[s]);
+        std::ifstream in(path, std::ios::binary);
+        in.seekg((std::streamoff)chunks[c].off * (std::streamoff)sizeof(double));
+        in.read(reinterpret_cast<char*>(pin[s].p),
+                (std::streamsize)(sizeof(double) * (size_t)chunks[c].n));
+        read_ok[s] = (bool)in;
+    };
+    auto upload = [&](size_t c) -> int {
+        const int s = (int)(c % 2);
+        if (!read_ok[s]) return fail(FSM_ERR_IO, "short read of target file");
+        HIP_TRY(hipMemcpyAsync(h[s]->T->p, pin[s].p, sizeof(double) * (size_t)chunks[c].n,
+                               hipMemcpyHostToDevice, copy));
+        HIP_TRY(hipEventRecord(up[s], copy));
+        recorded[s] = true;
+        h[s]->N = chunks[c].n;
+        h[s]->stats_m = -1;
+        h[s]->env_w = -1;
+        return FSM_OK;
+    };
+
+    std::unique_ptr<fsm_result> R(new fsm_result());
+    fsm_stats& st = R->st;
+    st.ts_n = G.ts_n;
+    st.k_m_l = G.kml;
+    st.forsize = G.forsize;
+    st.tail = G.tail;
+    st.m_k = G.m_k;
+    std::vector<Hit> hits;  // search / exact: in order; cutoff / top-k: candidates
+    double theta = std::numeric_limits<double>::infinity();
+    const bool is_cut = d->mode == FSM_MODE_CUTOFF, is_topk = d->mode == FSM_MODE_EXACT_TOPK;
+
+    read_chunk(0);
+    if ((rc = upload(0))) return cleanup(rc);
+    std::thread reader;
+    if (chunks.size() > 1) reader = std::thread(read_chunk, (size_t)1);
+    for (size_t c = 0; c < chunks.size(); c++) {
+        const int s = (int)(c % 2);
+        if (c + 1 < chunks.size()) {  // next chunk: read, then upload while this one runs
+            reader.join();
+            if ((rc = upload(c + 1))) return cleanup(rc);
+            if (c + 2 < chunks.size()) reader = std::thread(read_chunk, c + 2);
+        }
+        if (hipStreamWaitEvent(h[s]->stream, up[s], 0) != hipSuccess) {
+            if (reader.joinable()) reader.join();
+            return cleanup(fail(FSM_ERR_HIP, "hipStreamWaitEvent failed"));
+        }
+        // this chunk's query: windows [off, off + ts_n_c) of the target
+        const int64_t off = chunks[c].off, nwin = chunks[c].n - d->m + 1;
+        fsm_query_desc dc = *d;
+        const bool excl = !(d->exclude_begin == -1 && d->exclude_end == -1);
+        dc.exclude_begin = dc.exclude_end = -1;
+        if (excl) {
+            const int64_t b = std::max<int64_t>(d->exclude_begin - off, 0);
+            const int64_t e = std::min<int64_t>(d->exclude_end - off, nwin);
+            if (b < e) {
+                dc.exclude_begin = b;
+                dc.exclude_end = e;
+            }
+        }
+        const bool known = (is_cut || is_topk) && (int64_t)hits.size() >= d->k;
+        if (known) {  // later chunks: every window at or below the k-th so far
+            dc.mode = is_cut ? FSM_MODE_SEARCH : FSM_MODE_EXACT;
+            dc.cutoff = theta;
+        }
+        fsm_result* r = nullptr;
+        const bool few = (is_cut || is_topk) && !known && nwin < d->k;
+        rc = few ? FSM_ERR_FEWER_THAN_K : fsm_query(h[s], &dc, q, qlen, &r);
+        if (rc == FSM_ERR_FEWER_THAN_K && (is_cut || is_topk)) {
+            // fewer than k windows here: all of them take part
+            dc.mode = FSM_MODE_EXACT;
+            dc.cutoff = std::numeric_limits<double>::max();
+            rc = fsm_query(h[s], &dc, q, qlen, &r);
+        }
+        if (rc) {
+            if (reader.joinable()) reader.join();
+            return cleanup(rc);
+        }
+        std::unique_ptr<fsm_result> rr(r);
+        const fsm_stats& cs = rr->st;
+        st.valid += cs.valid;
+        st.invalid += cs.invalid;
+        st.after_lbip += cs.after_lbip;
+        st.after_glb += cs.after_glb;
+        st.exact_evals += cs.exact_evals;
+        st.eld_cells += cs.eld_cells;
+        st.ms_stats += cs.ms_stats;
+        st.ms_stats_finish += cs.ms_stats_finish;
+        st.ms_env += cs.ms_env;
+        st.ms_lbip += cs.ms_lbip;
+        st.ms_glb += cs.ms_glb;
+        st.ms_eld += cs.ms_eld;
+        for (size_t i = 0; i < rr->idx.size(); i++) hits.push_back({rr->dist[i], rr->idx[i] + off});
+        if (is_cut || is_topk) {
+            // keep the k (distance, index)-smallest (top-k: and their ties)
+            std::sort(hits.begin(), hits.end(), [](const Hit& a, const Hit& b) {
+                return a.d != b.d ? a.d < b.d : a.g < b.g;
+            });
+            if ((int64_t)hits.size() >= d->k) {
+                theta = hits[(size_t)d->k - 1].d;
+                size_t keep = (size_t)d->k;
+                if (is_topk)
+                    while (keep < hits.size() && hits[keep].d <= theta) keep++;
+                hits.resize(keep);
+            }
+        }
+    }
+    if (reader.joinable()) reader.join();
+    rc = cleanup(FSM_OK);
+    if ((is_cut || is_topk) && (int64_t)hits.size() < d->k)
+        return fail(FSM_ERR_FEWER_THAN_K,
+                    is_cut ? "Fewer valid, non-excluded candidates than requested k."
+                           : "Fewer finite exact distances than requested k.");
+    if (is_topk)  // ascending index, as the reference lists them
+        std::sort(hits.begin(), hits.end(), [](const Hit& a, const Hit& b) { return a.g < b.g; });
+    for (const Hit& x : hits) {
+        R->idx.push_back(x.g);
+        R->dist.push_back(x.d);
+    }
+    if (is_cut || is_topk) st.threshold = theta;
+    st.matches = R->idx.size();
+    st.ms_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0)
+            .count();
+    *out = R.release();
+    return rc;
+}

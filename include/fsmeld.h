@@ -117,6 +117,16 @@ int fsm_target_share(fsm_target* base, fsm_target** out);
 int fsm_query(fsm_target* t, const fsm_query_desc* desc, const double* query_raw,
               int64_t query_len, fsm_result** out);
 int fsm_result_stats(const fsm_result* r, fsm_stats* out);
+
+/* The same query over a target FILE of any size with at most device_budget
+ * bytes of device memory (the reference's per-segment file streaming,
+ * src/fsmeld.cpp:740-746, read_tail_segment :159-168): the file is read in
+ * chunks of whole macro-segments (the last one EOF-aligned, as the
+ * reference's tail segment), each uploaded while the previous chunk's
+ * cascade runs; results, thresholds and counters equal fsm_query's on the
+ * whole target.  FSM_ERR_OOM when the budget holds no one-segment chunk. */
+int fsm_query_file(const char* path, const fsm_query_desc* desc, const double* query_raw,
+                   int64_t query_len, uint64_t device_budget, fsm_result** out);
 int64_t fsm_result_count(const fsm_result* r);
 const int64_t* fsm_result_indices(const fsm_result* r);
 const double* fsm_result_distances(const fsm_result* r);

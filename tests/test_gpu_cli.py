@@ -79,3 +79,8 @@ def test_cli_cutoff_search_exact(gpu, tmp_path, metric, arg):
     assert row[6] == "ELD" and int(row[14]) == len(ids) and row[15] == "" and row[16] == ""
     lines = csv.read_text().splitlines()
     assert lines[0].startswith("dataset,normalization,metric,") and len(lines) == 3
+    # streamed through a device budget smaller than the target: the same report
+    env_b = dict(env, FSMELD_DEVICE_BUDGET=str(2 * (8 * 8000 + F.query_bytes(8000, metric, m, w))))
+    out_b = run(["search", tf, qf, m, cut, metric, w, arg], env_b)
+    idx_b = re.findall(r"^A\[(\d+)\]=(\d+)$", out_b, re.M)
+    assert np.array_equal([int(v) for _, v in idx_b], want)
