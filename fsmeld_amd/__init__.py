@@ -64,7 +64,8 @@ class _Stats(ctypes.Structure):
                 ("ms_lbip", ctypes.c_double), ("ms_glb", ctypes.c_double),
                 ("ms_eld", ctypes.c_double), ("ms_total", ctypes.c_double),
                 ("eld_cells", ctypes.c_uint64), ("ms_stats_finish", ctypes.c_double),
-                ("ms_env", ctypes.c_double)]
+                ("ms_env", ctypes.c_double), ("near_rechecked", ctypes.c_uint64),
+                ("near_changed", ctypes.c_uint64)]
 
 
 _D = ctypes.POINTER(ctypes.c_double)

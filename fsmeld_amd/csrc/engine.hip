@@ -599,6 +599,38 @@ int tighten_cutoff(fsm_target* t, const Geometry& G, const QueryTables& Q, Query
     return FSM_OK;
 }
 
+// Near-boundary re-verification: a window whose device distance lies within
+// kNearRel (relative) of a decision boundary is recomputed on the host in
+// FP64 (host_eld) and the host value decides.
+constexpr double kNearRel = 1e-9;
+inline double near_upper(double c) { return c * (1.0 + kNearRel) + 1e-300; }
+inline bool near(double d, double c) { return std::fabs(d - c) <= kNearRel * std::fabs(c) + 1e-300; }
+
+// Host distances of windows idx[i] for the i with near(dist[i], boundary):
+// samples and window statistics are read back from the device.  Returns the
+// number rechecked (*changed: how many the host value moved).
+int recheck_near(fsm_target* t, const Geometry& G, const HostQuery& H, const fsm_query_desc* d,
+                 double boundary, std::vector<int64_t>& idx, std::vector<double>& dist,
+                 uint64_t* rechecked, uint64_t* changed) {
+    const int m = d->m;
+    std::vector<double> raw((size_t)m), y((size_t)m);
+    for (size_t i = 0; i < idx.size(); i++) {
+        if (!near(dist[i], boundary)) continue;
+        const int64_t g = idx[i], si = stat_index(G, g);
+        double mu = 0.0, sd = 1.0;
+        HIP_TRY(hipMemcpy(raw.data(), t->T->as<double>() + g, sizeof(double) * m,
+                          hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&mu, t->st->a.as<double>() + si, sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&sd, t->st->b.as<double>() + si, sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < m; k++) y[k] = (raw[k] - mu) / sd;  // src/fsmeld.cpp:822-826
+        const double h = host_eld(d->metric, H.qn.data(), y.data(), m, d->w, d->arg);
+        ++*rechecked;
+        if (std::memcmp(&h, &dist[i], sizeof(h)) != 0) ++*changed;
+        dist[i] = h;
+    }
+    return FSM_OK;
+}
+
 // counters: [0] compacted, [1] valid, [2] after GLB, [3] ELD kept, [8] ELD work
 // queue, [9] ELD lane-iterations on live candidates, [10] GLB work queue.  The LB buffer is free once compaction has read it; GLB writes each
 // survivor's consumption bound there for the ELD abandon test.
@@ -656,6 +688,10 @@ int run_filter_verify(fsm_target* t, const Geometry& G, const QueryTables& Q,
     HIP_TRY(hipEventRecord(t->ev[4], t->stream));
     int rc = ensure_eld_scratch(t, P.metric, P.m, P.w);
     if (rc) return rc;
+    // ELD keeps windows up to the cutoff widened by kNearRel, so the ones just
+    // above it reach the host re-verification (GLB above decided at the
+    // cutoff itself, as the reference's cascade does)
+    v.p.cutoff = near_upper(P.cutoff);
     HIP_TRY(launch_eld(t->stream, v, eld_in, eld_total, eld_cnt, false, t->out_idx.as<int64_t>(),
                        t->out_dist.as<double>(), C + 3, C + 8, C + 9, t->scratch.as<double>(),
                        eld_scratch_threads()));
@@ -953,9 +989,40 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
         HIP_TRY(hipMemcpy(R->dist.data(), t->out_dist.p, sizeof(double) * nres,
                           hipMemcpyDeviceToHost));
     }
+    // near the boundary the host decides: search / exact at the cutoff (the
+    // device kept windows up to near_upper(cutoff)); cutoff and top-k at the
+    // k-th distance
+    {
+        double boundary = d->cutoff;
+        if (d->mode == FSM_MODE_CUTOFF && !R->dist.empty()) {
+            std::vector<double> s2(R->dist);
+            const size_t kk = std::min<size_t>((size_t)d->k, s2.size()) - 1;
+            std::nth_element(s2.begin(), s2.begin() + kk, s2.end());
+            boundary = s2[kk];
+        } else if (d->mode == FSM_MODE_EXACT_TOPK) {
+            boundary = topk_kth;
+        }
+        rc = recheck_near(t, G, H, d, boundary, R->idx, R->dist, &st.near_rechecked,
+                          &st.near_changed);
+        if (rc) return bail(rc);
+        if (d->mode == FSM_MODE_SEARCH || d->mode == FSM_MODE_EXACT ||
+            d->mode == FSM_MODE_EXACT_TOPK) {
+            const double lim = (d->mode == FSM_MODE_EXACT_TOPK) ? topk_kth : d->cutoff;
+            size_t o = 0;
+            for (size_t i = 0; i < R->idx.size(); i++)
+                if (R->dist[i] <= lim) {  // src/fsmeld.cpp:846-847
+                    R->idx[o] = R->idx[i];
+                    R->dist[o] = R->dist[i];
+                    o++;
+                }
+            R->idx.resize(o);
+            R->dist.resize(o);
+        }
+    }
+    const size_t nkept = R->idx.size();
     // order: ascending index (search / exact), (distance, index) for cutoff
-    std::vector<size_t> ord(nres);
-    for (size_t i = 0; i < nres; i++) ord[i] = i;
+    std::vector<size_t> ord(nkept);
+    for (size_t i = 0; i < nkept; i++) ord[i] = i;
     if (d->mode == FSM_MODE_CUTOFF) {
         std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) {
             if (R->dist[a] != R->dist[b]) return R->dist[a] < R->dist[b];

@@ -101,6 +101,10 @@ __device__ inline void block_append(bool keep, int64_t value, double dval, int64
     __syncthreads();
 }
 
+// Exact elastic distance of one window on the host in FP64 (x the query, y
+// the normalised candidate), for re-verification near a decision boundary.
+double host_eld(int metric, const double* x, const double* y, int m, int w, double arg);
+
 // Records msg as the calling thread's fsm_last_error() and returns code.
 int report_error(int code, const char* msg);
 

@@ -89,6 +89,11 @@ typedef struct {
      * from the running sums); the target envelope, when this query computed
      * it (ms_lbip no longer includes it) */
     double ms_stats_finish, ms_env;
+    /* windows whose device distance lay within 1e-9 (relative) of the
+     * decision boundary (the cutoff; for cutoff / top-k the k-th distance)
+     * and were recomputed on the host in FP64, whose value then decided;
+     * how many of those host values differed from the device's */
+    uint64_t near_rechecked, near_changed;
 } fsm_stats;
 
 /* ---- device and errors ---- */

@@ -431,3 +431,27 @@ def test_raw_mode_matches_oracle(gpu, metric, arg):
         fthr, franks, _, _ = O.cutoff(t, flat, m, 3, metric, w, arg)
     fg = T.cutoff(flat, m, 3, metric, w, arg, raw=True)
     assert bits(fg.threshold) == bits(fthr) and np.array_equal(fg.indices, franks)
+
+
+@pytest.mark.parametrize("metric,arg", METRICS)
+def test_near_cutoff_windows_are_reverified_on_the_host(gpu, metric, arg):
+    """With the cutoff exactly on a window's distance and one ulp either side,
+    the windows within 1e-9 of it are recomputed on the host (their count is
+    reported, and the host values equal the device's); the results equal the
+    oracle's at each cutoff."""
+    t = make_target(93, 1 << 14)
+    T = F.Target(t)
+    m, w = 48, 4
+    q, _ = F.synth_query(93, t, m, 0.2)
+    c = T.cutoff(q, m, 6, metric, w, arg)
+    assert c.stats["near_rechecked"] >= 1 and c.stats["near_changed"] == 0
+    d5 = c.distances[4]
+    for cut in (d5, np.nextafter(d5, -np.inf), np.nextafter(d5, np.inf)):
+        s = T.search(q, m, cut, metric, w, arg)
+        e = T.exact(q, m, cut, metric, w, arg)
+        assert s.stats["near_rechecked"] >= 1 and s.stats["near_changed"] == 0
+        assert e.stats["near_rechecked"] >= 1 and e.stats["near_changed"] == 0
+        os_ids, _ = O.search(t, q, m, cut, metric, w, arg)
+        oe_ids, oe_d, _ = O.exact(t, q, m, cut, metric, w, arg)
+        assert np.array_equal(s.indices, os_ids), cut
+        assert np.array_equal(e.indices, oe_ids) and same_bits(e.distances, oe_d), cut
