@@ -27,7 +27,7 @@ __all__ = [
     "synth_target", "synth_query", "device_count", "lib_path", "query_file",
 ]
 
-METRICS = {"dtw": 1, "edr": 2, "erp": 3, "lcss": 4, "msm": 5}
+METRICS = {"dtw": 1, "edr": 2, "erp": 3, "lcss": 4, "msm": 5, "swale": 6}
 MODES = {"search": 0, "cutoff": 1, "exact": 2, "exact_topk": 3}
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

@@ -173,12 +173,12 @@ int validate(const fsm_query_desc* d, int64_t N, const double* q, int64_t qlen, 
         return fail(FSM_ERR_ARG, "Unknown command");
     if (d->m < 2 || d->m > (1 << 27))
         return fail(FSM_ERR_ARG, "m must be at least 2 and fit the FFT indexing range.");
-    if (d->metric < FSM_METRIC_DTW || d->metric > FSM_METRIC_MSM)
+    if (d->metric < FSM_METRIC_DTW || d->metric > FSM_METRIC_SWALE)
         return fail(FSM_ERR_ARG, "Unsupported metric");
     if (d->w < 0 || d->w >= d->m) return fail(FSM_ERR_ARG, "w must satisfy 0 <= w < m.");
     if (!std::isfinite(d->arg)) return fail(FSM_ERR_ARG, "Invalid finite metric parameter");
     if ((d->metric == FSM_METRIC_EDR || d->metric == FSM_METRIC_LCSS ||
-         d->metric == FSM_METRIC_MSM) && d->arg < 0)
+         d->metric == FSM_METRIC_MSM || d->metric == FSM_METRIC_SWALE) && d->arg < 0)
         return fail(FSM_ERR_ARG, "Matching tolerance and MSM cost must be non-negative.");
     if (N < 2LL * d->m - 1)
         return fail(FSM_ERR_ARG, "Target must contain at least 2*m-1 complete double values.");
@@ -1206,8 +1206,10 @@ int fsm_stage_eld(fsm_target* t, const fsm_query_desc* d, const double* q, const
 int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const double* cands,
                   int64_t n, double* out) {
     if (!q || !cands || !out || n < 0) return fail(FSM_ERR_ARG, "null argument");
-    if (m < 2 || w < 0 || w >= m || metric < FSM_METRIC_DTW || metric > FSM_METRIC_MSM)
+    if (m < 2 || w < 0 || w >= m || metric < FSM_METRIC_DTW || metric > FSM_METRIC_SWALE)
         return fail(FSM_ERR_ARG, "invalid m, w or metric");
+    if (metric == FSM_METRIC_SWALE && arg < 0)
+        return fail(FSM_ERR_ARG, "Matching tolerance must be non-negative.");
     if (n == 0) return FSM_OK;
     if (!std::isfinite(arg)) return fail(FSM_ERR_ARG, "Invalid finite metric parameter");
     for (int i = 0; i < m; i++)

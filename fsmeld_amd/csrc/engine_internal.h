@@ -6,7 +6,11 @@
 
 namespace fsm {
 
-enum Metric : int { DTW = 1, EDR = 2, ERP = 3, LCSS = 4, MSM = 5 };
+enum Metric : int { DTW = 1, EDR = 2, ERP = 3, LCSS = 4, MSM = 5, SWALE = 6 };
+// include/funcs.h:30-31: SWALE's cost of a match within eps (the reward r)
+// and of a mismatch or a gap (p)
+constexpr double kSwaleR = 1.0;
+constexpr double kSwaleP = 5.0;
 
 // src/ElasticMeasures.cpp:7 -- value of out-of-band DP cells.
 constexpr double kEldSentinel = 1e18;

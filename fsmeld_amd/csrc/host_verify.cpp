@@ -3,10 +3,11 @@
 // windows whose device distance lies within 1e-9 of a decision boundary.
 //
 // The recurrences are the reference's (src/ElasticMeasures.cpp: dtw :24-61,
-// lcss :63-93, erp :95-130, edr :132-165, msm :167-198), each cell with the
-// reference's expression and operand order, evaluated over full-width rows
-// whose cells outside the Sakoe-Chiba band keep the value the reference's
-// buffers hold there (1e18, LCSS 0).  x is the query, y the candidate.
+// lcss :63-93, erp :95-130, edr :132-165, msm :167-198, swale :200-231),
+// each cell with the reference's expression and operand order, evaluated
+// over full-width rows whose cells outside the Sakoe-Chiba band keep the
+// value the reference's buffers hold there (1e18, LCSS 0).  x is the query,
+// y the candidate.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -126,6 +127,23 @@ double edr(const double* x, const double* y, int m, int w, double eps) {
     return r.cur[m];
 }
 
+// SWALE, src/ElasticMeasures.cpp:200-231 with the costs of include/funcs.h:30-31
+double swale(const double* x, const double* y, int m, int w, double eps) {
+    constexpr double kP = 5.0, kR = 1.0;
+    Rows r(m + 1, w, kOut);
+    r.cur[0] = 0.0;
+    for (int j = 1; j <= std::min(m, w); j++) r.cur[j] = r.cur[j - 1] + kP;
+    for (int i = 1; i <= m; i++) {
+        r.next(kOut, i);
+        if (i <= w) r.cur[0] = r.prev[0] + kP;
+        for (int j = std::max(1, i - w); j <= std::min(m, i + w); j++) {
+            const double mv = (std::abs(x[i - 1] - y[j - 1]) <= eps) ? kR : kP;
+            r.cur[j] = min3(r.prev[j - 1] + mv, r.prev[j] + kP, r.cur[j - 1] + kP);
+        }
+    }
+    return r.cur[m];
+}
+
 double msm(const double* x, const double* y, int m, int w, double c) {
     Rows r(m, w, kOut);
     r.cur[0] = std::abs(x[0] - y[0]);
@@ -153,6 +171,7 @@ double host_eld(int metric, const double* x, const double* y, int m, int w, doub
         case FSM_METRIC_EDR: return edr(x, y, m, w, arg);
         case FSM_METRIC_LCSS: return lcss(x, y, m, w, arg);
         case FSM_METRIC_MSM: return msm(x, y, m, w, arg);
+        case FSM_METRIC_SWALE: return swale(x, y, m, w, arg);
     }
     return std::nan("");
 }

@@ -14,6 +14,7 @@ FSM_EXTERN(ERP)
 FSM_EXTERN(EDR)
 FSM_EXTERN(LCSS)
 FSM_EXTERN(MSM)
+FSM_EXTERN(SWALE)
 #undef FSM_EXTERN
 
 int64_t eld_scratch_threads() { return (int64_t)512 * 256; }
@@ -36,6 +37,7 @@ static hipError_t dispatch_eld(hipStream_t st, const VerifyArgs& a, const int64_
         case EDR: return FSM_M(EDR);
         case LCSS: return FSM_M(LCSS);
         case MSM: return FSM_M(MSM);
+        case SWALE: return FSM_M(SWALE);
     }
 #undef FSM_M
     return hipErrorInvalidValue;

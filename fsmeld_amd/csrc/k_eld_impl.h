@@ -144,6 +144,14 @@ __device__ inline double cell(int i, int j, double up, double left, double diag,
         }
         const double sub = (fabs(xp - yp) <= arg) ? 0.0 : 1.0;
         return dmin3(up + 1.0, left + 1.0, diag + sub);
+    } else if (METRIC == SWALE) {  // src/ElasticMeasures.cpp:200-231, p = 5, r = 1
+        if (EDGE) {
+            if (i == 0 && j == 0) return 0.0;
+            if (i == 0) return left + kSwaleP;
+            if (j == 0) return up + kSwaleP;
+        }
+        const double mv = (fabs(xp - yp) <= arg) ? kSwaleR : kSwaleP;
+        return dmin3(diag + mv, up + kSwaleP, left + kSwaleP);
     } else if (METRIC == LCSS) {
         double d1 = diag, u = up, l = left;
         if (EDGE) {
@@ -173,7 +181,8 @@ constexpr size_t kEldLdsCap = 64 * 1024;
 constexpr double kPad = 1e300;
 // idle lanes that trigger a wave's refill (the integer EDR / LCSS kernels'
 // iterations are cheaper, so their refills are spaced further apart)
-template <int METRIC> constexpr int kRefillMin = (METRIC == EDR || METRIC == LCSS) ? 12 : 8;
+template <int METRIC>
+constexpr int kRefillMin = (METRIC == EDR || METRIC == LCSS || METRIC == SWALE) ? 12 : 8;
 // queue positions a wave takes per atomic; refills draw on the wave's pool
 // (short-lived candidates would otherwise make one atomic per few lanes on
 // a single counter)
@@ -223,6 +232,12 @@ template <> struct EldState<LCSS> {
     using type = int;
     static constexpr int waves = 3;
 };
+// SWALE's values are sums of its integer costs (1 and 5): int slots hold D
+// itself, at most 10 m plus the sentinel's growth of 5 per anti-diagonal
+template <> struct EldState<SWALE> {
+    using type = int;
+    static constexpr int waves = 3;
+};
 template <int METRIC> using eld_t = typename EldState<METRIC>::type;
 constexpr int kEdrSentinel = 1 << 29;
 __device__ __forceinline__ int band_min(int a, int b) { return min(a, b); }
@@ -236,7 +251,7 @@ __device__ __forceinline__ double band_value(eld_t<METRIC> v, int d) {
 template <int METRIC>
 __device__ __forceinline__ eld_t<METRIC> band_sentinel() {
     if constexpr (METRIC == LCSS) return 0;
-    else if constexpr (METRIC == EDR) return kEdrSentinel;
+    else if constexpr (METRIC == EDR || METRIC == SWALE) return kEdrSentinel;
     else return kEldSentinel;
 }
 
@@ -256,6 +271,10 @@ __device__ __forceinline__ void step_a(int r, double arg, eld_t<METRIC>* A,
     } else if constexpr (METRIC == LCSS) {  // x[i] = X[r + 1], y[j] = Y[W - r + 1]
         const int hit = (fabs(X[r + 1] - Y[W - r + 1]) <= arg) ? 1 : 0;
         A[r] = max(A[r] + hit, max(up, left));
+        return;
+    } else if constexpr (METRIC == SWALE) {  // x[i-1] = X[r], y[j-1] = Y[W - r]
+        const int mv = (fabs(X[r] - Y[W - r]) <= arg) ? 1 : 5;
+        A[r] = min(min(up, left) + 5, A[r] + mv);
         return;
     }
     if (METRIC == MSM) {
@@ -279,6 +298,10 @@ __device__ __forceinline__ void step_b(int r, double arg, const eld_t<METRIC>* A
     } else if constexpr (METRIC == LCSS) {  // x[i] = X[r + 2], y[j] = Y[W - r + 1]
         const int hit = (fabs(X[r + 2] - Y[W - r + 1]) <= arg) ? 1 : 0;
         B[r] = max(B[r] + hit, max(A[r], A[r + 1]));
+        return;
+    } else if constexpr (METRIC == SWALE) {  // x[i-1] = X[r + 1], y[j-1] = Y[W - r]
+        const int mv = (fabs(X[r + 1] - Y[W - r]) <= arg) ? 1 : 5;
+        B[r] = min(min(A[r], A[r + 1]) + 5, B[r] + mv);
         return;
     }
     if (METRIC == MSM) {
@@ -317,7 +340,8 @@ __device__ __forceinline__ void ring_shift(double* Y, double yn) {
 // Two iterations per pass of the register kernel's loop (the ring moves by
 // two, the loop's control runs once) where the band update is short enough
 // for the per-iteration overhead to matter.
-template <int METRIC> constexpr bool kEldPair = (METRIC == EDR || METRIC == LCSS);
+template <int METRIC>
+constexpr bool kEldPair = (METRIC == EDR || METRIC == LCSS || METRIC == SWALE);
 
 // Per-sample lower bound on the cost of consuming candidate sample y whose
 // band sees the query envelope [lo, hi] (DTW, ERP, MSM); for EDR 1 if y is
@@ -331,7 +355,19 @@ __device__ inline double consume_lb(double y, double lo, double hi, double arg) 
     if (METRIC == ERP) return fmin(e * e, (y - arg) * (y - arg));
     if (METRIC == EDR) return (e > arg * (1.0 - 1e-12)) ? 1.0 : 0.0;
     if (METRIC == MSM) return fmin(e, arg);
+    // SWALE: every sample is consumed by a step that costs at least r
+    if (METRIC == SWALE) return kSwaleR;
     return (e <= arg * (1.0 - 1e-12)) ? 1.0 : 0.0;
+}
+
+// The bound total of the abandon tests below where no GLB pass handed one
+// over: nothing for the distance sums, every sample for LCSS's matches, and
+// m r for SWALE (each sample costs at least r, as consume_lb counts).
+template <int METRIC>
+__device__ inline double no_glb_total(int m) {
+    if (METRIC == LCSS) return (double)m;
+    if (METRIC == SWALE) return kSwaleR * (double)m;
+    return 0.0;
 }
 
 // Abandon test after iteration b (the two anti-diagonals in A and B).  total
@@ -425,7 +461,7 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
                                              unsigned long long* queue,
                                              unsigned long long* work, int64_t n_min) {
     extern __shared__ double xlds[];
-    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
+    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR || METRIC == SWALE);
     // samples 0 .. b + KOFF lie at or before the frontier's largest column
     // (ERP / EDR cells consume y[j - 1]); LCSS: its smallest column
     constexpr int KOFF = (METRIC == LCSS) ? 0 : (GRID1 ? W - 1 : W);
@@ -448,8 +484,9 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
     using VT = eld_t<METRIC>;
     const VT S = band_sentinel<METRIC>();
     // the virtual cell (-1, -1): 0, or -1 for EDR with eps < 0; EDR stores
-    // D - (i + j) = D + 2
-    const VT V = (METRIC == EDR) ? ((arg >= 0.0) ? 2 : 1) : 0;
+    // D - (i + j) = D + 2.  SWALE: -1, so that cell (0, 0), a match of pad
+    // against pad at cost r, is 0
+    const VT V = (METRIC == EDR) ? ((arg >= 0.0) ? 2 : 1) : ((METRIC == SWALE) ? -1 : 0);
 
     const int span = m + 2 * kXPad;
     double* xs = xlds + kXPad;
@@ -514,7 +551,7 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
                         g = c;
                         tp = rows + c * (int64_t)m;
                     }
-                    total = in_total ? in_total[c] : ((METRIC == LCSS) ? (double)m : 0.0);
+                    total = in_total ? in_total[c] : no_glb_total<METRIC>(m);
                     b = BS;
 #pragma unroll
                     for (int k = 0; k <= W; k++) A[k] = B[k] = S;
@@ -552,7 +589,7 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
         // of the iteration and keeps all 2W + 1 comparison masks live in
         // scalar registers, which then spill to vector lanes.
         auto pin = [&]() {
-            if constexpr (METRIC == EDR || METRIC == LCSS) {
+            if constexpr (METRIC == EDR || METRIC == LCSS || METRIC == SWALE) {
 #pragma unroll
                 for (int k = 0; k <= W; k++) {
                     asm volatile("" : "+v"(A[k]));
@@ -671,7 +708,7 @@ __device__ double eld_rows(const double* __restrict__ q, const double* __restric
                            const CandSrc& c, int m, int w, double arg, MemArr b, MemArr y,
                            double thr, double total, const double* __restrict__ LQ,
                            const double* __restrict__ UQ, bool* dead) {
-    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
+    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR || METRIC == SWALE);
     const int M = GRID1 ? m : m - 1;
     const int xoff = GRID1 ? 1 : 0;
     const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;
@@ -785,7 +822,7 @@ __global__ void __launch_bounds__(256) k_eld_rows(VerifyArgs a, const double* __
         MemArr bb{scratch + tid, stride * 2};
         MemArr yy{scratch + tid + stride, stride * 2};
         const double total =
-            (in_total && have) ? in_total[ci] : ((METRIC == LCSS) ? (double)m : 0.0);
+            (in_total && have) ? in_total[ci] : no_glb_total<METRIC>(m);
         bool dead = false;
         const double d = eld_rows<METRIC>(a.q.qn, a.q.XG, c, m, w, a.p.arg, bb, yy, thr, total,
                                           LQ, UQ, &dead);
@@ -835,6 +872,11 @@ __device__ __forceinline__ double cell_sel(int i, int j, double up, double left,
         const double sub = (fabs(xp - yp) <= arg) ? 0.0 : 1.0;
         const double r = dmin3(up + 1.0, left + 1.0, diag + sub);
         return i0 ? (double)j : (j0 ? (double)i : r);
+    } else if (METRIC == SWALE) {
+        const double mv = (fabs(xp - yp) <= arg) ? kSwaleR : kSwaleP;
+        const double e1 = left + kSwaleP, e2 = up + kSwaleP;
+        const double r = dmin3(diag + mv, e2, e1);
+        return i0 ? (j0 ? 0.0 : e1) : (j0 ? e2 : r);
     } else if (METRIC == LCSS) {
         return cell<LCSS, true>(i, j, up, left, diag, xi, xp, yj, yp, xg, yg, arg);
     } else {  // MSM: msm_cost in the form of msm_cell (the same bits)
@@ -860,7 +902,7 @@ template <int METRIC, int CPL>
 __device__ __forceinline__ void wave_step(int par, int d, int w, int M, int m, double arg,
                                           const double* xs, const double* ys, double* V,
                                           int& live_cells) {
-    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
+    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR || METRIC == SWALE);
     constexpr int Q = 2 * CPL;
     const double S = (METRIC == LCSS) ? 0.0 : kEldSentinel;
     const int lane = threadIdx.x & 63;
@@ -909,7 +951,7 @@ __device__ __forceinline__ bool wave_band(int w, int M, int m, double arg, const
                                           bool check, double thr, double total,
                                           const double* __restrict__ LQ,
                                           const double* __restrict__ UQ) {
-    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
+    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR || METRIC == SWALE);
     constexpr int Q = 2 * CPL;
     const int D = 2 * M;
     const int nl = (2 * w + Q) / Q;  // lanes holding slots 0 .. 2w
@@ -972,7 +1014,7 @@ __global__ void __launch_bounds__(64) k_eld_wave(VerifyArgs a, const double* __r
                                                  unsigned long long* queue,
                                                  unsigned long long* work, int64_t n_max) {
     extern __shared__ double wlds[];
-    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR);
+    constexpr bool GRID1 = (METRIC == ERP || METRIC == EDR || METRIC == SWALE);
     constexpr int Q = 2 * CPL;
     const int64_t n = in_count ? (int64_t)*in_count : n_direct;
     if (n >= n_max) return;  // many candidates: the register kernel takes them
@@ -1016,7 +1058,7 @@ __global__ void __launch_bounds__(64) k_eld_wave(VerifyArgs a, const double* __r
         for (int j = lane - 1; j <= m; j += 64)
             ys[j] = (j >= 0 && j < m) ? (raw ? (tp[j] - mu) / sd : tp[j]) : 0.0;
         __syncthreads();
-        const double total = in_total ? in_total[c] : ((METRIC == LCSS) ? (double)m : 0.0);
+        const double total = in_total ? in_total[c] : no_glb_total<METRIC>(m);
         double V[Q];
 #pragma unroll
         for (int q = 0; q < Q; q++) V[q] = S;

@@ -25,6 +25,9 @@ __device__ inline double mx(double x, double y) { return (x > y) ? x : y; }    /
 __device__ inline double edr_subcost(double a, double b, double eps) {          // src/glb.cpp:8-10
     return (fabs(a - b) <= eps) ? 0.0 : 1.0;
 }
+__device__ inline double swale_subcost(double a, double b, double eps) {         // src/glb.cpp:11-14
+    return (fabs(a - b) <= eps) ? kSwaleR : kSwaleP;
+}
 __device__ inline bool env_match(double v, double lo, double hi, double eps) {  // src/glb.cpp:17-22
     if (v < lo) return lo - v <= eps;
     if (v > hi) return v - hi <= eps;
@@ -106,6 +109,7 @@ __device__ inline double glb_boundary(double T0, double Tl, double T2, const dou
     if (METRIC == DTW) return sqd(T0, Q[0]) + sqd(Tl, Q[b]);
     if (METRIC == ERP) return smin3(sqd(Q[b], Tl), sqd(Q[b], arg), sqd(Tl, arg));
     if (METRIC == EDR) return mn(edr_subcost(Q[b], Tl, arg), 1.0);
+    if (METRIC == SWALE) return mn(swale_subcost(Q[b], Tl, arg), kSwaleP);  // :282-286
     if (METRIC == MSM) {
         if ((T2 >= Tl && Tl >= Q[b]) || (T2 <= Tl && Tl <= Q[b]) ||
             (Q[b - 1] <= Q[b] && Q[b] <= Tl) || (Q[b - 1] >= Q[b] && Q[b] >= Tl))
@@ -157,6 +161,15 @@ __device__ double glb_one(const double* __restrict__ t, double mu, double sd,
             const double v = N.exact(t[i]);
             if (v > UQ[i]) t2q += edr_subcost(v, UQ[i], arg);
             else if (v < LQ[i]) t2q += edr_subcost(v, LQ[i], arg);
+            if (boundary + t2q > bsf) return kGlbAbandon;
+        }
+    } else if (METRIC == SWALE) {  // src/glb.cpp:282-297
+        if (n > 2 && boundary + t2q > bsf) return kGlbAbandon;
+        for (int i = 1; i < n - 1; ++i) {
+            if (surely_inside(N.approx(t[i]), LQ[i], UQ[i])) continue;
+            const double v = N.exact(t[i]);
+            if (v > UQ[i]) t2q += swale_subcost(v, UQ[i], arg);
+            else if (v < LQ[i]) t2q += swale_subcost(v, LQ[i], arg);
             if (boundary + t2q > bsf) return kGlbAbandon;
         }
     } else if (METRIC == LCSS) {
@@ -220,6 +233,10 @@ __device__ double glb_one(const double* __restrict__ t, double mu, double sd,
         } else if (METRIC == EDR) {
             if (q > UT) q2t += edr_subcost(q, UT, arg);
             else if (q < LT) q2t += edr_subcost(q, LT, arg);
+            if (boundary + q2t > bsf) return kGlbAbandon;
+        } else if (METRIC == SWALE) {  // src/glb.cpp:356-369
+            if (q > UT) q2t += swale_subcost(q, UT, arg);
+            else if (q < LT) q2t += swale_subcost(q, LT, arg);
             if (boundary + q2t > bsf) return kGlbAbandon;
         } else {  // MSM
             if (q > UT) q2t += mn(fabs(q - UT), arg);
@@ -559,7 +576,9 @@ k_glb(VerifyArgs a, const double* __restrict__ Q,
     const int64_t n = (int64_t)*in_count;
     // the fast decision needs contributions >= 0 and, for EDR / LCSS, an
     // envelope that counts as a match (eps > 0); values (vals) need the exact path
-    const bool fast = vals == nullptr && a.p.cutoff < kGlbAbandon && a.p.m > 2 &&
+    // (SWALE takes the exact path only)
+    const bool fast = METRIC != SWALE && vals == nullptr && a.p.cutoff < kGlbAbandon &&
+                      a.p.m > 2 &&
                       ((METRIC == DTW || METRIC == ERP) || (METRIC == MSM && a.p.arg >= 0.0) ||
                        a.p.arg > 0.0);
     // Blocks take runs of blockDim.x consecutive candidates from the queue
@@ -578,8 +597,11 @@ k_glb(VerifyArgs a, const double* __restrict__ Q,
         const double mu = active ? a.MU[si] : 0.0;
         const double sd = active ? a.SG[si] : 1.0;
         int verdict = 0;
-        // without the fast pass's bound: nothing (distances), every sample (LCSS)
-        double total = (METRIC == LCSS) ? (double)a.p.m : 0.0;
+        // without the fast pass's bound: nothing (distances), every sample
+        // (LCSS), r per sample (SWALE: every sample costs at least r)
+        double total = (METRIC == LCSS)    ? (double)a.p.m
+                       : (METRIC == SWALE) ? kSwaleR * (double)a.p.m
+                                           : 0.0;
         // When the block's candidate windows span at most stage_len - m
         // windows, their samples and envelope are read once per block into
         // LDS (the passes read each value up to m times).  The list is
@@ -667,6 +689,7 @@ hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
         case EDR: FSM_GLB(EDR); break;
         case LCSS: FSM_GLB(LCSS); break;
         case MSM: FSM_GLB(MSM); break;
+        case SWALE: FSM_GLB(SWALE); break;
         default: return hipErrorInvalidValue;
     }
 #undef FSM_GLB

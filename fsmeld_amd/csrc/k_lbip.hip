@@ -143,6 +143,11 @@ __device__ double h_upper(const QueryTables& Q, int metric, int m, double C, dou
             const double mid = (Q.S1[i2] - Q.S1[lo]) - (double)(i2 - lo) * t;
             return C * (double)(m - i2) + mid;
         }
+        case SWALE: {  // src/find_t_lp.cpp:827-927: p per q - eps > t, r per q >= t beyond
+            const int big = m - upper_bound(Q.TU, m, t);   // fl(q - eps) > t
+            const int ge = m - lower_bound(Q.qs, m, t);    // q >= t (includes big)
+            return kSwaleP * (double)big + kSwaleR * (double)(ge - big);
+        }
         default:  // EDR, LCSS
             return (double)(m - upper_bound(Q.TU, m, t));
     }
@@ -172,6 +177,11 @@ __device__ double h_lower(const QueryTables& Q, int metric, int m, double C, dou
             const double mid = (Q.S1[hi] - Q.S1[is]) - (double)(hi - is) * t;
             return C * (double)is + fabs(mid);
         }
+        case SWALE: {  // src/find_t_lp.cpp:399-509: p per q + eps < t, r per q <= t beyond
+            const int small = lower_bound(Q.TL, m, t);     // fl(q + eps) < t
+            const int le = upper_bound(Q.qs, m, t);        // q <= t (includes small)
+            return kSwaleP * (double)small + kSwaleR * (double)(le - small);
+        }
         default:
             return (double)lower_bound(Q.TL, m, t);
     }
@@ -197,6 +207,12 @@ __device__ inline void penalties(int metric, double q, double tl, double tu, dou
             if (q > tu)
                 vu = ((q - tu) * (q - tu) > (q - C) * (q - C)) ? (q - C) * (q - C)
                                                                 : (q - tu) * (q - tu);
+            break;
+        case SWALE:  // src/q_to_h.cpp:60-82, :128-150
+            if (q < tl - C) vl = kSwaleP;
+            else if (q < tl) vl = kSwaleR;
+            if (q > tu + C) vu = kSwaleP;
+            else if (q > tu) vu = kSwaleR;
             break;
         default:  // EDR, LCSS
             if (q < tl - C) vl = 1.0;

@@ -69,8 +69,9 @@ extern "C" int fsm_eld_pairs_noalloc(int metric, int m, int w, double arg, const
     using namespace fsm;
     if (n < 0 || (n > 0 && (!q || !cands || !out)))
         return fail(FSM_ERR_ARG, "null device pointer or negative count");
-    if (m < 2 || w < 0 || w >= m || metric < FSM_METRIC_DTW || metric > FSM_METRIC_MSM)
-        return fail(FSM_ERR_ARG, "invalid m, w or metric");
+    if (m < 2 || w < 0 || w >= m || metric < FSM_METRIC_DTW || metric > FSM_METRIC_SWALE ||
+        (metric == FSM_METRIC_SWALE && arg < 0))
+        return fail(FSM_ERR_ARG, "invalid m, w, metric or tolerance");
     if (!std::isfinite(arg)) return fail(FSM_ERR_ARG, "Invalid finite metric parameter");
     if (m > FSM_NOALLOC_MAX_M)
         return fail(FSM_ERR_ARG, "m exceeds FSM_NOALLOC_MAX_M, the scratch array's query length");

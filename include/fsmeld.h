@@ -38,6 +38,10 @@ extern "C" {
 #define FSM_METRIC_ERP 3
 #define FSM_METRIC_LCSS 4
 #define FSM_METRIC_MSM 5
+/* SWALE (include/funcs.h:17, src/ElasticMeasures.cpp:200-231): match cost 1
+ * within eps, mismatch and gap cost 5; implemented by the reference but
+ * rejected by its command line (src/main.cpp:57-59), so only this ABI takes it */
+#define FSM_METRIC_SWALE 6
 
 /* query modes: the reference's three commands (src/main.cpp:90-104) */
 #define FSM_MODE_SEARCH 0 /* FSMELD()                 include/funcs.h:67-68 */
@@ -57,7 +61,7 @@ typedef struct {
     int metric;            /* FSM_METRIC_* */
     int m;                 /* query / window length, m >= 2 */
     int w;                 /* Sakoe-Chiba radius, 0 <= w < m */
-    double arg;            /* ERP gap g, EDR/LCSS epsilon, MSM cost c; ignored by DTW */
+    double arg;            /* ERP gap g, EDR/LCSS/SWALE epsilon, MSM cost c; ignored by DTW */
     double cutoff;         /* search / exact: distance cutoff xi >= 0 */
     int k;                 /* cutoff / exact top-k: rank k >= 1 */
     int64_t exclude_begin; /* candidate exclusion [begin, end); -1 = none */

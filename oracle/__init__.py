@@ -28,7 +28,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(_HERE, "liboracle.so")
 REF_ELD = os.path.join(_HERE, "_ref", "libref_eld.so")
 
-METRICS = {"dtw": 1, "edr": 2, "erp": 3, "lcss": 4, "msm": 5}
+METRICS = {"dtw": 1, "edr": 2, "erp": 3, "lcss": 4, "msm": 5, "swale": 6}
 
 _lib = ctypes.CDLL(LIB)
 _D = ctypes.POINTER(ctypes.c_double)

@@ -37,7 +37,9 @@
 
 namespace {
 
-enum { M_DTW = 1, M_EDR = 2, M_ERP = 3, M_LCSS = 4, M_MSM = 5 };
+enum { M_DTW = 1, M_EDR = 2, M_ERP = 3, M_LCSS = 4, M_MSM = 5, M_SWALE = 6 };
+// include/funcs.h:30-31: SWALE's match reward and mismatch / gap cost
+constexpr double kSwaleR = 1.0, kSwaleP = 5.0;
 
 constexpr double kEldSentinel = 1e18;        // src/ElasticMeasures.cpp:7
 constexpr double kGlbAbandon = 1008610086.0; // include/def.h:7
@@ -193,7 +195,8 @@ void sort_runs(std::vector<Unique>& R, int nr, bool ascending) {
         std::sort(R.begin(), mid, [](const Unique& a, const Unique& b) { return a.value > b.value; });
 }
 
-double D_cost(double q, double C, int metric) {  // src/find_t_lp.cpp:148-159
+double D_cost(double q, double C, int metric) {  // src/find_t_lp.cpp:148-166
+    if (metric == M_SWALE) return kSwaleP;
     if (metric == M_MSM) return C;
     if (metric == M_ERP || metric == M_DTW) return (q - C) * (q - C);
     if (metric == M_EDR || metric == M_LCSS) return 1;
@@ -202,16 +205,17 @@ double D_cost(double q, double C, int metric) {  // src/find_t_lp.cpp:148-159
 double tau_u(double q, double C, int metric) {  // src/find_t_lp.cpp:167-176
     if (metric == M_MSM) return q - C;
     if (metric == M_ERP) return q - std::abs(q - C);
-    if (metric == M_EDR || metric == M_LCSS) return q - C;
+    if (metric == M_EDR || metric == M_LCSS || metric == M_SWALE) return q - C;
     return q;
 }
 double tau_l(double q, double C, int metric) {  // src/find_t_lp.cpp:177-186
     if (metric == M_MSM) return q + C;
     if (metric == M_ERP) return q + std::abs(q - C);
-    if (metric == M_EDR || metric == M_LCSS) return q + C;
+    if (metric == M_EDR || metric == M_LCSS || metric == M_SWALE) return q + C;
     return q;
 }
-double f_p(const double* mp, int p, int metric, double t) {  // src/find_t_lp.cpp:187-199
+double f_p(const double* mp, int p, int metric, double t) {  // src/find_t_lp.cpp:187-204
+    if (metric == M_SWALE) return mp[0] * kSwaleR;
     if (p == 0) {
         if (metric == M_LCSS || metric == M_EDR) return 0;
         return mp[0];
@@ -297,7 +301,7 @@ double find_tu_dtw(const double* U, int l, const double* qs, int m, int p) {
     return R[first_max(LB.data(), nr)].value;
 }
 
-// src/find_t_lp.cpp:288-398 (MSM, EDR, LCSS)
+// src/find_t_lp.cpp:288-398 (MSM, EDR, LCSS); :399-509 (SWALE, the same sweep)
 double find_tl_generic(const double* L, int l, const double* qs, int m, int p, double C_,
                        int metric) {
     const int n = l - 2 * m + 2;
@@ -349,7 +353,7 @@ double find_tl_generic(const double* L, int l, const double* qs, int m, int p, d
     return R[first_max(LB.data(), nr)].value;
 }
 
-// src/find_t_lp.cpp:726-825 (MSM, EDR, LCSS)
+// src/find_t_lp.cpp:726-825 (MSM, EDR, LCSS); :827-927 (SWALE, the same sweep)
 double find_tu_generic(const double* U, int l, const double* qs, int m, int p, double C_,
                        int metric) {
     const int n = l - 2 * m + 2;
@@ -517,7 +521,7 @@ void thresholds(int metric, const double* Un, const double* Ln, int l, const dou
     }
 }
 
-// src/q_to_h.cpp:11-134 (SWALE overloads out of scope).
+// src/q_to_h.cpp:11-134; SWALE :60-82, :128-150.
 void q_to_h(int metric, const double* q, int m, double tl, double tu, double C,
             double* hl, double* hu) {
     for (int i = 0; i < m; i++) {
@@ -543,6 +547,12 @@ void q_to_h(int metric, const double* q, int m, double tl, double tu, double C,
             case M_LCSS:
                 if (q[i] < tl - C) vl = 1;
                 if (q[i] > tu + C) vu = 1;
+                break;
+            case M_SWALE:
+                if (q[i] < tl - C) vl = kSwaleP;
+                else if (q[i] < tl) vl = kSwaleR;
+                if (q[i] > tu + C) vu = kSwaleP;
+                else if (q[i] > tu) vu = kSwaleR;
                 break;
         }
         hl[i] = vl;
@@ -590,6 +600,10 @@ bool lbip_exceeds_cutoff(double bound, double cutoff, int metric) {
 // src/glb.cpp:8-31 helpers and :200-373 glb_elastic.
 inline double edr_subcost(double a, double b, double eps) {
     return (std::abs(a - b) <= eps) ? 0.0 : 1.0;
+}
+// src/glb.cpp:11-14: the match reward within eps, else the mismatch cost
+inline double swale_subcost(double a, double b, double eps) {
+    return (std::abs(a - b) <= eps) ? kSwaleR : kSwaleP;
 }
 inline bool env_match(double v, double lo, double hi, double eps) {
     if (v < lo) return lo - v <= eps;
@@ -664,6 +678,16 @@ double glb(int metric, const double* Q, int lenQ, const double* T, int lenT, dou
             }
             break;
         }
+        case M_SWALE: {  // src/glb.cpp:282-297
+            const double e = swale_subcost(Q[lenQ - 1], T[lenT - 1], arg);
+            boundary = (e < kSwaleP) ? e : kSwaleP;
+            for (int i = 1; i < lenT - 1; ++i) {
+                if (T[i] > UQ[i]) t2q += swale_subcost(T[i], UQ[i], arg);
+                else if (T[i] < LQ[i]) t2q += swale_subcost(T[i], LQ[i], arg);
+                if (boundary + t2q > bsf) return kGlbAbandon;
+            }
+            break;
+        }
         default:
             return kGlbAbandon;
     }
@@ -721,6 +745,15 @@ double glb(int metric, const double* Q, int lenQ, const double* T, int lenT, dou
                     const double d = std::abs(Q[i] - LT[i]);
                     q2t += (d < arg) ? d : arg;
                 }
+                if (boundary + q2t > bsf) return kGlbAbandon;
+            }
+            return boundary + ((t2q > q2t) ? t2q : q2t);
+        }
+        case M_SWALE: {  // src/glb.cpp:356-369
+            double q2t = 0.0;
+            for (int i = 1; i < lenQ - 1; ++i) {
+                if (Q[i] > UT[i]) q2t += swale_subcost(Q[i], UT[i], arg);
+                else if (Q[i] < LT[i]) q2t += swale_subcost(Q[i], LT[i], arg);
                 if (boundary + q2t > bsf) return kGlbAbandon;
             }
             return boundary + ((t2q > q2t) ? t2q : q2t);
@@ -894,9 +927,33 @@ double eld_msm(const double* x, const double* y, int m, double c, int w) {
     return R.prev[w];
 }
 
+// src/ElasticMeasures.cpp:200-231 (x = query, y = candidate, p = 5, r = 1):
+// the EDR grid with gap cost p and diagonal cost r within eps, else p.
+double eld_swale(const double* x, const double* y, int m, double eps, int w) {
+    if (eps < 0.0) return std::numeric_limits<double>::quiet_NaN();
+    BandRows& R = g_rows;
+    R.init(2 * w + 1, kEldSentinel);
+    R.prev[w] = 0.0;
+    for (int j = 1; j <= std::min(m, w); ++j) R.prev[j + w] = R.prev[j - 1 + w] + kSwaleP;
+    for (int i = 1; i <= m; ++i) {
+        R.clear_cur(kEldSentinel);
+        double* C = R.cur;
+        const double* P = R.prev;
+        if (i <= w) C[w - i] = P[w - i + 1] + kSwaleP;
+        const int minw = std::max(1, i - w), maxw = std::min(m, i + w);
+        for (int j = minw, k = minw - i + w; j <= maxw; ++j, ++k) {
+            const double mv = (std::abs(x[i - 1] - y[j - 1]) <= eps) ? kSwaleR : kSwaleP;
+            C[k] = min3(P[k] + mv, P[k + 1] + kSwaleP, C[k - 1] + kSwaleP);
+        }
+        R.swap();
+    }
+    return R.prev[w];
+}
+
 // src/fsmeld.cpp:170-183
 double eld(int metric, const double* q, const double* t, int m, int w, double arg) {
     switch (metric) {
+        case M_SWALE: return eld_swale(q, t, m, arg, w);
         case M_MSM: return eld_msm(q, t, m, arg, w);
         case M_ERP: return eld_erp(q, t, m, arg, w);
         case M_LCSS: return eld_lcss(q, t, m, w, arg);
@@ -1111,7 +1168,7 @@ void verify_all(const double* T, int64_t N, const Geometry& G, const Query& Q, i
 
 int check_args(int64_t N, int m, int metric, int w) {
     if (m < 2 || w < 0 || w >= m) return -1;
-    if (metric < M_DTW || metric > M_MSM) return -1;
+    if (metric < M_DTW || metric > M_SWALE) return -1;
     if (N < 2LL * m - 1) return -1;
     return 0;
 }

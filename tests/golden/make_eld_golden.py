@@ -17,7 +17,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 import oracle  # noqa: E402
 
-ARGS = {1: [0.0], 3: [0.0, 0.7], 2: [0.0, 0.25, 1.5], 4: [0.0, 0.25, 1.5], 5: [0.0, 0.5, 3.0]}
+ARGS = {1: [0.0], 3: [0.0, 0.7], 2: [0.0, 0.25, 1.5], 4: [0.0, 0.25, 1.5], 5: [0.0, 0.5, 3.0],
+        6: [0.0, 0.25, 1.5]}
 
 
 def main():
@@ -27,7 +28,7 @@ def main():
     rng = np.random.default_rng(20261203)
     P = ctypes.POINTER(ctypes.c_double)
     cases = []
-    for metric in (1, 2, 3, 4, 5):
+    for metric in (1, 2, 3, 4, 5, 6):   # 6: SWALE (appended: earlier cases unchanged)
         for arg in ARGS[metric]:
             for _ in range(12):
                 m = int(rng.integers(2, 25))

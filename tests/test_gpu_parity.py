@@ -14,7 +14,8 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-METRICS = [("dtw", 0.0), ("erp", 0.0), ("edr", 0.25), ("lcss", 0.25), ("msm", 0.5)]
+METRICS = [("dtw", 0.0), ("erp", 0.0), ("edr", 0.25), ("lcss", 0.25), ("msm", 0.5),
+           ("swale", 0.25)]
 
 
 def bits(x):

@@ -42,9 +42,10 @@ def test_host_eld_equals_oracle(host):
     for trial in range(2500):
         m = int(rng.integers(2, 70))
         w = int(rng.integers(0, m))
-        metric = int(rng.integers(1, 6))
+        metric = int(rng.integers(1, 7))
         arg = {1: 0.0, 2: float(rng.uniform(0, 1)), 3: float(rng.normal()),
-               4: float(rng.uniform(0, 1)), 5: float(rng.uniform(0, 2))}[metric]
+               4: float(rng.uniform(0, 1)), 5: float(rng.uniform(0, 2)),
+               6: float(rng.uniform(0, 1))}[metric]
         q = np.round(rng.normal(size=m), 1 if trial % 4 == 0 else 12)
         t = np.round(rng.normal(size=m), 1) if trial % 3 else q + rng.normal(size=m) * 0.02
         t = np.ascontiguousarray(t)

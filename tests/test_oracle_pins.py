@@ -310,3 +310,51 @@ def test_raw_mode_is_the_cascade_on_unnormalised_values(metric, arg):
     # the mode matters: z-normalised distances are not these
     zd = O.exact(t, q, m, 1e300, metric, w, arg)[1]
     assert not np.array_equal(zd[: d.size], d[: zd.size])
+
+
+# ------------------------------------------------------------------ SWALE ---
+def test_swale_closed_forms():
+    """SWALE (src/ElasticMeasures.cpp:200-231; match within eps 1, mismatch or
+    gap 5): identical series cost m (all diagonal matches); with w = 0 the
+    path is the diagonal, sum of 1 per matching and 5 per other pair; a
+    brute-force full DP from the definition equals the banded one at
+    w = m - 1; and no path beats 1 per step of the longer series."""
+    rng = np.random.default_rng(31)
+    for m in (2, 5, 17, 40):
+        q = rng.normal(size=m)
+        t = q + rng.normal(size=m) * 0.3
+        assert O.eld("swale", q, q, int(rng.integers(0, m)), 0.1) == m
+        diag = np.where(np.abs(q - t) <= 0.2, 1.0, 5.0).sum()
+        assert O.eld("swale", q, t, 0, 0.2) == diag
+        D = np.full((m + 1, m + 1), np.inf)
+        D[0, 0] = 0.0
+        for i in range(m + 1):
+            for j in range(m + 1):
+                if i == j == 0:
+                    continue
+                best = np.inf
+                if i and j:
+                    best = D[i - 1, j - 1] + (1.0 if abs(q[i - 1] - t[j - 1]) <= 0.2 else 5.0)
+                if i:
+                    best = min(best, D[i - 1, j] + 5.0)
+                if j:
+                    best = min(best, D[i, j - 1] + 5.0)
+                D[i, j] = best
+        assert O.eld("swale", q, t, m - 1, 0.2) == D[m, m]
+        assert O.eld("swale", q, t, m // 3, 0.2) >= m
+
+
+def test_swale_glb_known_answers_and_bound():
+    # Case A, eps = 0.5: boundary min(sub(1, 0), 5) = 5 (|1 - 0| > .5); T -> Q at
+    # T2 = 3 above U_Q2 = 2 by 1 > .5: 5; Q -> T nothing -> 5 + 5
+    assert oracle_glb("swale", A[0], A[1], 0.5, 1) == 10.0
+    # eps = 1: the boundary and the T -> Q term match within eps: 1 + 1
+    assert oracle_glb("swale", A[0], A[1], 1.0, 1) == 2.0
+    # Case B, eps = .5: boundary 5; Q -> T at Q1 = 3 above U_T1 = 1 by 2: 5 -> 10
+    assert oracle_glb("swale", B[0], B[1], 0.5, 1) == 10.0
+    rng = np.random.default_rng(32)
+    for _ in range(300):
+        m = int(rng.integers(3, 14))
+        w = int(rng.integers(0, m))
+        q, t = rng.normal(size=m), rng.normal(size=m)
+        assert oracle_glb("swale", q, t, 0.3, w) <= O.eld("swale", q, t, w, 0.3)

@@ -9,7 +9,8 @@ import pytest
 import fsmeld_amd as F
 import oracle as O
 
-METRICS = [("dtw", 0.0), ("erp", 0.0), ("edr", 0.25), ("lcss", 0.25), ("msm", 0.5)]
+METRICS = [("dtw", 0.0), ("erp", 0.0), ("edr", 0.25), ("lcss", 0.25), ("msm", 0.5),
+           ("swale", 0.25)]
 
 
 # ---------------------------------------------------------------- mvmean ---
@@ -94,6 +95,8 @@ def h_upper(metric, q, t, C):
         return sum(min((x - t) ** 2, (x - C) ** 2) for x in q if x > t)
     if metric == "msm":
         return sum(min(x - t, C) for x in q if x > t)
+    if metric == "swale":   # p = 5 beyond eps, r = 1 within it (find_t_lp.cpp:827-927)
+        return sum(5 if x - C > t else 1 for x in q if x >= t)
     return sum(1 for x in q if x - C > t)
 
 
@@ -104,6 +107,8 @@ def h_lower(metric, q, t, C):
         return sum(min((x - t) ** 2, (x - C) ** 2) for x in q if x < t)
     if metric == "msm":
         return sum(min(t - x, C) for x in q if x < t)
+    if metric == "swale":   # find_t_lp.cpp:399-509
+        return sum(5 if x + C < t else 1 for x in q if x <= t)
     return sum(1 for x in q if x + C < t)
 
 
@@ -153,6 +158,10 @@ def test_q_to_h_known_answers():
     for metric in ("edr", "lcss"):
         hl, hu = O.q_to_h(metric, q, 0.5, 1.0, 0.25)
         assert hl.tolist() == [1.0, 1.0, 0.0] and hu.tolist() == [0.0, 0.0, 1.0]
+    # SWALE eps=0.75: 5 beyond t -/+ eps, 1 between t and t -/+ eps, else 0
+    # (q = -2: below 0.5 - 0.75; q = 0: in [-0.25, 0.5); q = 1.5: in (1, 1.75])
+    hl, hu = O.q_to_h("swale", q, 0.5, 1.0, 0.75)
+    assert hl.tolist() == [5.0, 1.0, 0.0] and hu.tolist() == [0.0, 0.0, 1.0]
 
 
 def test_sip_direct_is_a_correlation():
@@ -191,7 +200,9 @@ def test_glb_is_a_lower_bound_of_eld(metric, arg):
         g = O.glb(metric, q, t, arg, LQ, UQ, w, 1e300)
         d = O.eld(metric, q, t, w, arg)
         assert g <= d + 1e-12, (trial, g, d)
-        assert O.glb(metric, q, q, arg, LQ, UQ, w, 1e300) == 0.0
+        # identical series: nothing outside the envelopes; SWALE's boundary
+        # still charges its match cost r = 1 for the last pair
+        assert O.glb(metric, q, q, arg, LQ, UQ, w, 1e300) == (1.0 if metric == "swale" else 0.0)
     # early abandon returns the sentinel of include/def.h:7
     m, w = 40, 4
     q = z(np.cumsum(rng.normal(size=m)))
@@ -217,7 +228,9 @@ def test_eld_special_cases():
         m = int(rng.integers(2, 40))
         q, t = rng.normal(size=m), rng.normal(size=m)
         for metric, arg in METRICS:
-            assert O.eld(metric, q, q, int(rng.integers(0, m)), arg) == 0.0
+            # identical series: 0, except SWALE, whose matches cost r = 1 each
+            want = float(m) if metric == "swale" else 0.0
+            assert O.eld(metric, q, q, int(rng.integers(0, m)), arg) == want
         # w = 0 reduces every measure to its lock-step form
         assert np.isclose(O.eld("dtw", q, t, 0, 0.0), np.sqrt(np.sum((q - t) ** 2)), rtol=1e-12)
         assert np.isclose(O.eld("erp", q, t, 0, 0.3), np.sqrt(np.sum((q - t) ** 2)), rtol=1e-12)

@@ -33,9 +33,10 @@ def test_oracle_eld_matches_reference_build_random():
     for trial in range(1500):
         m = int(rng.integers(2, 90))
         w = int(rng.integers(0, m))
-        metric = int(rng.integers(1, 6))
+        metric = int(rng.integers(1, 7))   # 6: SWALE
         arg = {1: 0.0, 2: float(rng.uniform(0, 1)), 3: float(rng.normal()),
-               4: float(rng.uniform(0, 1)), 5: float(rng.uniform(0, 2))}[metric]
+               4: float(rng.uniform(0, 1)), 5: float(rng.uniform(0, 2)),
+               6: float(rng.uniform(0, 1))}[metric]
         q = rng.normal(size=m)
         t = rng.normal(size=m) if trial % 3 else q + rng.normal(size=m) * 0.02
         want = ref.ref_eld(metric, q.ctypes.data_as(P), t.ctypes.data_as(P), m, w, arg)
