@@ -29,8 +29,9 @@ samples, k_m_l windows each):
                    window slot out (the two running sums)
   k_window_finish  16 B in, 17 B out per window slot (mean, std, flag)
   k_envelope       8 N bytes in, 16 N out (lower and upper envelope)
-  k_lbip           ts_n x (8 m_k / k_m_l + 8) bytes (target with segment
-                   overlap in, one FP64 bound out)
+  k_lbip           17 B in per window slot (mean, std, flag), 16 B in per
+                   segment sample (lower and upper envelope), one FP64
+                   bound out per window
   k_glb<M>         LB_IP survivors x 3 m FP64 operations (two directional
                    sums over the window), against the FP64 VALU roof
   k_eld<M>         band cells the kernel evaluates for live candidates x the
@@ -232,7 +233,7 @@ def main():
         8.0 * G["S"] * G["m_k"] + 16.0 * slots)
     hbm("k_window_finish", stage["stats_finish"], 33.0 * slots)
     hbm("k_envelope", stage["env"], 24.0 * n)
-    hbm("k_lbip", stage["lbip"], G["ts_n"] * (8.0 * G["m_k"] / G["k_m_l"] + 8.0))
+    hbm("k_lbip", stage["lbip"], 17.0 * slots + 16.0 * G["S"] * G["m_k"] + 8.0 * G["ts_n"])
     band = m * (2 * w + 1) - w * (w + 1)
     for metric, _ in cfg["metrics"]:
         code = METRIC_CODE[metric]

@@ -2,8 +2,11 @@
 //
 // Per segment (src/fsmeld.cpp:756-808):
 //   1. min/max of window mean and std over the windows containing each
-//      position (min_max_window, src/lower_upper_lemire.cpp:94-126): van Herk
-//      prefix/suffix scans over blocks of m windows;
+//      position (min_max_window, src/lower_upper_lemire.cpp:94-126): per tile
+//      of 64 positions, wave scans of the window runs at the tile's ends and
+//      the extremes of the whole 64-window chunks between (segments up to
+//      16384 positions, everything in LDS); van Herk prefix/suffix scans over
+//      blocks of m windows in global scratch for longer segments;
 //   2. centred envelope L/U of the raw segment (lower_upper_lemire, :8-44),
 //      taken from the target-wide envelope except within w of the edges;
 //   3. normalised envelopes (src/lb_normalize.cpp:9-32);
@@ -13,10 +16,11 @@
 //      hold the maximum, from a sort of those values only: select_threshold);
 //   5. penalties h (src/q_to_h.cpp) in exact 64-bit fixed point, rounded
 //      down, and their prefix sums CH;
-//   6. masks g (src/lb.cpp) as bitmaps built with wavefront ballots;
+//   6. masks g (src/lb.cpp) as bitmaps built with wavefront ballots, and
+//      the ascending lists of their transitions;
 //   7. per window j: z = sum over runs [a, b) of set mask bits inside
-//      [j, j+m) of CH[b-j] - CH[a-j] (walked as the mask's transitions:
-//      trans_sum) -- the sliding inner product of
+//      [j, j+m) of CH[b-j] - CH[a-j] (walked as the mask's transitions, a
+//      wave's 64 windows together: list_sum) -- the sliding inner product of
 //      src/sip.cpp evaluated exactly, so the bound is a certified lower bound
 //      of the FP64 penalties; then trans (src/fsmeld.cpp:84-106).
 // Min / max / compare steps are exact, so for NaN-free segments steps 1-3
@@ -28,8 +32,6 @@
 
 namespace fsm {
 
-constexpr int kLdsKeys = 8192;
-
 __host__ __device__ inline int npad_of(int n) {
     int p = 1;
     while (p < n) p <<= 1;
@@ -38,43 +40,30 @@ __host__ __device__ inline int npad_of(int n) {
 
 // Workgroup size per segment: 128 threads up to 1024 keys when there are
 // enough segments to fill the device with two-wave workgroups (short segments
-// are then dominated by the workgroup's barriers), 256 up to 4096, 1024
+// are then dominated by the workgroup's barriers), 256 up to 1024, 512 up to
+// 4096 (two workgroups and 16 waves per CU with both envelopes in LDS), 1024
 // beyond, so that long segments spread their per-position work over more
-// waves.
+// waves.  (npad, the keys' power of two, equals m_k.)
 inline int lbip_threads(int npad, int64_t segments) {
-    if (npad <= 1024 && segments >= 4096) return 128;
-    return npad <= 4096 ? 256 : 1024;
+    if (npad <= 1024) return segments >= 4096 ? 128 : 256;
+    return npad <= 4096 ? 512 : 1024;
 }
 
-// The segment's sort keys live in LDS up to 8192 of them, 16384 with a
-// 1024-thread workgroup (one per CU: 128 KiB of its 160, and 24 KiB for the
-// threshold buckets), else in the block's global scratch.  The penalty prefix sums and mask bitmaps of the
-// later steps reuse the keys' LDS space: the keys are dead once both
-// thresholds are known.
-__host__ __device__ inline bool keys_in_lds(int npad, int nt) {
-    return npad <= (nt >= 1024 ? 16384 : kLdsKeys);
-}
-
-size_t lbip_slab_bytes(const Geometry& G) {
-    const int nthr = G.m_k - 2 * G.m + 2;
-    const int npad = npad_of(nthr);
-    size_t b = sizeof(double) * (8 * (size_t)G.kml + 2 * (size_t)G.m_k);
-    if (!keys_in_lds(npad, lbip_threads(npad, G.S))) b += sizeof(double) * (size_t)npad;
-    return (b + 255) & ~(size_t)255;
-}
 
 // Threshold selection by buckets (select_threshold): thr_buckets buckets of
-// value-ordered keys and a candidate list of up to thr_cap keys.  A
-// 256-thread workgroup with 4096 keys takes the smaller figures, so that its
-// LDS (about 38 KiB) lets four workgroups share a CU instead of three.
-constexpr int kThrBucketsMax = 256;
-__host__ __device__ inline bool thr_small(int npad, int nt) { return nt < 1024 && npad >= 4096; }
-// (one bucket per thread at most: the bucket scans run a thread per bucket)
+// value-ordered keys and a candidate list of up to thr_cap keys, sized per
+// workgroup shape to what its share of the CU's LDS leaves: 128 threads
+// (eight workgroups per CU): 64 buckets, npad / 8 candidates; 256 (m_k <=
+// 1024): 256 buckets, npad / 2; 512 (m_k 2048 - 4096, two workgroups per CU
+// beside up to 64 KiB of envelopes): 128 buckets, npad / 4; 1024: 256
+// buckets, npad / 8.  (At most one bucket per thread: the bucket scans run a
+// thread per bucket.)
 __host__ __device__ inline int thr_buckets(int npad, int nt) {
-    return thr_small(npad, nt) ? 128 : (nt < kThrBucketsMax ? nt : kThrBucketsMax);
+    (void)npad;
+    return nt == 128 ? 64 : (nt == 512 ? 128 : 256);
 }
 __host__ __device__ inline int thr_cap(int npad, int nt) {
-    const int c = thr_small(npad, nt) ? npad / 16 : npad / 8;
+    const int c = nt == 256 ? npad / 2 : (nt == 512 ? npad / 4 : npad / 8);
     return c > 64 ? c : 64;
 }
 __host__ __device__ inline size_t thr_lds_bytes(int npad, int nt) {
@@ -82,14 +71,48 @@ __host__ __device__ inline size_t thr_lds_bytes(int npad, int nt) {
            (size_t)thr_buckets(npad, nt) * (2 * sizeof(uint64_t) + 4 * sizeof(int));
 }
 
+// Layouts by segment length.  kDual (m_k <= 8192): U_n and -L_n both in LDS
+// (two m_k arrays), from one pass over the window statistics; with m_k = 4096
+// and 512 threads, 71 KiB, two workgroups per CU.  kSingle (m_k = 16384): one
+// m_k array, U_n for the upper threshold and mask, then -L_n (two passes);
+// 156 KiB with 1024 threads.  Beside the arrays: a zone for the chunk
+// extremes and then the threshold scratch, and the two mask bitmaps.
+// kSlab (longer segments): van Herk arrays and both envelopes in the block's
+// global scratch.
+enum LbipLayout { kDual = 0, kSingle = 1, kSlab = 2 };
+__host__ __device__ inline int lbip_layout(int mk) {
+    return mk <= 8192 ? kDual : (mk <= 16384 ? kSingle : kSlab);
+}
+__host__ __device__ inline size_t zone_bytes(int npad, int nt, int kml) {
+    const size_t c = 4 * sizeof(double) * (size_t)((kml + 63) / 64);
+    const size_t t = thr_lds_bytes(npad, nt);
+    return c > t ? c : t;
+}
+
 size_t lbip_lds_bytes(const Geometry& G) {
     const int nthr = G.m_k - 2 * G.m + 2;
     const int npad = npad_of(nthr);
     const int nw = (G.m_k + 63) / 64;
-    const size_t late = sizeof(int64_t) * 2 * (size_t)(G.m + 1) + sizeof(uint64_t) * 4 * (size_t)nw;
     const int nt = lbip_threads(npad, G.S);
-    const size_t k = keys_in_lds(npad, nt) ? sizeof(double) * (size_t)npad : 0;
-    return std::max(late, k + thr_lds_bytes(npad, nt));
+    const int lay = lbip_layout(G.m_k);
+    if (lay != kSlab)
+        return sizeof(double) * (size_t)G.m_k * (lay == kDual ? 2 : 1) +
+               zone_bytes(npad, nt, (int)G.kml) + 2 * sizeof(uint64_t) * (size_t)nw;
+    // kSlab: the keys in global scratch; the threshold scratch, then the
+    // penalty prefix sums and the mask bitmaps, in LDS
+    const size_t late = sizeof(int64_t) * 2 * (size_t)(G.m + 1) + sizeof(uint64_t) * 4 * (size_t)nw;
+    return std::max(late, thr_lds_bytes(npad, nt));
+}
+
+// Per-block global scratch: the threshold's full-sort buffer (npad keys);
+// with kSlab also the van Herk arrays and both envelopes.
+size_t lbip_slab_bytes(const Geometry& G) {
+    const int nthr = G.m_k - 2 * G.m + 2;
+    const int npad = npad_of(nthr);
+    size_t b = sizeof(double) * (size_t)npad;
+    if (lbip_layout(G.m_k) == kSlab)
+        b += sizeof(double) * (8 * (size_t)G.kml + 2 * (size_t)G.m_k);
+    return (b + 255) & ~(size_t)255;
 }
 
 // ---------------------------------------------------------------- helpers ---
@@ -366,8 +389,277 @@ __device__ __forceinline__ double lane63(double v) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Exclusive scan of per-thread int64 totals; returns this thread's offset.
-__device__ int64_t block_exscan(int64_t x, int64_t* sh) {
+// Inclusive min (MN) and / or max (MX) scans over the wave's lanes in lane
+// order, for NaN-free values (a lane without a source keeps the identity).
+template <int CTRL, int ROWS, bool MN, bool MX>
+__device__ __forceinline__ void scan_step_nf(double& vmn, double& vmx) {
+    if (MN) vmn = vmin(vmn, dpp_f64_or<CTRL, ROWS>(vmn, __builtin_inf()));
+    if (MX) vmx = vmax(vmx, dpp_f64_or<CTRL, ROWS>(vmx, -__builtin_inf()));
+}
+template <bool MN, bool MX>
+__device__ __forceinline__ void wave_scan_nf(double& vmn, double& vmx) {
+    scan_step_nf<0x111, 0xf, MN, MX>(vmn, vmx);
+    scan_step_nf<0x112, 0xf, MN, MX>(vmn, vmx);
+    scan_step_nf<0x114, 0xf, MN, MX>(vmn, vmx);
+    scan_step_nf<0x118, 0xf, MN, MX>(vmn, vmx);
+    scan_step_nf<0x142, 0xa, MN, MX>(vmn, vmx);
+    scan_step_nf<0x143, 0xc, MN, MX>(vmn, vmx);
+}
+
+// ------------------------------------------ window-statistic ranges, LDS ---
+// Position i of a segment is covered by the windows [lo(i), hi(i)] =
+// [max(0, i - m + 1), min(i, k_m_l - 1)] (src/lower_upper_lemire.cpp:94-126).
+// A wave takes 64 consecutive positions p0 .. p0 + 63 (a tile; p0 a multiple
+// of 64, m >= 64).  With A / B the tile's first / last position, each range
+// splits into
+//   [lo(i), lo(B))     a suffix of the left run [lo(A), lo(B)),
+//   [lo(B), hi(A)]     the middle shared by the tile (not empty for m >= 64),
+//   (hi(A), hi(i)]     a prefix of the right run (hi(A), hi(B)].
+// The runs are at most 63 windows: one load per lane and a wave scan each
+// (the left run loaded in reverse, so its scan gives suffixes).  The middle
+// is a partial 64-window chunk at either end (one load per lane each) and
+// the whole chunks between, whose extremes a first pass put in LDS.
+// NaN statistics are tracked apart (ballots of the NaN lanes; chunk extremes
+// are NaN when the chunk holds one) and make the result NaN, as the
+// NaN-propagating min / max of a sequential scan do; the finite extremes are
+// exact, so the values equal the sequential scan's up to the sign of a zero
+// and NaN payloads.
+// The tile's window geometry (wave-uniform but for lo / hi) and its load
+// indices, each clamped into the run it reads so that every lane loads
+// unconditionally (the in* flags say which loads count).
+struct TileGeom {
+    int loB, hiA, lo, hi, ca, cb;
+    int il, ir, ih, it;
+    bool inl, inr, inh, intl;
+};
+__device__ __forceinline__ TileGeom tile_geom(int p0, int m, int kml, int mk, int lane) {
+    TileGeom g;
+    const int iB = (p0 + 63 < mk - 1) ? p0 + 63 : mk - 1;
+    const int i = (p0 + lane < iB) ? p0 + lane : iB;
+    const int loA = (p0 - m + 1 > 0) ? p0 - m + 1 : 0;
+    g.loB = (iB - m + 1 > 0) ? iB - m + 1 : 0;
+    g.hiA = (p0 < kml - 1) ? p0 : kml - 1;
+    const int hiB = (iB < kml - 1) ? iB : kml - 1;
+    g.lo = (i - m + 1 > 0) ? i - m + 1 : 0;
+    g.hi = (i < kml - 1) ? i : kml - 1;
+    g.ca = g.loB >> 6;
+    g.cb = g.hiA >> 6;
+    g.inl = lane < g.loB - loA;
+    g.inr = lane < hiB - g.hiA;
+    g.inh = (g.ca != g.cb) ? (lane < 64 - (g.loB & 63)) : (g.loB + lane <= g.hiA);
+    g.intl = (g.ca != g.cb) && lane <= (g.hiA & 63);
+    g.il = (g.loB - 1 - lane > loA) ? g.loB - 1 - lane : loA;
+    g.ir = (g.hiA + 1 + lane < hiB) ? g.hiA + 1 + lane : hiB;
+    g.ih = (g.loB + lane < g.hiA) ? g.loB + lane : g.hiA;
+    g.it = ((g.cb << 6) + lane < g.hiA) ? (g.cb << 6) + lane : g.hiA;
+    return g;
+}
+struct TileVals {
+    double l, r, h, t;
+};
+__device__ __forceinline__ TileVals tile_load(const double* __restrict__ src, const TileGeom& g) {
+    return TileVals{src[g.il], src[g.ir], src[g.ih], src[g.it]};
+}
+
+// Scan part of one statistic: the left / right runs' scans per lane and the
+// middle's partial chunks reduced over the wave; NaN lanes as ballots.
+template <bool MN, bool MX>
+struct TileScan {
+    double lmn, lmx, rmn, rmx, mmn, mmx;
+    uint64_t nanl, nanr;
+    bool nanm;
+};
+template <bool MN, bool MX>
+__device__ __forceinline__ TileScan<MN, MX> tile_scan(const TileVals& v, const TileGeom& g) {
+    const double inf = __builtin_inf();
+    TileScan<MN, MX> r;
+    r.nanl = __ballot(g.inl && v.l != v.l);
+    r.nanr = __ballot(g.inr && v.r != v.r);
+    r.nanm = __ballot((g.inh && v.h != v.h) || (g.intl && v.t != v.t)) != 0ull;
+    const bool okl = g.inl && v.l == v.l, okr = g.inr && v.r == v.r;
+    const bool okh = g.inh && v.h == v.h, okt = g.intl && v.t == v.t;
+    r.lmn = okl ? v.l : inf;
+    r.lmx = okl ? v.l : -inf;
+    r.rmn = okr ? v.r : inf;
+    r.rmx = okr ? v.r : -inf;
+    const double tmn = okt ? v.t : inf, tmx = okt ? v.t : -inf;
+    r.mmn = vmin(okh ? v.h : inf, tmn);
+    r.mmx = vmax(okh ? v.h : -inf, tmx);
+    wave_scan_nf<MN, MX>(r.lmn, r.lmx);  // lane j: [lo(B) - 1 - j, lo(B))
+    wave_scan_nf<MN, MX>(r.rmn, r.rmx);  // lane j: (hi(A), hi(A) + 1 + j]
+    wave_scan_nf<MN, MX>(r.mmn, r.mmx);
+    if (MN) r.mmn = lane63(r.mmn);
+    if (MX) r.mmx = lane63(r.mmx);
+    return r;
+}
+// The position's extremes from the scans, once the whole chunks are folded
+// into mmn / mmx (NaN propagating).
+template <bool MN, bool MX>
+__device__ __forceinline__ void tile_finish(const TileScan<MN, MX>& r, const TileGeom& g,
+                                            double* omn, double* omx) {
+    double mmn = r.mmn, mmx = r.mmx;
+    bool nan = r.nanm || (MN ? mmn != mmn : mmx != mmx);
+    const int jl = (g.loB - 1 - g.lo) & 63, jr = (g.hi - g.hiA - 1) & 63;
+    const double gln = MN ? __shfl(r.lmn, jl, 64) : 0.0, glx = MX ? __shfl(r.lmx, jl, 64) : 0.0;
+    const double grn = MN ? __shfl(r.rmn, jr, 64) : 0.0, grx = MX ? __shfl(r.rmx, jr, 64) : 0.0;
+    if (g.lo < g.loB) {
+        nan = nan || (r.nanl & ((2ull << jl) - 1)) != 0ull;
+        mmn = vmin(mmn, gln);
+        mmx = vmax(mmx, glx);
+    }
+    if (g.hi > g.hiA) {
+        nan = nan || (r.nanr & ((2ull << jr) - 1)) != 0ull;
+        mmn = vmin(mmn, grn);
+        mmx = vmax(mmx, grx);
+    }
+    if (MN) *omn = nan ? __builtin_nan("") : mmn;
+    if (MX) *omx = nan ? __builtin_nan("") : mmx;
+}
+
+// The tile's extremes of both statistics: the mean's minimum (MUMN) and / or
+// maximum (MUMX), the std's minimum and maximum.  C: the chunk extremes.
+template <bool MUMN, bool MUMX>
+__device__ __forceinline__ void tile_ext(const TileVals& vm, const TileVals& vs,
+                                         const TileGeom& g, const double* C, int nch,
+                                         double* mumin, double* mumax, double* sgmin,
+                                         double* sgmax) {
+    TileScan<MUMN, MUMX> a = tile_scan<MUMN, MUMX>(vm, g);
+    TileScan<true, true> b = tile_scan<true, true>(vs, g);
+    for (int c = g.ca + 1; c < g.cb; c++) {  // whole chunks
+        if constexpr (MUMN) a.mmn = nmin(a.mmn, C[c]);
+        if constexpr (MUMX) a.mmx = nmax(a.mmx, C[nch + c]);
+        b.mmn = nmin(b.mmn, C[2 * nch + c]);
+        b.mmx = nmax(b.mmx, C[3 * nch + c]);
+    }
+    tile_finish<MUMN, MUMX>(a, g, mumin, mumax);
+    tile_finish<true, true>(b, g, sgmin, sgmax);
+}
+
+// Extremes of the 64-window chunks [64 c, 64 c + 64) of the segment's mean
+// and std: C[c], C[nch + c] (mean min, max), C[2 nch + c], C[3 nch + c]
+// (std min, max); NaN when the chunk holds a NaN statistic.
+template <int NT>
+__device__ void chunk_extremes(const double* __restrict__ MU, const double* __restrict__ SG,
+                               int kml, int nch, double* C) {
+    const double inf = __builtin_inf(), nan = __builtin_nan("");
+    const int lane = threadIdx.x & 63;
+    for (int c = threadIdx.x >> 6; c < nch; c += NT / 64) {
+        const int j = c * 64 + lane;
+        const bool in = j < kml;
+        const int jc = in ? j : kml - 1;
+        const double vm = MU[jc], vs = SG[jc];
+        const bool nm = __ballot(in && vm != vm) != 0ull;
+        const bool ns = __ballot(in && vs != vs) != 0ull;
+        const bool okm = in && vm == vm, oks = in && vs == vs;
+        double a = okm ? vm : inf, b = okm ? vm : -inf;
+        double e = oks ? vs : inf, f = oks ? vs : -inf;
+        wave_scan_nf<true, true>(a, b);
+        wave_scan_nf<true, true>(e, f);
+        a = lane63(a);
+        b = lane63(b);
+        e = lane63(e);
+        f = lane63(f);
+        if (lane == 0) {
+            C[c] = nm ? nan : a;
+            C[nch + c] = nm ? nan : b;
+            C[2 * nch + c] = ns ? nan : e;
+            C[3 * nch + c] = ns ? nan : f;
+        }
+    }
+}
+
+// Keys of one threshold: how many are not NaN, whether one is -inf, and the
+// finite range.
+struct KeyStats {
+    int cnt = 0;
+    int ninf = 0;
+    double lo = __builtin_inf(), hi = -__builtin_inf();
+    __device__ void add(double v) {
+        cnt += (v == v);
+        ninf |= (v == -__builtin_inf());
+        if (__builtin_isfinite(v)) {
+            lo = fmin(lo, v);
+            hi = fmax(hi, v);
+        }
+    }
+    // the smallest key (rank 0) once reduced over the block
+    __device__ double kmin() const { return ninf ? -__builtin_inf() : lo; }
+};
+
+// The normalised upper envelope U_n (PART & 1: into Xu) and / or the negated
+// lower envelope -L_n (PART & 2: into Xl) of every position of the segment
+// (src/lb_normalize.cpp:9-32), from the raw envelopes Eu / El (the
+// target-wide ones, or for positions within w of the segment's edges the
+// segment's own, src/lower_upper_lemire.cpp:8-44) and the statistics'
+// extremes over the position's windows.  ku / kl: the keys (positions m - 1
+// .. m_k - m).
+template <int PART, int NT>
+__device__ void norm_env_lds(const double* __restrict__ MU, const double* __restrict__ SG,
+                             const double* __restrict__ t, const double* __restrict__ Eu,
+                             const double* __restrict__ El, const double* C, int m, int mk,
+                             int kml, int w, double* Xu, double* Xl, KeyStats* ku,
+                             KeyStats* kl) {
+    constexpr bool U = PART & 1, L = PART & 2;
+    const int lane = threadIdx.x & 63;
+    const int nch = (kml + 63) / 64;
+    auto finish = [&](int i, double Uv, double Lv, double mumin, double mumax, double sgmin,
+                      double sgmax) {
+        if (i < w || i >= mk - w) {
+            const int lo = i - w > 0 ? i - w : 0;
+            const int hi = i + w < mk - 1 ? i + w : mk - 1;
+            Uv = t[lo];
+            Lv = t[lo];
+            for (int x = lo + 1; x <= hi; x++) {
+                if (U) Uv = nmax(Uv, t[x]);
+                if (L) Lv = nmin(Lv, t[x]);
+            }
+        }
+        const bool key = i >= m - 1 && i <= mk - m;
+        if constexpr (U) {
+            const double nu = Uv - mumin;
+            const double v = (nu >= 0) ? nu / sgmin : nu / sgmax;
+            Xu[i] = v;
+            if (key) ku->add(v);
+        }
+        if constexpr (L) {
+            const double nl = Lv - mumax;
+            const double v = -((nl <= 0) ? nl / sgmin : nl / sgmax);
+            Xl[i] = v;
+            if (key) kl->add(v);
+        }
+    };
+    if (m >= 64) {
+        for (int p0 = (threadIdx.x >> 6) * 64; p0 < mk; p0 += NT) {
+            const int i = p0 + lane;  // < mk: m_k is a multiple of 64 for m >= 64
+            const TileGeom g = tile_geom(p0, m, kml, mk, lane);
+            const TileVals vm = tile_load(MU, g), vs = tile_load(SG, g);
+            // (not used within w of the edges)
+            const double Uv = U ? Eu[i] : 0.0, Lv = L ? El[i] : 0.0;
+            double mumin = 0.0, mumax = 0.0, sgmin, sgmax;
+            tile_ext<U, L>(vm, vs, g, C, nch, &mumin, &mumax, &sgmin, &sgmax);
+            finish(i, Uv, Lv, mumin, mumax, sgmin, sgmax);
+        }
+    } else {  // ranges of fewer than 64 windows: a direct loop per position
+        for (int i = threadIdx.x; i < mk; i += NT) {
+            const int lo = i - m + 1 > 0 ? i - m + 1 : 0;
+            const int hi = i < kml - 1 ? i : kml - 1;
+            double mumin = MU[lo], mumax = MU[lo], sgmin = SG[lo], sgmax = SG[lo];
+            for (int x = lo + 1; x <= hi; x++) {
+                mumin = nmin(mumin, MU[x]);
+                mumax = nmax(mumax, MU[x]);
+                sgmin = nmin(sgmin, SG[x]);
+                sgmax = nmax(sgmax, SG[x]);
+            }
+            const bool edge = i < w || i >= mk - w;
+            finish(i, (U && !edge) ? Eu[i] : 0.0, (L && !edge) ? El[i] : 0.0, mumin, mumax,
+                   sgmin, sgmax);
+        }
+    }
+}
+
+// Exclusive scan of per-thread int64 totals; returns this thread's offset
+// (and the block's total in *total when given).
+__device__ int64_t block_exscan(int64_t x, int64_t* sh, int64_t* total = nullptr) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int64_t inc = x;
     for (int o = 1; o < 64; o <<= 1) {
@@ -376,34 +668,50 @@ __device__ int64_t block_exscan(int64_t x, int64_t* sh) {
     }
     if (lane == 63) sh[wid] = inc;
     __syncthreads();
-    int64_t off = 0;
-    for (int k = 0; k < wid; k++) off += sh[k];
+    int64_t off = 0, tot = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+        if (k < wid) off += sh[k];
+        tot += sh[k];
+    }
     __syncthreads();
+    if (total) *total = tot;
     return off + inc - x;
 }
 
-// Block-wide min and max of one value each per thread (ends with a barrier).
-__device__ void block_min_max(double* lo, double* hi, double* sh) {
-    double a = *lo, b = *hi;
+// A KeyStats per thread reduced over the block, into every thread (one
+// barrier pair).
+__device__ void block_keystats(KeyStats* ks, int* shc, int64_t* shn, double* shm) {
+    int c = ks->cnt, n = ks->ninf;
+    double a = ks->lo, b = ks->hi;
     for (int o = 32; o > 0; o >>= 1) {
+        c += __shfl_xor(c, o, 64);
+        n |= __shfl_xor(n, o, 64);
         a = fmin(a, __shfl_xor(a, o, 64));
         b = fmax(b, __shfl_xor(b, o, 64));
     }
     const int nwv = (int)(blockDim.x >> 6), wid = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        sh[wid] = a;
-        sh[nwv + wid] = b;
+        shc[wid] = c;
+        shn[wid] = n;
+        shm[wid] = a;
+        shm[nwv + wid] = b;
     }
     __syncthreads();
-    a = sh[0];
-    b = sh[nwv];
-    for (int k = 1; k < nwv; k++) {
-        a = fmin(a, sh[k]);
-        b = fmax(b, sh[nwv + k]);
+    c = 0;
+    n = 0;
+    a = __builtin_inf();
+    b = -__builtin_inf();
+    for (int k = 0; k < nwv; k++) {
+        c += shc[k];
+        n |= (int)shn[k];
+        a = fmin(a, shm[k]);
+        b = fmax(b, shm[nwv + k]);
     }
     __syncthreads();
-    *lo = a;
-    *hi = b;
+    ks->cnt = c;
+    ks->ninf = n;
+    ks->lo = a;
+    ks->hi = b;
 }
 
 // ------------------------------------------------------- threshold choice ---
@@ -436,15 +744,22 @@ __device__ inline int bucket_of(double v, double lo, double scale, int nb) {
     return 1 + (int)fmin(fmax(x, 0.0), (double)(nb - 3));
 }
 
-// keys[0 .. nthr): the values, NaN for positions that do not count; [lo, hi]:
-// the finite keys' range.  tlds: thr_lds_bytes(npad, NT) of LDS.  The full sort
-// (fallback) reorders keys[0 .. npad) in place.
+// keys[0 .. nthr): the values, NaN for positions that do not count; ks: their
+// statistics, reduced over the block.  tlds: thr_lds_bytes(npad, NT) of LDS.
+// The full sort (fallback) orders the keys in sortbuf[0 .. npad), which may be
+// keys itself.
 template <int NT, class HF, class EF>
-__device__ double select_threshold(double* keys, int nthr, int npad, int nval, double lo,
-                                   double hi, HF hf, EF herr, unsigned char* tlds,
-                                   ArgBest* sh_best, int* sh_cnt, int64_t* sh_scan,
-                                   double* sh_key) {
+__device__ double select_threshold(const double* keys, int nthr, int npad, const KeyStats& ks,
+                                   HF hf, EF herr, unsigned char* tlds, double* sortbuf,
+                                   ArgBest* sh_best, int64_t* sh_scan, double* sh_key) {
+    const int nval = ks.cnt;
+    const double lo = ks.lo, hi = ks.hi;
     if (nval == 0) return __builtin_nan("");
+    // the rank-0 key, and h there first: the bucket pass's atomics hide the
+    // latency of its searches
+    const double kmin = ks.kmin();
+    const double h0 = hf(kmin);
+    if (h0 != h0) return kmin;  // src/find_t_lp.cpp:206-216 keeps index 0
     const int nbk = thr_buckets(npad, NT);
     const int tid = threadIdx.x;
     const int cap = thr_cap(npad, NT);
@@ -477,12 +792,6 @@ __device__ double select_threshold(double* keys, int nthr, int npad, int nval, d
     const int c = (tid < nbk) ? cnt[tid] : 0;
     const int st = (int)block_exscan(c, sh_scan);
     if (tid < nbk) bst[tid] = st;
-    if (tid < nbk && c > 0 && st == 0) *sh_key = key_of(bmin[tid]);  // rank 0
-    __syncthreads();
-    const double kmin = *sh_key;
-    __syncthreads();
-    const double h0 = hf(kmin);
-    if (h0 != h0) return kmin;  // src/find_t_lp.cpp:206-216 keeps index 0
 
     ArgBest att{__builtin_nan(""), 0};
     double ub = 0.0;
@@ -497,8 +806,9 @@ __device__ double select_threshold(double* keys, int nthr, int npad, int nval, d
     const ArgBest lb = block_best<true>(att, sh_best);
     const bool keep = tid < nbk && c > 0 && !(ub < lb.v);
     const int kc = keep ? c : 0;
-    const int cs = (int)block_exscan(kc, sh_scan);
-    const int K = block_count(kc, sh_cnt);
+    int64_t ktot;
+    const int cs = (int)block_exscan(kc, sh_scan, &ktot);
+    const int K = (int)ktot;
     if (tid < nbk) cst[tid] = keep ? cs : -1;
     __syncthreads();
     if (K <= cap) {
@@ -535,18 +845,18 @@ __device__ double select_threshold(double* keys, int nthr, int npad, int nval, d
     // full order: NaN and padding sort last
     for (int i = tid; i < npad; i += NT) {
         const double v = (i < nthr) ? keys[i] : __builtin_nan("");
-        if (v != v) keys[i] = __builtin_inf();
+        sortbuf[i] = (v != v) ? __builtin_inf() : v;
     }
     __threadfence_block();
     __syncthreads();
-    bitonic_sort(keys, npad);
+    bitonic_sort(sortbuf, npad);
     ArgBest best{__builtin_nan(""), 0};
     for (int p = tid; p < nval; p += NT) {
-        const double tv = keys[p];
+        const double tv = sortbuf[p];
         best = better_max(best, ArgBest{hf(tv) * (double)(p + 1), p});
     }
     best = block_best<true>(best, sh_best);
-    const double t = keys[best.i];
+    const double t = sortbuf[best.i];
     __syncthreads();
     return t;
 }
@@ -583,11 +893,44 @@ __device__ inline int64_t trans_sum(const uint64_t* gw, const uint64_t* tw, cons
     return acc;
 }
 
+// The same sum for the 64 windows j0 .. j0 + 63 of a wave (lane: j), from
+// the ascending list P[0 .. n) of the mask's transitions: the wave walks the
+// transitions t in (j0, j0 + 63 + m) together (t uniform), each lane adding
+// -CH[t - j] where a run starts (g[t] set) and +CH[t - j] where one ends, for
+// j < t < j + m.  The caller adds CH[m] when g[j + m - 1] is set.
+__device__ inline int64_t list_sum(const int* P, int n, const uint64_t* gw, const int64_t* CH,
+                                   int j0, int j, int m) {
+    int lo = 0, hi = n;  // first t > j0
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (P[mid] > j0) hi = mid; else lo = mid + 1;
+    }
+    const int lim = j0 + 63 + m;
+    int64_t acc = 0;
+    for (int k = lo; k < n; k++) {
+        const int t = P[k];
+        if (t >= lim) break;
+        const int d = t - j;
+        const bool in = d > 0 && d < m;
+        const int64_t c = CH[in ? d : 0];
+        const bool start = (gw[t >> 6] >> (t & 63)) & 1;
+        acc += in ? (start ? -c : c) : 0;
+    }
+    return acc;
+}
+
 // ------------------------------------------------------------------ kernel ---
-// 128- and 256-thread workgroups fit four waves per SIMD.
-template <int NT>
-__global__ void __launch_bounds__(NT)
-    __attribute__((amdgpu_waves_per_eu(NT <= 256 ? 4 : 1))) k_lbip(LbipArgs a) {
+// kDual / kSingle keep the normalised envelopes in LDS (X, and X2 = -L_n for
+// kDual), then the penalty prefix sums and mask transitions in X's space;
+// the statistics' ranges come from tiles of 64 positions (tile_ext) and the
+// chunk extremes, so nothing but the bounds leaves the workgroup.  kSlab runs
+// van Herk prefix / suffix scans over blocks of m windows into the block's
+// global scratch, with U_n, L_n and the keys there too.
+// One kernel per layout, so that none carries another's registers; every
+// workgroup size fits four waves per SIMD.
+template <int NT, int LAYOUT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
+    k_lbip(LbipArgs a) {
     extern __shared__ unsigned char lds_raw[];
     __shared__ ArgBest sh_best[NT / 64];
     __shared__ double sh_sum[NT / 64];
@@ -604,18 +947,72 @@ __global__ void __launch_bounds__(NT)
     const int nthr = mk - 2 * m + 2;
     const int npad = npad_of(nthr);
     const int nw = (mk + 63) / 64;
+    constexpr bool xl = LAYOUT != kSlab;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 
-    int64_t* CHl = (int64_t*)lds_raw;
-    int64_t* CHu = CHl + (m + 1);
-    uint64_t* GLw = (uint64_t*)(CHu + (m + 1));
-    uint64_t* GUw = GLw + nw;
-    uint64_t* TLw = GUw + nw;  // transitions: bit p = g[p] xor g[p - 1]
-    uint64_t* TUw = TLw + nw;
     double* slab = (double*)((char*)a.slab + (size_t)blockIdx.x * a.slab_bytes_per_block);
-    double* PQ = slab;  // 8 arrays of kml: (2*stat + dir)
+    // LDS layout: X (and X2), the zone (chunk extremes, then threshold
+    // scratch), masks
+    double* X = (double*)lds_raw;
+    double* X2 = X + mk;
+    unsigned char* zone = lds_raw + sizeof(double) * (size_t)mk * (LAYOUT == kDual ? 2 : 1);
+    // global-scratch layout: 8 van Herk arrays of kml (2 * stat + dir), U_n,
+    // L_n, keys
+    double* PQ = slab;
     double* Un = slab + 8 * (size_t)kml;
     double* Ln = Un + mk;
-    double* keys = keys_in_lds(npad, NT) ? (double*)lds_raw : (Ln + mk);
+    int64_t *CHl, *CHu;
+    uint64_t *GLw, *GUw, *TLw, *TUw;  // masks, transitions: bit p = g[p] xor g[p - 1]
+    if (xl) {
+        GLw = (uint64_t*)(zone + zone_bytes(npad, NT, kml));
+        GUw = GLw + nw;
+        CHl = (int64_t*)X;  // X is dead once both masks are built
+        CHu = CHl + (m + 1);
+        TLw = (uint64_t*)(CHu + (m + 1));
+        TUw = TLw + nw;
+    } else {
+        CHl = (int64_t*)lds_raw;
+        CHu = CHl + (m + 1);
+        GLw = (uint64_t*)(CHu + (m + 1));
+        GUw = GLw + nw;
+        TLw = GUw + nw;
+        TUw = TLw + nw;
+    }
+    double* keys = xl ? X + (m - 1) : Ln + mk;
+    double* sortbuf = xl ? slab : keys;  // the threshold's full-sort fallback
+    unsigned char* tlds = xl ? zone : lds_raw;
+
+    // (|computed h - exact h| is far below 1e-9 of the terms' magnitude:
+    // prefix sums of at most m terms of |q| + |t| + |C| or its square)
+    const double qmax = fmax(fabs(Q.qs[0]), fabs(Q.qs[m - 1]));
+    auto herr = [&](double at) {
+        const double A = qmax + at + fabs(C);
+        return 1e-9 * (double)m * (A * A + A);
+    };
+    auto hf_u = [&](double t) { return h_upper(Q, metric, m, C, t); };
+    auto hf_l = [&](double t) { return h_lower(Q, metric, m, C, -t); };
+    // threshold from keys k of stats ks: upper (ascending ranks of U_n, G =
+    // rank + 1) or lower (ascending ranks of -L_n are L_n's from the top;
+    // negation is exact, so t_l = -the key)
+    auto threshold = [&](const double* k, KeyStats ks, bool upper) {
+        block_keystats(&ks, sh_cnt, sh_scan, sh_mm);
+        if (upper)
+            return select_threshold<NT>(k, nthr, npad, ks, hf_u, herr, tlds, sortbuf, sh_best,
+                                        sh_scan, &sh_key);
+        const double tn = select_threshold<NT>(k, nthr, npad, ks, hf_l, herr, tlds, sortbuf,
+                                               sh_best, sh_scan, &sh_key);
+        return (tn == tn) ? -tn : tn;
+    };
+    // mask bitmaps: g_u = U_n <= t_u from X[] = U_n, g_l = L_n >= t_l from
+    // X[] = -L_n
+    auto mask = [&](const double* Xs, double thr, bool upper, uint64_t* out) {
+        for (int wi = wid; wi < nw; wi += NT / 64) {
+            const int pos = wi * 64 + lane;
+            const bool in = pos < mk;
+            const uint64_t b = __ballot(in && (upper ? Xs[pos] <= thr : -Xs[pos] >= thr));
+            if (lane == 0) out[wi] = b;
+        }
+    };
 
     for (int64_t s = blockIdx.x; s < G.S; s += gridDim.x) {
         const bool tail = (s == G.forsize);
@@ -626,165 +1023,188 @@ __global__ void __launch_bounds__(NT)
         const uint8_t* INV = a.INV + s * G.kml;
         const int jfirst = tail ? G.tail_first : 0;
         const int64_t gbase = s * G.kml;
+        double t_u, t_l;
 
-        // 1. van Herk prefix (dir 0) / suffix (dir 1) scans over blocks of m
-        //    windows: one wave per (statistic, direction, block), 64 windows
-        //    per step, min and max of the same statistic in one pass.  The
-        //    NaN-propagating min / max are associative, so the values equal a
-        //    sequential scan's.
-        {
-            const int nb = (kml + m - 1) / m;
-            const int lane = threadIdx.x & 63;
-            const double inf = __builtin_inf();
-            for (int item = threadIdx.x >> 6; item < 4 * nb; item += NT / 64) {
-                const int sg = item / (2 * nb);  // 0: mean, 1: std
-                const int rem = item - sg * 2 * nb;
-                const int dir = rem / nb;
-                const int b = rem - dir * nb;
-                const double* src = sg ? SG : MU;
-                double* omin = PQ + (size_t)(2 * (2 * sg) + dir) * kml;
-                double* omax = PQ + (size_t)(2 * (2 * sg + 1) + dir) * kml;
-                const int j0 = b * m;
-                const int j1 = (j0 + m < kml) ? j0 + m : kml;
-                double cmin = inf, cmax = -inf;
-                // the loads of up to kScanG chunks are issued before the
-                // chunk-to-chunk carry chain that consumes them
-                constexpr int kScanG = 4;
-                for (int c0 = 0; c0 < j1 - j0; c0 += 64 * kScanG) {
-                    double v[kScanG];
+        if constexpr (LAYOUT == kDual) {
+            double* Cx = (double*)zone;
+            if (m >= 64) {
+                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx);
+                __syncthreads();
+            }
+            KeyStats ku, kl;
+            norm_env_lds<3, NT>(MU, SG, t, a.Uraw + seg0, a.Lraw + seg0, Cx, m, mk, kml, w, X,
+                                X2, &ku, &kl);
+            __syncthreads();  // (the zone's chunk extremes are dead from here)
+            t_u = threshold(X + (m - 1), ku, true);
+            mask(X, t_u, true, GUw);
+            t_l = threshold(X2 + (m - 1), kl, false);
+            mask(X2, t_l, false, GLw);
+            __syncthreads();
+        } else if constexpr (LAYOUT == kSingle) {
+            double* Cx = (double*)zone;
+            KeyStats ku, kl;
+            if (m >= 64) {
+                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx);
+                __syncthreads();
+            }
+            norm_env_lds<1, NT>(MU, SG, t, a.Uraw + seg0, nullptr, Cx, m, mk, kml, w, X, nullptr,
+                                &ku, nullptr);
+            __syncthreads();
+            t_u = threshold(keys, ku, true);
+            mask(X, t_u, true, GUw);
+            __syncthreads();
+            if (m >= 64) {  // the zone held threshold scratch since
+                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx);
+                __syncthreads();
+            }
+            norm_env_lds<2, NT>(MU, SG, t, nullptr, a.Lraw + seg0, Cx, m, mk, kml, w, nullptr, X,
+                                nullptr, &kl);
+            __syncthreads();
+            t_l = threshold(keys, kl, false);
+            mask(X, t_l, false, GLw);
+            __syncthreads();
+        } else {
+            // 1. van Herk prefix (dir 0) / suffix (dir 1) scans over blocks of
+            //    m windows: one wave per (statistic, direction, block), 64
+            //    windows per step, min and max of the same statistic in one
+            //    pass.  The NaN-propagating min / max are associative, so the
+            //    values equal a sequential scan's.
+            {
+                const int nb = (kml + m - 1) / m;
+                const double inf = __builtin_inf();
+                for (int item = wid; item < 4 * nb; item += NT / 64) {
+                    const int sg = item / (2 * nb);  // 0: mean, 1: std
+                    const int rem = item - sg * 2 * nb;
+                    const int dir = rem / nb;
+                    const int b = rem - dir * nb;
+                    const double* src = sg ? SG : MU;
+                    double* omin = PQ + (size_t)(2 * (2 * sg) + dir) * kml;
+                    double* omax = PQ + (size_t)(2 * (2 * sg + 1) + dir) * kml;
+                    const int j0 = b * m;
+                    const int j1 = (j0 + m < kml) ? j0 + m : kml;
+                    double cmin = inf, cmax = -inf;
+                    // the loads of up to kScanG chunks are issued before the
+                    // chunk-to-chunk carry chain that consumes them
+                    constexpr int kScanG = 4;
+                    for (int c0 = 0; c0 < j1 - j0; c0 += 64 * kScanG) {
+                        double v[kScanG];
 #pragma unroll
-                    for (int u = 0; u < kScanG; u++) {
-                        const int c = c0 + 64 * u;
-                        // prefix: ascending windows; suffix: descending windows
-                        const int j = (dir == 0) ? j0 + c + lane : j1 - 1 - c - lane;
-                        const bool in = (dir == 0) ? (j < j1) : (j >= j0);
-                        v[u] = in ? src[j] : 0.0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < kScanG; u++) {
-                        const int c = c0 + 64 * u;
-                        if (c >= j1 - j0) break;
-                        const int j = (dir == 0) ? j0 + c + lane : j1 - 1 - c - lane;
-                        const bool in = (dir == 0) ? (j < j1) : (j >= j0);
-                        double vmin = in ? v[u] : inf, vmax = in ? v[u] : -inf;
-                        wave_scan_min_max(vmin, vmax);
-                        vmin = nmin(cmin, vmin);
-                        vmax = nmax(cmax, vmax);
-                        if (in) {
-                            omin[j] = vmin;
-                            omax[j] = vmax;
+                        for (int u = 0; u < kScanG; u++) {
+                            const int c = c0 + 64 * u;
+                            // prefix: ascending windows; suffix: descending
+                            const int j = (dir == 0) ? j0 + c + lane : j1 - 1 - c - lane;
+                            const bool in = (dir == 0) ? (j < j1) : (j >= j0);
+                            v[u] = in ? src[j] : 0.0;
                         }
-                        cmin = lane63(vmin);
-                        cmax = lane63(vmax);
+#pragma unroll
+                        for (int u = 0; u < kScanG; u++) {
+                            const int c = c0 + 64 * u;
+                            if (c >= j1 - j0) break;
+                            const int j = (dir == 0) ? j0 + c + lane : j1 - 1 - c - lane;
+                            const bool in = (dir == 0) ? (j < j1) : (j >= j0);
+                            double vmn = in ? v[u] : inf, vmx = in ? v[u] : -inf;
+                            wave_scan_min_max(vmn, vmx);
+                            vmn = nmin(cmin, vmn);
+                            vmx = nmax(cmax, vmx);
+                            if (in) {
+                                omin[j] = vmn;
+                                omax[j] = vmx;
+                            }
+                            cmin = lane63(vmn);
+                            cmax = lane63(vmx);
+                        }
                     }
                 }
             }
-        }
-        __threadfence_block();
-        __syncthreads();
+            __threadfence_block();
+            __syncthreads();
 
-        // 2-3. envelopes, stat ranges, normalised envelopes; upper keys.
-        //     Each thread takes kEnvG positions at a time and issues their
-        //     loads (envelope, the eight prefix / suffix values) first.
-        constexpr int kEnvG = 2;
-        int ucnt = 0;
-        double klo = __builtin_inf(), khi = -__builtin_inf();  // finite keys
-        for (int i0 = threadIdx.x; i0 < mk; i0 += kEnvG * NT) {
-            double Lg[kEnvG], Ug[kEnvG], PS[kEnvG][8];
+            // 2-3. envelopes, stat ranges, normalised envelopes; upper keys.
+            //     Each thread takes kEnvG positions at a time and issues their
+            //     loads (envelope, the eight prefix / suffix values) first.
+            constexpr int kEnvG = 2;
+            KeyStats ku;
+            for (int i0 = threadIdx.x; i0 < mk; i0 += kEnvG * NT) {
+                double Lg[kEnvG], Ug[kEnvG], PS[kEnvG][8];
 #pragma unroll
-            for (int e = 0; e < kEnvG; e++) {
-                const int i = i0 + e * NT;
-                if (i >= mk) break;
-                const bool edge = i < w || i >= mk - w;
-                Lg[e] = edge ? 0.0 : a.Lraw[seg0 + i];
-                Ug[e] = edge ? 0.0 : a.Uraw[seg0 + i];
-                const int64_t lo = i - m + 1 > 0 ? i - m + 1 : 0;
-                const int64_t hi = i < kml - 1 ? i : kml - 1;
+                for (int e = 0; e < kEnvG; e++) {
+                    const int i = i0 + e * NT;
+                    if (i >= mk) break;
+                    const bool edge = i < w || i >= mk - w;
+                    Lg[e] = edge ? 0.0 : a.Lraw[seg0 + i];
+                    Ug[e] = edge ? 0.0 : a.Uraw[seg0 + i];
+                    const int64_t lo = i - m + 1 > 0 ? i - m + 1 : 0;
+                    const int64_t hi = i < kml - 1 ? i : kml - 1;
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    PS[e][2 * q] = PQ[(size_t)(2 * q) * kml + hi];          // prefix at hi
-                    PS[e][2 * q + 1] = PQ[(size_t)(2 * q + 1) * kml + lo];  // suffix at lo
+                    for (int q = 0; q < 4; q++) {
+                        PS[e][2 * q] = PQ[(size_t)(2 * q) * kml + hi];          // prefix at hi
+                        PS[e][2 * q + 1] = PQ[(size_t)(2 * q + 1) * kml + lo];  // suffix at lo
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < kEnvG; e++) {
+                    const int i = i0 + e * NT;
+                    if (i >= mk) break;
+                    double Lv = Lg[e], Uv = Ug[e];
+                    if (i < w || i >= mk - w) {
+                        const int lo = i - w > 0 ? i - w : 0;
+                        const int hi = i + w < mk - 1 ? i + w : mk - 1;
+                        Lv = t[lo];
+                        Uv = t[lo];
+                        for (int x = lo + 1; x <= hi; x++) {
+                            Lv = nmin(Lv, t[x]);
+                            Uv = nmax(Uv, t[x]);
+                        }
+                    }
+                    const int64_t lo = i - m + 1 > 0 ? i - m + 1 : 0;
+                    const int64_t hi = i < kml - 1 ? i : kml - 1;
+                    const double mumin = stat_range(PS[e][0], PS[e][1], lo, hi, m, true);
+                    const double mumax = stat_range(PS[e][2], PS[e][3], lo, hi, m, false);
+                    const double sgmin = stat_range(PS[e][4], PS[e][5], lo, hi, m, true);
+                    const double sgmax = stat_range(PS[e][6], PS[e][7], lo, hi, m, false);
+                    const double nu = Uv - mumin;
+                    const double un = (nu >= 0) ? nu / sgmin : nu / sgmax;
+                    const double nl = Lv - mumax;
+                    const double ln = (nl <= 0) ? nl / sgmin : nl / sgmax;
+                    Un[i] = un;
+                    Ln[i] = ln;
+                    if (i >= m - 1 && i <= mk - m) {
+                        keys[i - (m - 1)] = un;
+                        ku.add(un);
+                    }
                 }
             }
-#pragma unroll
-            for (int e = 0; e < kEnvG; e++) {
-            const int i = i0 + e * NT;
-            if (i >= mk) break;
-            double Lv = Lg[e], Uv = Ug[e];
-            if (i < w || i >= mk - w) {
-                const int lo = i - w > 0 ? i - w : 0;
-                const int hi = i + w < mk - 1 ? i + w : mk - 1;
-                Lv = t[lo];
-                Uv = t[lo];
-                for (int x = lo + 1; x <= hi; x++) {
-                    Lv = nmin(Lv, t[x]);
-                    Uv = nmax(Uv, t[x]);
+            __threadfence_block();
+            __syncthreads();
+
+            // 4a. upper threshold: ascending ranks of U_n, G = rank + 1
+            t_u = threshold(keys, ku, true);
+
+            // 4b. lower threshold: ranks from the top of L_n, G = rank + 1: the
+            //     ascending ranks of -L_n (negation is exact)
+            KeyStats kl;
+            for (int i = threadIdx.x; i < nthr; i += blockDim.x) {
+                const double nl = -Ln[i + m - 1];
+                keys[i] = nl;
+                kl.add(nl);
+            }
+            __threadfence_block();
+            __syncthreads();
+            t_l = threshold(keys, kl, false);
+
+            // mask bitmaps: g_l = L_n >= t_l, g_u = U_n <= t_u
+            for (int wi = wid; wi < nw; wi += NT / 64) {
+                const int pos = wi * 64 + lane;
+                const bool in = pos < mk;
+                const uint64_t bl = __ballot(in && (Ln[pos] >= t_l));
+                const uint64_t bu = __ballot(in && (Un[pos] <= t_u));
+                if (lane == 0) {
+                    GLw[wi] = bl;
+                    GUw[wi] = bu;
                 }
             }
-            const int64_t lo = i - m + 1 > 0 ? i - m + 1 : 0;
-            const int64_t hi = i < kml - 1 ? i : kml - 1;
-            const double mumin = stat_range(PS[e][0], PS[e][1], lo, hi, m, true);
-            const double mumax = stat_range(PS[e][2], PS[e][3], lo, hi, m, false);
-            const double sgmin = stat_range(PS[e][4], PS[e][5], lo, hi, m, true);
-            const double sgmax = stat_range(PS[e][6], PS[e][7], lo, hi, m, false);
-            const double nu = Uv - mumin;
-            const double un = (nu >= 0) ? nu / sgmin : nu / sgmax;
-            const double nl = Lv - mumax;
-            const double ln = (nl <= 0) ? nl / sgmin : nl / sgmax;
-            Un[i] = un;
-            Ln[i] = ln;
-            if (i >= m - 1 && i <= mk - m) {
-                keys[i - (m - 1)] = un;
-                ucnt += (un == un);
-                if (__builtin_isfinite(un)) {
-                    klo = fmin(klo, un);
-                    khi = fmax(khi, un);
-                }
-            }
-            }
+            __syncthreads();
         }
-        __threadfence_block();
-        __syncthreads();
-
-        // 4a. upper threshold: ascending ranks of U_n, G = rank + 1
-        // (|computed h - exact h| is far below 1e-9 of the terms' magnitude:
-        // prefix sums of at most m terms of |q| + |t| + |C| or its square)
-        const double qmax = fmax(fabs(Q.qs[0]), fabs(Q.qs[m - 1]));
-        auto herr = [&](double at) {
-            const double A = qmax + at + fabs(C);
-            return 1e-9 * (double)m * (A * A + A);
-        };
-        unsigned char* tlds = lds_raw + (keys_in_lds(npad, NT) ? sizeof(double) * npad : 0);
-        const int nval_u = block_count(ucnt, sh_cnt);
-        block_min_max(&klo, &khi, sh_mm);
-        const double t_u = select_threshold<NT>(
-            keys, nthr, npad, nval_u, klo, khi,
-            [&](double t) { return h_upper(Q, metric, m, C, t); }, herr, tlds, sh_best, sh_cnt,
-            sh_scan, &sh_key);
-
-        // 4b. lower threshold: ranks from the top of L_n, G = rank + 1: the
-        //     ascending ranks of -L_n (negation is exact)
-        int lcnt = 0;
-        klo = __builtin_inf();
-        khi = -__builtin_inf();
-        for (int i = threadIdx.x; i < nthr; i += blockDim.x) {
-            const double nl = -Ln[i + m - 1];
-            keys[i] = nl;
-            lcnt += (nl == nl);
-            if (__builtin_isfinite(nl)) {
-                klo = fmin(klo, nl);
-                khi = fmax(khi, nl);
-            }
-        }
-        __threadfence_block();
-        __syncthreads();
-        const int nval_l = block_count(lcnt, sh_cnt);
-        block_min_max(&klo, &khi, sh_mm);
-        const double tln = select_threshold<NT>(
-            keys, nthr, npad, nval_l, klo, khi,
-            [&](double t) { return h_lower(Q, metric, m, C, -t); }, herr, tlds, sh_best, sh_cnt,
-            sh_scan, &sh_key);
-        const double t_l = (tln == tln) ? -tln : tln;
         if (threadIdx.x == 0) {
             if (a.TUo) a.TUo[s] = t_u;
             if (a.TLo) a.TLo[s] = t_l;
@@ -836,23 +1256,10 @@ __global__ void __launch_bounds__(NT)
             }
         }
 
-        // 6. mask bitmaps: g_l = L_n >= t_l, g_u = U_n <= t_u
-        {
-            const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-            for (int wi = wid; wi < nw; wi += NT / 64) {
-                const int pos = wi * 64 + lane;
-                const bool in = pos < mk;
-                const bool gl = in && (Ln[pos] >= t_l);
-                const bool gu = in && (Un[pos] <= t_u);
-                const uint64_t bl = __ballot(gl);
-                const uint64_t bu = __ballot(gu);
-                if (lane == 0) {
-                    GLw[wi] = bl;
-                    GUw[wi] = bu;
-                }
-            }
-        }
-        __syncthreads();
+        // 6. mask transitions: bitmaps, and for kDual / kSingle their
+        //    ascending lists (positions t with g[t] != g[t - 1]) after the
+        //    bitmaps in X's space, one word per thread (nw <= NT), when they
+        //    fit (lcap entries each)
         for (int wi = threadIdx.x; wi < nw; wi += NT) {
             const uint64_t pl = wi ? GLw[wi - 1] >> 63 : 0ull;
             const uint64_t pu = wi ? GUw[wi - 1] >> 63 : 0ull;
@@ -860,19 +1267,38 @@ __global__ void __launch_bounds__(NT)
             TUw[wi] = GUw[wi] ^ ((GUw[wi] << 1) | pu);
         }
         __syncthreads();
+        int* PL = (int*)(TUw + nw);
+        const int lcap = (LAYOUT == kDual) ? mk : mk / 2;
+        int* PU = PL + lcap;
+        int nl = 0, nu = 0;
+        bool lists = false;
+        if constexpr (xl) {
+            const int wi = threadIdx.x;
+            const int64_t c = (wi < nw) ? ((int64_t)__builtin_popcountll(TLw[wi]) |
+                                           ((int64_t)__builtin_popcountll(TUw[wi]) << 32))
+                                        : 0;
+            int64_t tot;
+            const int64_t off = block_exscan(c, sh_scan, &tot);  // low / high halves
+            nl = (int)(tot & 0xffffffff);
+            nu = (int)(tot >> 32);
+            lists = nl <= lcap && nu <= lcap;
+            if (lists && wi < nw) {
+                int ol = (int)(off & 0xffffffff), ou = (int)(off >> 32);
+                for (uint64_t b = TLw[wi]; b; b &= b - 1) PL[ol++] = wi * 64 + __builtin_ctzll(b);
+                for (uint64_t b = TUw[wi]; b; b &= b - 1) PU[ou++] = wi * 64 + __builtin_ctzll(b);
+            }
+            __syncthreads();
+        }
 
         // 7. per-window bound
         const double scale = ldexp(1.0, -F);
         ArgBest rep{__builtin_nan(""), -1};
-        for (int j = jfirst + threadIdx.x; j < kml; j += blockDim.x) {
+        auto bound = [&](int j, int64_t z) {
             const int64_t g = gbase + (j - jfirst);
             double lb;
-            if (INV[j]) {
-                lb = __builtin_nan("");
-            } else if (!finite) {
+            if (INV[j] || !finite) {
                 lb = __builtin_nan("");
             } else {
-                const int64_t z = trans_sum(GLw, TLw, CHl, j, m) + trans_sum(GUw, TUw, CHu, j, m);
                 const double zd = __ll2double_rd(z) * scale;
                 if (metric == DTW || metric == ERP) lb = sqrt(fmax(0.0, zd));
                 else if (metric == LCSS) lb = fmax(0.0, zd / (double)m);
@@ -880,6 +1306,21 @@ __global__ void __launch_bounds__(NT)
                 if (!is_excluded(a.p, g)) rep = better_min(rep, ArgBest{lb, g});
             }
             a.LB[g] = lb;
+        };
+        if (lists) {
+            for (int j0 = jfirst + wid * 64; j0 < kml; j0 += NT) {
+                const int j = j0 + lane;
+                const int jc = j < kml ? j : kml - 1;
+                int64_t z = list_sum(PL, nl, GLw, CHl, j0, jc, m) +
+                            list_sum(PU, nu, GUw, CHu, j0, jc, m);
+                const int hi = jc + m - 1;
+                if ((GLw[hi >> 6] >> (hi & 63)) & 1) z += CHl[m];
+                if ((GUw[hi >> 6] >> (hi & 63)) & 1) z += CHu[m];
+                if (j < kml) bound(j, z);
+            }
+        } else {
+            for (int j = jfirst + threadIdx.x; j < kml; j += blockDim.x)
+                bound(j, trans_sum(GLw, TLw, CHl, j, m) + trans_sum(GUw, TUw, CHu, j, m));
         }
         if (a.rep_lb) {
             rep = block_best<false>(rep, sh_best);
@@ -897,9 +1338,19 @@ hipError_t launch_lbip(hipStream_t st, const LbipArgs& a) {
     const int npad = npad_of(a.G.m_k - 2 * a.G.m + 2);
     const dim3 g((unsigned)a.blocks);
     const int nt = lbip_threads(npad, a.G.S);
-    if (nt == 128) hipLaunchKernelGGL(k_lbip<128>, g, dim3(128), lds, st, a);
-    else if (nt == 256) hipLaunchKernelGGL(k_lbip<256>, g, dim3(256), lds, st, a);
-    else hipLaunchKernelGGL(k_lbip<1024>, g, dim3(1024), lds, st, a);
+    switch (lbip_layout(a.G.m_k)) {
+        case kDual:
+            if (nt == 128) hipLaunchKernelGGL((k_lbip<128, kDual>), g, dim3(128), lds, st, a);
+            else if (nt == 256) hipLaunchKernelGGL((k_lbip<256, kDual>), g, dim3(256), lds, st, a);
+            else if (nt == 512) hipLaunchKernelGGL((k_lbip<512, kDual>), g, dim3(512), lds, st, a);
+            else hipLaunchKernelGGL((k_lbip<1024, kDual>), g, dim3(1024), lds, st, a);
+            break;
+        case kSingle:  // m_k = 16384: 1024 threads
+            hipLaunchKernelGGL((k_lbip<1024, kSingle>), g, dim3(1024), lds, st, a);
+            break;
+        default:  // m_k > 16384: 1024 threads
+            hipLaunchKernelGGL((k_lbip<1024, kSlab>), g, dim3(1024), lds, st, a);
+    }
     return hipGetLastError();
 }
 
