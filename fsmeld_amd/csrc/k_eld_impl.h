@@ -91,6 +91,17 @@ __device__ inline double msm_cell(double up, double left, double diag, double e,
     const double c3 = left + (c + s3);
     return dmin3(c1, c2, c3);
 }
+// The same with the row's clamp bounds given: dxlo = min(0, dx), dxhi =
+// max(0, dx) (the register kernel stages them per query row in LDS).
+__device__ __forceinline__ double msm_cell_rb(double up, double left, double diag, double e,
+                                              double dxlo, double dxhi, double dyn, double c) {
+    const double c1 = diag + fabs(e);
+    const double s2 = fabs(fmax(fmin(e, dxhi), dxlo));
+    const double s3 = msm_split(dyn, e);
+    const double c2 = up + (c + s2);
+    const double c3 = left + (c + s3);
+    return dmin3(c1, c2, c3);
+}
 
 // Candidate sample source: raw target samples normalised on the fly, or rows
 // that are already normalised.
@@ -259,7 +270,7 @@ __device__ __forceinline__ eld_t<METRIC> band_sentinel() {
 template <int METRIC, int W>
 __device__ __forceinline__ void step_a(int r, double arg, eld_t<METRIC>* A,
                                        const eld_t<METRIC>* B, const double* X,
-                                       const double* Y) {
+                                       const double* Y, const double* DL, const double* DH) {
     using VT = eld_t<METRIC>;
     const VT S = band_sentinel<METRIC>();  // out-of-band neighbour
     const VT up = (r >= 1) ? B[r - 1] : S;
@@ -277,9 +288,9 @@ __device__ __forceinline__ void step_a(int r, double arg, eld_t<METRIC>* A,
         A[r] = min(min(up, left) + 5, A[r] + mv);
         return;
     }
-    if (METRIC == MSM) {
-        A[r] = msm_cell(up, left, A[r], X[r + 1] - Y[W - r + 1], X[r + 1] - X[r],
-                        Y[W - r] - Y[W - r + 1], arg);
+    if (METRIC == MSM) {  // DL / DH[k]: the bounds of x[b - 1 + k] - x[b - 2 + k]
+        A[r] = msm_cell_rb(up, left, A[r], X[r + 1] - Y[W - r + 1], DL[r + 1], DH[r + 1],
+                           Y[W - r] - Y[W - r + 1], arg);
         return;
     }
     A[r] = cell<METRIC, false>(0, 0, up, left, A[r], X[r + 1], X[r], Y[W - r + 1], Y[W - r],
@@ -290,7 +301,8 @@ __device__ __forceinline__ void step_a(int r, double arg, eld_t<METRIC>* A,
 // Slot update of the odd anti-diagonal (B[r], cell (b + 1 + r, b + W - r)).
 template <int METRIC, int W>
 __device__ __forceinline__ void step_b(int r, double arg, const eld_t<METRIC>* A,
-                                       eld_t<METRIC>* B, const double* X, const double* Y) {
+                                       eld_t<METRIC>* B, const double* X, const double* Y,
+                                       const double* DL, const double* DH) {
     if constexpr (METRIC == EDR) {  // x[i-1] = X[r + 1], y[j-1] = Y[W - r]
         const int sub = (fabs(X[r + 1] - Y[W - r]) <= arg) ? 0 : 1;
         B[r] = min(min(A[r], A[r + 1]), B[r] - 2 + sub);  // D - (i + j)
@@ -305,8 +317,8 @@ __device__ __forceinline__ void step_b(int r, double arg, const eld_t<METRIC>* A
         return;
     }
     if (METRIC == MSM) {
-        B[r] = msm_cell(A[r], A[r + 1], B[r], X[r + 2] - Y[W - r + 1], X[r + 2] - X[r + 1],
-                        Y[W - r] - Y[W - r + 1], arg);
+        B[r] = msm_cell_rb(A[r], A[r + 1], B[r], X[r + 2] - Y[W - r + 1], DL[r + 2], DH[r + 2],
+                           Y[W - r] - Y[W - r + 1], arg);
         return;
     }
     B[r] = cell<METRIC, false>(0, 0, A[r], A[r + 1], B[r], X[r + 2], X[r + 1], Y[W - r + 1],
@@ -321,12 +333,49 @@ __device__ __forceinline__ void step_b(int r, double arg, const eld_t<METRIC>* A
 // live only briefly.
 template <int METRIC, int W>
 __device__ __forceinline__ void ad_step(double arg, eld_t<METRIC>* A, eld_t<METRIC>* B,
-                                        const double* X, const double* Y) {
-    step_a<METRIC, W>(0, arg, A, B, X, Y);
+                                        const double* X, const double* Y, const double* DL,
+                                        const double* DH) {
+    if constexpr (METRIC == MSM) {
+        // The cell pair (step_a(r), step_b(r - 1)) reads query row r + 1
+        // (X, DL, DH[r + 1]) from LDS; the row of the next pair is loaded
+        // during this one.  An empty asm after each pair (its two values in
+        // and out, memory clobbered) keeps the compiler from moving the
+        // pairs' arithmetic or the row reads across pairs: unconstrained, it
+        // hoists every row of the iteration into registers, which spill.
+        double xr[W + 2], lr[W + 2], hr[W + 2];
+        xr[1] = X[1];
+        lr[1] = DL[1];
+        hr[1] = DH[1];
+#pragma unroll
+        for (int r = 0; r <= W; r++) {
+            if (r + 2 <= W + 1) {
+                xr[r + 2] = X[r + 2];
+                lr[r + 2] = DL[r + 2];
+                hr[r + 2] = DH[r + 2];
+            }
+            {
+                const eld_t<METRIC> S = band_sentinel<METRIC>();
+                const double up = (r >= 1) ? B[r - 1] : S;
+                const double left = (r <= W - 1) ? B[r] : S;
+                A[r] = msm_cell_rb(up, left, A[r], xr[r + 1] - Y[W - r + 1], lr[r + 1], hr[r + 1],
+                                   Y[W - r] - Y[W - r + 1], arg);
+            }
+            if (r >= 1) {
+                const int q = r - 1;
+                B[q] = msm_cell_rb(A[q], A[q + 1], B[q], xr[q + 2] - Y[W - q + 1], lr[q + 2],
+                                   hr[q + 2], Y[W - q] - Y[W - q + 1], arg);
+                asm volatile("" : "+v"(A[r]), "+v"(B[q])::"memory");
+            } else {
+                asm volatile("" : "+v"(A[r])::"memory");
+            }
+        }
+        return;
+    }
+    step_a<METRIC, W>(0, arg, A, B, X, Y, DL, DH);
 #pragma unroll
     for (int r = 1; r <= W; r++) {
-        step_a<METRIC, W>(r, arg, A, B, X, Y);
-        step_b<METRIC, W>(r - 1, arg, A, B, X, Y);
+        step_a<METRIC, W>(r, arg, A, B, X, Y, DL, DH);
+        step_b<METRIC, W>(r - 1, arg, A, B, X, Y, DL, DH);
     }
 }
 
@@ -342,6 +391,83 @@ __device__ __forceinline__ void ring_shift(double* Y, double yn) {
 // for the per-iteration overhead to matter.
 template <int METRIC>
 constexpr bool kEldPair = (METRIC == EDR || METRIC == LCSS || METRIC == SWALE);
+// MSM runs two iterations fused (msm_fused2).
+template <int METRIC>
+constexpr bool kEldFused = (METRIC == MSM);
+
+// Two MSM iterations (b and b + 1) in one skewed sweep: step s runs pair s
+// of iteration b (step_a(s), step_b(s - 1)) and then pair q = s - 1 of
+// iteration b + 1, whose ring is Y + 1 and query window X + 1.  Pair q of
+// iteration b + 1 reads iteration b's B[q - 1], B[q] (written by its pairs q
+// and q + 1 = s) and overwrites A[q] and B[q - 1], which iteration b no
+// longer reads; every cell gets exactly the operands of the two ad_step calls
+// in sequence.  Both pairs of step s take query row b + s (X[s + 1] and its
+// clamp bounds DL / DH[s + 1]), and column b + W - r's bounds col(r) serve
+// steps r .. r + 3, so each row is read and each column's bounds computed
+// once per two iterations.  Y[W + 2] holds iteration b + 1's new sample.  *r1:
+// the end cell's slot after iteration b.  The empty asm statements (values
+// in and out, memory clobbered) keep each step's arithmetic and the next
+// row's reads in place: left free, the compiler hoists every row of the
+// sweep into registers, which spill.
+template <int W>
+__device__ __forceinline__ void msm_fused2(double arg, double* A, double* B, const double* X,
+                                           const double* Y, const double* DL, const double* DH,
+                                           double* r1) {
+    const double S = kEldSentinel;
+    double xr[W + 3], lr[W + 3], hr[W + 3];
+    double cl[W + 2], ch[W + 2];  // col(r) at r + 1: min(0, dyn), max(0, dyn)
+    auto col = [&](int r) {  // dyn = y_{j-1} - y_j of column j = b + W - r
+        const double d = Y[W - r] - Y[W - r + 1];
+        cl[r + 1] = fmin(d, 0.0);
+        ch[r + 1] = fmax(d, 0.0);
+    };
+    // msm_cell with the row's and the column's clamp bounds given
+    auto cellf = [&](double up, double left, double diag, double e, int row, int c) {
+        const double c1 = diag + fabs(e);
+        const double s2 = fabs(fmax(fmin(e, hr[row]), lr[row]));
+        const double s3 = fabs(fmax(fmin(e, ch[c + 1]), cl[c + 1]));
+        const double c2 = up + (arg + s2);
+        const double c3 = left + (arg + s3);
+        return dmin3(c1, c2, c3);
+    };
+    xr[1] = X[1];
+    lr[1] = DL[1];
+    hr[1] = DH[1];
+    col(-1);
+#pragma unroll
+    for (int s = 0; s <= W + 1; s++) {
+        if (s + 2 <= W + 2) {
+            xr[s + 2] = X[s + 2];
+            lr[s + 2] = DL[s + 2];
+            hr[s + 2] = DH[s + 2];
+        }
+        if (s <= W) {  // iteration b, pair s
+            col(s);
+            const double up = (s >= 1) ? B[s - 1] : S;
+            const double left = (s <= W - 1) ? B[s] : S;
+            A[s] = cellf(up, left, A[s], xr[s + 1] - Y[W - s + 1], s + 1, s);
+            if (s >= 1) {
+                B[s - 1] = cellf(A[s - 1], A[s], B[s - 1], xr[s + 1] - Y[W - s + 2], s + 1, s - 1);
+                asm volatile("" : "+v"(A[s]), "+v"(B[s - 1])::"memory");
+            } else {
+                asm volatile("" : "+v"(A[s])::"memory");
+            }
+            if (s == ((W & 1) ? (W + 1) / 2 : W / 2)) *r1 = (W & 1) ? B[(W - 1) / 2] : A[W / 2];
+        }
+        if (s >= 1) {  // iteration b + 1, pair q (ring Y + 1, window X + 1)
+            const int q = s - 1;
+            const double up = (q >= 1) ? B[q - 1] : S;
+            const double left = (q <= W - 1) ? B[q] : S;
+            A[q] = cellf(up, left, A[q], xr[s + 1] - Y[W - q + 2], s + 1, q - 1);
+            if (q >= 1) {
+                B[q - 1] = cellf(A[q - 1], A[q], B[q - 1], xr[s + 1] - Y[W - q + 3], s + 1, q - 2);
+                asm volatile("" : "+v"(A[q]), "+v"(B[q - 1])::"memory");
+            } else {
+                asm volatile("" : "+v"(A[q])::"memory");
+            }
+        }
+    }
+}
 
 // Per-sample lower bound on the cost of consuming candidate sample y whose
 // band sees the query envelope [lo, hi] (DTW, ERP, MSM); for EDR 1 if y is
@@ -492,6 +618,10 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
     double* xs = xlds + kXPad;
     double* lqs = xlds + span + kXPad;
     double* uqs = lqs + span;
+    // MSM: the rows' clamp bounds min(0, x_i - x_{i-1}), max(0, x_i - x_{i-1}),
+    // indexed like xlds (the first padded row: 0, a pad against a pad)
+    double* dxl = xlds + 3 * span;
+    double* dxh = dxl + span;
     for (int i = threadIdx.x; i < span; i += blockDim.x) {
         const int qi = i - kXPad;
         const bool in_q = qi >= 0 && qi < m;
@@ -499,6 +629,14 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
         if (check) {
             xlds[span + i] = in_q ? LQ[qi] : 0.0;
             xlds[2 * span + i] = in_q ? UQ[qi] : 0.0;
+        }
+        if constexpr (METRIC == MSM) {
+            const bool prev_q = qi - 1 >= 0 && qi - 1 < m;
+            const double dx = (i == 0) ? 0.0
+                                       : (in_q ? a.q.qn[qi] : kPad) -
+                                             (prev_q ? a.q.qn[qi - 1] : kPad);
+            xlds[3 * span + i] = fmin(dx, 0.0);
+            xlds[4 * span + i] = fmax(dx, 0.0);
         }
     }
     __syncthreads();
@@ -514,6 +652,7 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
     for (int k = 0; k < W + 2; k++) Y[k] = 0.0;
     const double* tp = raw ? a.T : rows;
     double mu = 0.0, sd = 1.0, total = 0.0, p = 0.0;
+    double y1n = kPad, y2n = kPad;  // kEldFused: the next two new samples
     int64_t ci = 0, g = 0;
     int b = 0, it = 0;
     unsigned live_iters = 0;
@@ -564,6 +703,14 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
                         if (j >= 0 && j < m) v = raw ? (tp[j] - mu) / sd : tp[j];
                         Y[k] = v;
                     }
+                    if constexpr (kEldFused<METRIC>) {  // samples BS + W + 1, BS + W + 2
+                        y1n = (BS + W + 1 < m) ? (raw ? (tp[BS + W + 1] - mu) / sd
+                                                      : tp[BS + W + 1])
+                                               : kPad;
+                        y2n = (BS + W + 2 < m) ? (raw ? (tp[BS + W + 2] - mu) / sd
+                                                      : tp[BS + W + 2])
+                                               : kPad;
+                    }
                     p = 0.0;
                     if (check) {
 #pragma unroll
@@ -605,12 +752,57 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
             if (raw) yv = (yv - mu) / sd;
             return (jn == jj) ? yv : kPad;
         };
+        if constexpr (kEldFused<METRIC>) {
+            // iterations b and b + 1 (msm_fused2); a lane whose candidate ends
+            // at b keeps its value (r1), as in the pair path below.  The
+            // next pass's two samples are loaded here and normalised after
+            // the sweep.
+            Y[W + 2] = y1n;
+            const int jn1 = b + W + 3, jn2 = b + W + 4;
+            const int jj1 = min(max(jn1, 0), m - 1), jj2 = min(max(jn2, 0), m - 1);
+            const double t1 = tp[jj1], t2 = tp[jj2];
+            double r1;
+            msm_fused2<W>(arg, A, B, xs + (b - 1), Y, dxl + kXPad + (b - 1),
+                          dxh + kXPad + (b - 1), &r1);
+            if (check) {
+                int K = b + KOFF;
+                if (K >= 0 && K < m) p += consume_lb<METRIC>(Y[KOFF + 1], lqs[K], uqs[K], arg);
+                K = b + 1 + KOFF;
+                if (K >= 0 && K < m) p += consume_lb<METRIC>(Y[KOFF + 2], lqs[K], uqs[K], arg);
+            }
+            Y[W + 3] = y2n;
+#pragma unroll
+            for (int k = 0; k < W + 2; k++) Y[k] = Y[k + 2];
+            y1n = (jn1 == jj1) ? (raw ? (t1 - mu) / sd : t1) : kPad;
+            y2n = (jn2 == jj2) ? (raw ? (t2 - mu) / sd : t2) : kPad;
+            const bool fin1 = busy && b == bend;
+            const bool fin = busy && (fin1 || b + 1 == bend);
+            live_iters += busy ? (fin1 ? 1u : 2u) : 0u;
+            if (busy) b += 2;
+            if (__any(fin)) {
+                const double rv = fin1 ? r1 : ((W & 1) ? B[(W - 1) / 2] : A[W / 2]);
+                const double d =
+                    eld_result<METRIC>(band_value<METRIC>(rv, 2 * bend + W + (W & 1)), m);
+                if (keep_all) {
+                    if (fin) out_dist[ci] = d;
+                } else {
+                    // src/fsmeld.cpp:846-847
+                    wave_append(fin && __builtin_isfinite(d) && d <= cut, g, d, out_idx,
+                                out_dist, out_count);
+                }
+                busy = busy && !fin;
+            }
+            if (check && ((++it) & 1) == 0)
+                busy = busy && ad_alive<METRIC, W>(A, B, b - 1, M, m, thr, total, p);
+            continue;
+        }
         if constexpr (kEldPair<METRIC>) {
             // iterations b and b + 1; a lane whose candidate ends at b keeps
             // its value (r1) and runs the second half on its stale state
             // (every index stays inside the padded query, the sample index is
             // clamped), then idles like any finished lane
-            ad_step<METRIC, W>(arg, A, B, xs + (b - 1), Y);
+            ad_step<METRIC, W>(arg, A, B, xs + (b - 1), Y, dxl + kXPad + (b - 1),
+                               dxh + kXPad + (b - 1));
             pin();
             const double y1 = sample(b);
             if (check) {
@@ -620,7 +812,7 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
             Y[W + 2] = y1;
             const bool fin1 = busy && b == bend;
             const VT r1 = (W & 1) ? B[(W - 1) / 2] : A[W / 2];
-            ad_step<METRIC, W>(arg, A, B, xs + b, Y + 1);
+            ad_step<METRIC, W>(arg, A, B, xs + b, Y + 1, dxl + kXPad + b, dxh + kXPad + b);
             pin();
             const double y2 = sample(b + 1);
             if (check) {
@@ -651,7 +843,8 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
                 busy = busy && ad_alive<METRIC, W>(A, B, b - 1, M, m, thr, total, p);
             continue;
         }
-        ad_step<METRIC, W>(arg, A, B, xs + (b - 1), Y);
+        ad_step<METRIC, W>(arg, A, B, xs + (b - 1), Y, dxl + kXPad + (b - 1),
+                           dxh + kXPad + (b - 1));
         pin();
         const int jn = b + W + 1;
         const int jj = min(max(jn, 0), m - 1);
@@ -1095,7 +1288,10 @@ __global__ void __launch_bounds__(64) k_eld_wave(VerifyArgs a, const double* __r
 template <int = 0>
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
-inline size_t reg_lds_bytes(int m) { return sizeof(double) * (size_t)(m + 2 * kXPad) * 3; }
+// query, envelope (abandoning) and, for MSM, the rows' clamp bounds
+inline size_t reg_lds_bytes(int metric, int m) {
+    return sizeof(double) * (size_t)(m + 2 * kXPad) * (metric == MSM ? 5 : 3);
+}
 
 inline size_t wave_lds_bytes(int m) { return sizeof(double) * 2 * (size_t)(m + 2); }
 
@@ -1104,8 +1300,7 @@ inline size_t wave_lds_bytes(int m) { return sizeof(double) * 2 * (size_t)(m + 2
 constexpr int kRegMaxW = 25;  // the launcher instantiates k_eld for W = 0 .. kRegMaxW
 constexpr int64_t kEldFewCandidates = 1024;
 inline bool uses_register_band(int metric, int m, int w) {
-    (void)metric;
-    return reg_lds_bytes(m) <= kEldLdsCap && w <= kRegMaxW;
+    return reg_lds_bytes(metric, m) <= kEldLdsCap && w <= kRegMaxW;
 }
 inline bool uses_wave_band(int metric, int m, int w) {
     return !uses_register_band(metric, m, w) && w <= 127 && wave_lds_bytes(m) <= kEldLdsCap;
@@ -1165,7 +1360,7 @@ hipError_t launch_metric(hipStream_t st, const VerifyArgs& a, const int64_t* in,
     // persistent: enough waves to fill every SIMD; later blocks find the
     // queue drained and exit
     const dim3 grid(1024);
-    const size_t lds = reg_lds_bytes(a.p.m);
+    const size_t lds = reg_lds_bytes(METRIC, a.p.m);
     switch (w) {
 #define FSM_ELD(WW)                                                                            \
     case WW:                                                                                   \
