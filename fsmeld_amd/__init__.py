@@ -103,6 +103,12 @@ _sig("fsm_stage_glb_filter", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _I64
 _sig("fsm_stage_eld", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _I64, ctypes.c_int64, _D])
 _sig("fsm_eld_pairs", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      _D, _D, ctypes.c_int64, _D])
+# device-memory stage entries: pointers and the stream as integer addresses
+_sig("fsm_stage_window_stats_dev", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int64,
+                                                  ctypes.c_int64, _VP, _VP, _VP, _VP])
+_sig("fsm_stage_lbip_dev", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _VP, _VP, _VP, _VP])
+for _n in ("fsm_stage_glb_dev", "fsm_stage_glb_filter_dev", "fsm_stage_eld_dev"):
+    _sig(_n, ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, _VP, ctypes.c_int64, _VP, _VP])
 _sig("fsm_target_share", ctypes.c_int, [_VP, ctypes.POINTER(_VP)])
 _sig("fsm_target_trim", None, [_VP])
 _sig("fsm_query_file", ctypes.c_int, [ctypes.c_char_p, _VP, _D, ctypes.c_int64, ctypes.c_uint64,
@@ -358,6 +364,37 @@ class Target:
         _check(_lib.fsm_stage_eld(self._h, ctypes.byref(d), _ptr(q), _ptr(win, _I64),
                                   int(win.size), _ptr(out)))
         return out
+
+
+    # ---- the stages on device memory: device addresses (ints) in, outputs
+    # written in place on the device, ordered on `stream` (a hipStream_t
+    # address; 0: the null stream) ----
+    def window_stats_dev(self, m, mu, sigma, invalid, stream=0, exclude=None):
+        xb, xe = (-1, -1) if exclude is None else exclude
+        _check(_lib.fsm_stage_window_stats_dev(self._h, int(m), int(xb), int(xe), mu, sigma,
+                                               invalid, stream))
+
+    def lbip_dev(self, query, m, metric, w, arg, lb, t_u=None, t_l=None, stream=0, raw=False):
+        q = _f64(query)
+        d = _desc("search", metric, m, w, arg, raw=raw)
+        _check(_lib.fsm_stage_lbip_dev(self._h, ctypes.byref(d), _ptr(q), lb, t_u, t_l, stream))
+
+    def _verify_dev(self, fn, query, m, metric, w, arg, windows, n, out, stream, bsf=None):
+        q = _f64(query)
+        d = _desc("search", metric, m, w, arg, cutoff_value=0.0 if bsf is None else bsf)
+        _check(fn(self._h, ctypes.byref(d), _ptr(q), windows, int(n), out, stream))
+
+    def glb_dev(self, query, m, metric, w, arg, windows, n, out, bsf, stream=0):
+        self._verify_dev(_lib.fsm_stage_glb_dev, query, m, metric, w, arg, windows, n, out,
+                         stream, bsf)
+
+    def glb_keep_dev(self, query, m, metric, w, arg, windows, n, keep, bsf, stream=0):
+        self._verify_dev(_lib.fsm_stage_glb_filter_dev, query, m, metric, w, arg, windows, n,
+                         keep, stream, bsf)
+
+    def eld_dev(self, query, m, metric, w, arg, windows, n, out, stream=0):
+        self._verify_dev(_lib.fsm_stage_eld_dev, query, m, metric, w, arg, windows, n, out,
+                         stream)
 
 
 def eld_pairs(metric, q, cands, w, arg) -> np.ndarray:

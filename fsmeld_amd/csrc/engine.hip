@@ -470,8 +470,11 @@ int ensure_env(fsm_target* t, int w, bool* computed = nullptr) {
     return FSM_OK;
 }
 
+// lb_out / tu_out / tl_out: device buffers that take the bounds and
+// thresholds instead of the target's own (the device-memory stage entry)
 int run_lbip(fsm_target* t, const Geometry& G, const QueryTables& Q, const QueryParams& P,
-             bool with_rep) {
+             bool with_rep, double* lb_out = nullptr, double* tu_out = nullptr,
+             double* tl_out = nullptr) {
     HIP_TRY(t->lb.ensure(sizeof(double) * (size_t)G.ts_n));
     HIP_TRY(t->tu.ensure(sizeof(double) * (size_t)G.S));
     HIP_TRY(t->tl.ensure(sizeof(double) * (size_t)G.S));
@@ -495,6 +498,11 @@ int run_lbip(fsm_target* t, const Geometry& G, const QueryTables& Q, const Query
     a.TLo = t->tl.as<double>();
     a.rep_lb = with_rep ? t->rep_lb.as<double>() : nullptr;
     a.rep_idx = with_rep ? t->rep_idx.as<int64_t>() : nullptr;
+    if (lb_out) {
+        a.LB = lb_out;
+        a.TUo = tu_out;
+        a.TLo = tl_out;
+    }
     a.slab = t->slab.p;
     a.slab_bytes_per_block = per;
     a.blocks = blocks;
@@ -1201,6 +1209,208 @@ int fsm_stage_glb_filter(fsm_target* t, const fsm_query_desc* d, const double* q
 int fsm_stage_eld(fsm_target* t, const fsm_query_desc* d, const double* q, const int64_t* win,
                   int64_t n, double* out) {
     return stage_verify(t, d, q, win, n, out, STAGE_ELD_VALUES);
+}
+
+// ------------------------------------- stage entry points, device memory ---
+}  // extern "C"
+
+namespace {
+
+// The target's stream waits for the work the caller enqueued on its stream,
+// and the caller's stream then waits for the stage's work: events only, no
+// host synchronisation.
+struct StreamJoin {
+    hipStream_t cs, ts;
+    hipEvent_t e = nullptr;
+    StreamJoin(void* c, hipStream_t t) : cs((hipStream_t)c), ts(t) {}
+    hipError_t enter() {
+        hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        if (r == hipSuccess) r = hipEventRecord(e, cs);
+        if (r == hipSuccess) r = hipStreamWaitEvent(ts, e, 0);
+        return r;
+    }
+    hipError_t leave() {
+        hipError_t r = hipEventRecord(e, ts);
+        if (r == hipSuccess) r = hipStreamWaitEvent(cs, e, 0);
+        return r;
+    }
+    ~StreamJoin() {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
+// per-window statistics in global window order (mvmean's order), exclusion
+// [xb, xe) marked invalid as the host entry does
+__global__ void k_stats_gather(Geometry G, const double* __restrict__ MU,
+                               const double* __restrict__ SG, const uint8_t* __restrict__ INV,
+                               int64_t xb, int64_t xe, double* mu, double* sg, uint8_t* inv) {
+    const bool excl = xb >= 0 && xe > xb;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < G.ts_n;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = stat_index(G, g);
+        mu[g] = MU[s];
+        sg[g] = SG[s];
+        inv[g] = INV[s] || (excl && g >= xb && g < xe);
+    }
+}
+
+// *bad = 1 when a window index lies outside [0, ts_n)
+__global__ void k_check_windows(const int64_t* __restrict__ win, int64_t n, int64_t ts_n,
+                                unsigned long long* bad) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (win[i] < 0 || win[i] >= ts_n) *bad = 1ull;
+}
+
+__global__ void k_set_counters(unsigned long long* c, unsigned long long c0,
+                               unsigned long long c1) {
+    c[0] = c0;
+    c[1] = c1;
+}
+
+__global__ void k_zero_words(unsigned* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+// bit g of bits for each kept window g, then keep[i] = bit win[i]
+__global__ void k_mark_kept(const int64_t* __restrict__ kept,
+                            const unsigned long long* __restrict__ count, unsigned* bits) {
+    const int64_t n = (int64_t)*count;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        atomicOr(&bits[kept[i] >> 5], 1u << (kept[i] & 31));
+}
+__global__ void k_keep_flags(const int64_t* __restrict__ win, int64_t n,
+                             const unsigned* __restrict__ bits, uint8_t* keep) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        keep[i] = (bits[win[i] >> 5] >> (win[i] & 31)) & 1u;
+}
+
+inline dim3 grid_for(int64_t n) {
+    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(4096, (n + 255) / 256)));
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsm_stage_window_stats_dev(fsm_target* t, int m, int64_t xb, int64_t xe, double* mu,
+                               double* sigma, uint8_t* invalid, void* stream) {
+    if (!t || !mu || !sigma || !invalid) return fail(FSM_ERR_ARG, "null argument");
+    Geometry G;
+    if (m < 2 || !make_geometry(t->N, m, &G)) return fail(FSM_ERR_ARG, "invalid m");
+    StreamJoin j(stream, t->stream);
+    HIP_TRY(j.enter());
+    bool fresh;
+    int rc = ensure_stats(t, G, &fresh);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_stats_gather, grid_for(G.ts_n), dim3(256), 0, t->stream, G,
+                       t->st->a.as<double>(), t->st->b.as<double>(), t->st->c.as<uint8_t>(), xb,
+                       xe, mu, sigma, invalid);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(j.leave());
+    return FSM_OK;
+}
+
+int fsm_stage_lbip_dev(fsm_target* t, const fsm_query_desc* d, const double* q, double* lb,
+                       double* t_u, double* t_l, void* stream) {
+    if (!t || !lb) return fail(FSM_ERR_ARG, "null argument");
+    int rc = validate(d, t->N, q, d ? d->m : 0, false);
+    if (rc) return rc;
+    Geometry G;
+    if (!make_geometry(t->N, d->m, &G)) return fail(FSM_ERR_ARG, "invalid m");
+    StreamJoin j(stream, t->stream);
+    HIP_TRY(j.enter());
+    HostQuery H;
+    prepare_query(q, d->m, d->w, d->arg, &H, d->raw != 0);
+    QueryTables Q{};
+    if ((rc = upload_query(t, H, d->m, &Q))) return rc;
+    QueryParams P = params_of(d);
+    bool fresh;
+    if ((rc = ensure_stats(t, G, &fresh, d->raw != 0))) return rc;
+    if ((rc = ensure_env(t, d->w))) return rc;
+    if ((rc = run_lbip(t, G, Q, P, false, lb, t_u, t_l))) return rc;
+    HIP_TRY(j.leave());
+    return FSM_OK;
+}
+
+static int stage_verify_dev(fsm_target* t, const fsm_query_desc* d, const double* q,
+                            const int64_t* win, int64_t n, void* out, StageKind kind,
+                            void* stream) {
+    const bool glb = kind != STAGE_ELD_VALUES;
+    if (!t || !out || (n > 0 && !win) || n < 0) return fail(FSM_ERR_ARG, "null argument");
+    int rc = validate(d, t->N, q, d ? d->m : 0, false);
+    if (rc) return rc;
+    Geometry G;
+    if (!make_geometry(t->N, d->m, &G)) return fail(FSM_ERR_ARG, "invalid m");
+    if (n == 0) return FSM_OK;
+    StreamJoin j(stream, t->stream);
+    HIP_TRY(j.enter());
+    unsigned long long* Cd = t->counters.as<unsigned long long>();
+    // the window indices are checked on the device before any kernel reads
+    // through them (one word read back)
+    hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(1), 0, t->stream, Cd + 11, 0ull, 0ull);
+    hipLaunchKernelGGL(k_check_windows, grid_for(n), dim3(256), 0, t->stream, win, n, G.ts_n,
+                       Cd + 11);
+    HIP_TRY(hipGetLastError());
+    unsigned long long bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, Cd + 11, sizeof(bad), hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    if (bad) return fail(FSM_ERR_ARG, "window index out of range");
+    HostQuery H;
+    prepare_query(q, d->m, d->w, d->arg, &H, d->raw != 0);
+    QueryTables Q{};
+    if ((rc = upload_query(t, H, d->m, &Q))) return rc;
+    QueryParams P = params_of(d);
+    bool fresh;
+    if ((rc = ensure_stats(t, G, &fresh, d->raw != 0))) return rc;
+    if (glb && (rc = ensure_env(t, d->w))) return rc;
+    if ((rc = ensure_eld_scratch(t, d->metric, d->m, d->w))) return rc;
+    hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(1), 0, t->stream, Cd + 6,
+                       (unsigned long long)n, 0ull);
+    HIP_TRY(hipGetLastError());
+    VerifyArgs v = verify_args(t, G, Q, P);
+    if (kind == STAGE_GLB_VALUES) {
+        HIP_TRY(launch_glb(t->stream, v, win, Cd + 6, nullptr, nullptr, nullptr,
+                           static_cast<double*>(out), Cd + 10));
+    } else if (kind == STAGE_ELD_VALUES) {
+        HIP_TRY(launch_eld(t->stream, v, win, nullptr, Cd + 6, true, nullptr,
+                           static_cast<double*>(out), nullptr, Cd + 8, nullptr,
+                           t->scratch.as<double>(), eld_scratch_threads()));
+    } else {
+        const int64_t nbits = (G.ts_n + 31) / 32;
+        HIP_TRY(t->list_b.ensure(sizeof(int64_t) * (size_t)n));
+        HIP_TRY(t->lb.ensure(sizeof(unsigned) * (size_t)nbits));  // the bitmap
+        unsigned* bits = t->lb.as<unsigned>();
+        HIP_TRY(launch_glb(t->stream, v, win, Cd + 6, t->list_b.as<int64_t>(), Cd + 7, nullptr,
+                           nullptr, Cd + 10));
+        hipLaunchKernelGGL(k_zero_words, grid_for(nbits), dim3(256), 0, t->stream, bits, nbits);
+        hipLaunchKernelGGL(k_mark_kept, grid_for(n), dim3(256), 0, t->stream,
+                           t->list_b.as<int64_t>(), Cd + 7, bits);
+        hipLaunchKernelGGL(k_keep_flags, grid_for(n), dim3(256), 0, t->stream, win, n, bits,
+                           static_cast<uint8_t*>(out));
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(j.leave());
+    return FSM_OK;
+}
+
+int fsm_stage_glb_dev(fsm_target* t, const fsm_query_desc* d, const double* q,
+                      const int64_t* win, int64_t n, double* out, void* stream) {
+    return stage_verify_dev(t, d, q, win, n, out, STAGE_GLB_VALUES, stream);
+}
+
+int fsm_stage_glb_filter_dev(fsm_target* t, const fsm_query_desc* d, const double* q,
+                             const int64_t* win, int64_t n, uint8_t* keep, void* stream) {
+    return stage_verify_dev(t, d, q, win, n, keep, STAGE_GLB_KEEP, stream);
+}
+
+int fsm_stage_eld_dev(fsm_target* t, const fsm_query_desc* d, const double* q,
+                      const int64_t* win, int64_t n, double* out, void* stream) {
+    return stage_verify_dev(t, d, q, win, n, out, STAGE_ELD_VALUES, stream);
 }
 
 int fsm_eld_pairs(int metric, int m, int w, double arg, const double* q, const double* cands,

@@ -241,6 +241,29 @@ int fsm_stage_glb_filter(fsm_target* t, const fsm_query_desc* desc, const double
 int fsm_stage_eld(fsm_target* t, const fsm_query_desc* desc, const double* query_raw,
                   const int64_t* windows, int64_t n, double* out);
 
+/* ---- stage-level entry points on device memory ----
+ * The same stages with the window list and the outputs in DEVICE memory
+ * (caller-owned), enqueued on the caller's `stream` (a hipStream_t; NULL: the
+ * null stream): the target's own stream first waits for what `stream` has
+ * enqueued, and `stream` then waits for the stage (events, no host
+ * synchronisation), so the outputs are ready when `stream` reaches that
+ * point.  The query stays a host array of m samples, normalised on the host
+ * as the reference does (src/fsmeld.cpp:622-643).  The window-list entries
+ * check their indices on the device first and read one word back
+ * (FSM_ERR_ARG when one lies outside [0, ts_n)).  Outputs as in the host
+ * entries above, element for element. */
+int fsm_stage_window_stats_dev(fsm_target* t, int m, int64_t xb, int64_t xe, double* mu,
+                               double* sigma, uint8_t* invalid, void* stream);
+int fsm_stage_lbip_dev(fsm_target* t, const fsm_query_desc* desc, const double* query_raw,
+                       double* lb, double* t_u, double* t_l, void* stream);
+int fsm_stage_glb_dev(fsm_target* t, const fsm_query_desc* desc, const double* query_raw,
+                      const int64_t* windows, int64_t n, double* out, void* stream);
+int fsm_stage_glb_filter_dev(fsm_target* t, const fsm_query_desc* desc,
+                             const double* query_raw, const int64_t* windows, int64_t n,
+                             uint8_t* keep, void* stream);
+int fsm_stage_eld_dev(fsm_target* t, const fsm_query_desc* desc, const double* query_raw,
+                      const int64_t* windows, int64_t n, double* out, void* stream);
+
 /* Exact elastic distance of n already-normalised rows cands[n][m] against q[m]
  * (dtw/erp/edr/lcss/msm, include/funcs.h:76-81, argument order of
  * src/fsmeld.cpp:170-183).  No target needed; all values must be finite
