@@ -391,41 +391,48 @@ __device__ __forceinline__ void ring_shift(double* Y, double yn) {
 // for the per-iteration overhead to matter.
 template <int METRIC>
 constexpr bool kEldPair = (METRIC == EDR || METRIC == LCSS || METRIC == SWALE);
-// MSM runs two iterations fused (msm_fused2).
+// MSM runs kEldFuseK iterations fused (msm_fused): four up to radius 22,
+// three beyond, where four would exceed 256 registers and spill (at W = 25,
+// 157.6 ms for two, 152.7 for three, 151.1 with 24 registers spilled for four
+// on the plain large workload).
 template <int METRIC>
 constexpr bool kEldFused = (METRIC == MSM);
-
-// Two MSM iterations (b and b + 1) in one skewed sweep: step s runs pair s
-// of iteration b (step_a(s), step_b(s - 1)) and then pair q = s - 1 of
-// iteration b + 1, whose ring is Y + 1 and query window X + 1.  Pair q of
-// iteration b + 1 reads iteration b's B[q - 1], B[q] (written by its pairs q
-// and q + 1 = s) and overwrites A[q] and B[q - 1], which iteration b no
-// longer reads; every cell gets exactly the operands of the two ad_step calls
-// in sequence.  Both pairs of step s take query row b + s (X[s + 1] and its
-// clamp bounds DL / DH[s + 1]), and column b + W - r's bounds col(r) serve
-// steps r .. r + 3, so each row is read and each column's bounds computed
-// once per two iterations.  Y[W + 2] holds iteration b + 1's new sample.  *r1:
-// the end cell's slot after iteration b.  The empty asm statements (values
-// in and out, memory clobbered) keep each step's arithmetic and the next
-// row's reads in place: left free, the compiler hoists every row of the
-// sweep into registers, which spill.
 template <int W>
-__device__ __forceinline__ void msm_fused2(double arg, double* A, double* B, const double* X,
-                                           const double* Y, const double* DL, const double* DH,
-                                           double* r1) {
+constexpr int kEldFuseK = (W <= 22) ? 4 : 3;
+
+// K MSM iterations (b .. b + K - 1) in one skewed sweep: step s runs, for
+// k = 0 .. K - 1 in order, pair q = s - k (step_a(q), step_b(q - 1)) of
+// iteration b + k, whose ring is Y + k and query window X + k.  Pair q of
+// iteration b + k reads iteration b + k - 1's B[q - 1], B[q] (written by that
+// iteration's pairs q and q + 1, steps s - 1 and s, before it) and A[q] (its
+// diagonal), and overwrites A[q] and B[q - 1], which iteration b + k - 1 no
+// longer reads: every cell gets exactly the operands of the K ad_step calls
+// in sequence.  All pairs of step s take query row b + s (X[s + 1] and its
+// clamp bounds DL / DH[s + 1]), and column b + W - r's bounds col(r) serve
+// steps r .. r + 2K - 1, so each row is read and each column's bounds
+// computed once per K iterations.  Y[W + 2 .. W + K] hold iterations b + 1 ..
+// b + K - 1's new samples.  rk[k], k < K - 1: the end cell's slot after
+// iteration b + k.  The empty asm statements (values in and out, memory
+// clobbered) keep each pair's arithmetic and the next row's reads in place:
+// left free, the compiler hoists every row of the sweep into registers, which
+// spill.
+template <int W, int K>
+__device__ __forceinline__ void msm_fused(double arg, double* A, double* B, const double* X,
+                                          const double* Y, const double* DL, const double* DH,
+                                          double* rk) {
     const double S = kEldSentinel;
-    double xr[W + 3], lr[W + 3], hr[W + 3];
-    double cl[W + 2], ch[W + 2];  // col(r) at r + 1: min(0, dyn), max(0, dyn)
+    double xr[W + K + 1], lr[W + K + 1], hr[W + K + 1];  // rows 1 .. W + K
+    double cl[W + K + 1], ch[W + K + 1];  // col(r) at r + K: min(0, dyn), max(0, dyn)
     auto col = [&](int r) {  // dyn = y_{j-1} - y_j of column j = b + W - r
         const double d = Y[W - r] - Y[W - r + 1];
-        cl[r + 1] = fmin(d, 0.0);
-        ch[r + 1] = fmax(d, 0.0);
+        cl[r + K] = fmin(d, 0.0);
+        ch[r + K] = fmax(d, 0.0);
     };
     // msm_cell with the row's and the column's clamp bounds given
     auto cellf = [&](double up, double left, double diag, double e, int row, int c) {
         const double c1 = diag + fabs(e);
         const double s2 = fabs(fmax(fmin(e, hr[row]), lr[row]));
-        const double s3 = fabs(fmax(fmin(e, ch[c + 1]), cl[c + 1]));
+        const double s3 = fabs(fmax(fmin(e, ch[c + K]), cl[c + K]));
         const double c2 = up + (arg + s2);
         const double c3 = left + (arg + s3);
         return dmin3(c1, c2, c3);
@@ -433,38 +440,32 @@ __device__ __forceinline__ void msm_fused2(double arg, double* A, double* B, con
     xr[1] = X[1];
     lr[1] = DL[1];
     hr[1] = DH[1];
-    col(-1);
 #pragma unroll
-    for (int s = 0; s <= W + 1; s++) {
-        if (s + 2 <= W + 2) {
+    for (int r = -(K - 1); r < 0; r++) col(r);
+#pragma unroll
+    for (int s = 0; s <= W + K - 1; s++) {
+        if (s + 2 <= W + K) {
             xr[s + 2] = X[s + 2];
             lr[s + 2] = DL[s + 2];
             hr[s + 2] = DH[s + 2];
         }
-        if (s <= W) {  // iteration b, pair s
-            col(s);
-            const double up = (s >= 1) ? B[s - 1] : S;
-            const double left = (s <= W - 1) ? B[s] : S;
-            A[s] = cellf(up, left, A[s], xr[s + 1] - Y[W - s + 1], s + 1, s);
-            if (s >= 1) {
-                B[s - 1] = cellf(A[s - 1], A[s], B[s - 1], xr[s + 1] - Y[W - s + 2], s + 1, s - 1);
-                asm volatile("" : "+v"(A[s]), "+v"(B[s - 1])::"memory");
-            } else {
-                asm volatile("" : "+v"(A[s])::"memory");
-            }
-            if (s == ((W & 1) ? (W + 1) / 2 : W / 2)) *r1 = (W & 1) ? B[(W - 1) / 2] : A[W / 2];
-        }
-        if (s >= 1) {  // iteration b + 1, pair q (ring Y + 1, window X + 1)
-            const int q = s - 1;
+        if (s <= W) col(s);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int q = s - k;
+            if (q < 0 || q > W) continue;
             const double up = (q >= 1) ? B[q - 1] : S;
             const double left = (q <= W - 1) ? B[q] : S;
-            A[q] = cellf(up, left, A[q], xr[s + 1] - Y[W - q + 2], s + 1, q - 1);
+            A[q] = cellf(up, left, A[q], xr[s + 1] - Y[W - q + 1 + k], s + 1, q - k);
             if (q >= 1) {
-                B[q - 1] = cellf(A[q - 1], A[q], B[q - 1], xr[s + 1] - Y[W - q + 3], s + 1, q - 2);
+                B[q - 1] = cellf(A[q - 1], A[q], B[q - 1], xr[s + 1] - Y[W - q + 2 + k], s + 1,
+                                 q - 1 - k);
                 asm volatile("" : "+v"(A[q]), "+v"(B[q - 1])::"memory");
             } else {
                 asm volatile("" : "+v"(A[q])::"memory");
             }
+            if (k < K - 1 && q == ((W & 1) ? (W + 1) / 2 : W / 2))
+                rk[k] = (W & 1) ? B[(W - 1) / 2] : A[W / 2];
         }
     }
 }
@@ -645,14 +646,18 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
     // the query side X[k] = xs[b - 1 + k] is read from LDS (lanes run at
     // their own iteration b); the candidate ring Y stays in registers
     VT A[W + 1], B[W + 1];
-    double Y[W + 4];  // kEldPair: Y[W + 2], Y[W + 3] take the two new samples
+    // kEldPair: Y[W + 2], Y[W + 3] take the two new samples; kEldFused: Y[W + 2 ..
+    // W + 1 + kEldFuseK] the fused iterations' new samples
+    double Y[W + 2 + (kEldFused<METRIC> ? kEldFuseK<W> : 2)];
 #pragma unroll
     for (int k = 0; k <= W; k++) A[k] = B[k] = S;
 #pragma unroll
     for (int k = 0; k < W + 2; k++) Y[k] = 0.0;
     const double* tp = raw ? a.T : rows;
     double mu = 0.0, sd = 1.0, total = 0.0, p = 0.0;
-    double y1n = kPad, y2n = kPad;  // kEldFused: the next two new samples
+    double yn[kEldFuseK<W>];  // kEldFused: the next pass's new samples
+#pragma unroll
+    for (int k = 0; k < kEldFuseK<W>; k++) yn[k] = kPad;
     int64_t ci = 0, g = 0;
     int b = 0, it = 0;
     unsigned live_iters = 0;
@@ -703,13 +708,12 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
                         if (j >= 0 && j < m) v = raw ? (tp[j] - mu) / sd : tp[j];
                         Y[k] = v;
                     }
-                    if constexpr (kEldFused<METRIC>) {  // samples BS + W + 1, BS + W + 2
-                        y1n = (BS + W + 1 < m) ? (raw ? (tp[BS + W + 1] - mu) / sd
-                                                      : tp[BS + W + 1])
-                                               : kPad;
-                        y2n = (BS + W + 2 < m) ? (raw ? (tp[BS + W + 2] - mu) / sd
-                                                      : tp[BS + W + 2])
-                                               : kPad;
+                    if constexpr (kEldFused<METRIC>) {  // samples BS + W + 1 + k
+#pragma unroll
+                        for (int k = 0; k < kEldFuseK<W>; k++) {
+                            const int j = BS + W + 1 + k;
+                            yn[k] = (j < m) ? (raw ? (tp[j] - mu) / sd : tp[j]) : kPad;
+                        }
                     }
                     p = 0.0;
                     if (check) {
@@ -753,34 +757,44 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
             return (jn == jj) ? yv : kPad;
         };
         if constexpr (kEldFused<METRIC>) {
-            // iterations b and b + 1 (msm_fused2); a lane whose candidate ends
-            // at b keeps its value (r1), as in the pair path below.  The
-            // next pass's two samples are loaded here and normalised after
-            // the sweep.
-            Y[W + 2] = y1n;
-            const int jn1 = b + W + 3, jn2 = b + W + 4;
-            const int jj1 = min(max(jn1, 0), m - 1), jj2 = min(max(jn2, 0), m - 1);
-            const double t1 = tp[jj1], t2 = tp[jj2];
-            double r1;
-            msm_fused2<W>(arg, A, B, xs + (b - 1), Y, dxl + kXPad + (b - 1),
-                          dxh + kXPad + (b - 1), &r1);
-            if (check) {
-                int K = b + KOFF;
-                if (K >= 0 && K < m) p += consume_lb<METRIC>(Y[KOFF + 1], lqs[K], uqs[K], arg);
-                K = b + 1 + KOFF;
-                if (K >= 0 && K < m) p += consume_lb<METRIC>(Y[KOFF + 2], lqs[K], uqs[K], arg);
-            }
-            Y[W + 3] = y2n;
+            // iterations b .. b + K - 1 (msm_fused); a lane whose candidate
+            // ends at b + k < b + K - 1 keeps that iteration's value (rk[k]),
+            // as in the pair path below.  The next pass's samples are loaded
+            // here and normalised after the sweep.
+            constexpr int K = kEldFuseK<W>;
 #pragma unroll
-            for (int k = 0; k < W + 2; k++) Y[k] = Y[k + 2];
-            y1n = (jn1 == jj1) ? (raw ? (t1 - mu) / sd : t1) : kPad;
-            y2n = (jn2 == jj2) ? (raw ? (t2 - mu) / sd : t2) : kPad;
-            const bool fin1 = busy && b == bend;
-            const bool fin = busy && (fin1 || b + 1 == bend);
-            live_iters += busy ? (fin1 ? 1u : 2u) : 0u;
-            if (busy) b += 2;
+            for (int k = 0; k + 1 < K; k++) Y[W + 2 + k] = yn[k];
+            double tn[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) tn[k] = tp[min(max(b + K + k + W + 1, 0), m - 1)];
+            double rk[K > 1 ? K - 1 : 1];
+            msm_fused<W, K>(arg, A, B, xs + (b - 1), Y, dxl + kXPad + (b - 1),
+                            dxh + kXPad + (b - 1), rk);
+            if (check) {
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const int Ks = b + k + KOFF;
+                    if (Ks >= 0 && Ks < m)
+                        p += consume_lb<METRIC>(Y[KOFF + 1 + k], lqs[Ks], uqs[Ks], arg);
+                }
+            }
+            Y[W + 1 + K] = yn[K - 1];
+#pragma unroll
+            for (int k = 0; k < W + 2; k++) Y[k] = Y[k + K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int j = b + K + k + W + 1;
+                yn[k] = (j >= 0 && j < m) ? (raw ? (tn[k] - mu) / sd : tn[k]) : kPad;
+            }
+            const int kf = bend - b;  // the iteration, if any, this pass ends the candidate in
+            const bool fin = busy && kf >= 0 && kf < K;
+            live_iters += busy ? (unsigned)(fin ? kf + 1 : K) : 0u;
+            if (busy) b += K;
             if (__any(fin)) {
-                const double rv = fin1 ? r1 : ((W & 1) ? B[(W - 1) / 2] : A[W / 2]);
+                double rv = (W & 1) ? B[(W - 1) / 2] : A[W / 2];
+#pragma unroll
+                for (int k = 0; k + 1 < K; k++)
+                    if (kf == k) rv = rk[k];
                 const double d =
                     eld_result<METRIC>(band_value<METRIC>(rv, 2 * bend + W + (W & 1)), m);
                 if (keep_all) {
@@ -792,7 +806,8 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
                 }
                 busy = busy && !fin;
             }
-            if (check && ((++it) & 1) == 0)
+            // about every fourth iteration, as below
+            if (check && (K >= 3 || ((++it) & 1) == 0))
                 busy = busy && ad_alive<METRIC, W>(A, B, b - 1, M, m, thr, total, p);
             continue;
         }
