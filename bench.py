@@ -131,6 +131,9 @@ def main():
     ap.add_argument("--rates", default="50,200,1000")
     ap.add_argument("--requests", type=int, default=60, help="arrivals per rate (serve)")
     ap.add_argument("--workers", type=int, default=8, help="server workers (serve)")
+    ap.add_argument("--prec-m", default="128,1024", help="query lengths (precision)")
+    ap.add_argument("--prec-metrics", default="dtw,erp,edr,lcss,msm",
+                    help="metrics (precision)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -532,41 +535,98 @@ def run_small(args):
         "speedup_vs_cpu": round(cpu_s / gpu_s, 1), "same_windows_as_cpu": same}), flush=True)
 
 
+def _mfma_flops(n, m):
+    """FP32 matrix-core flops one LB_IP launch issues in its contraction: per
+    segment ceil(windows / 256) tiles of 2 sides x ceil((m + 15) / 4) steps of
+    a 16x16x4 product (2048 flops)."""
+    import fsmeld_amd as F
+    G = F.geometry(n, m)
+    tiles = G["forsize"] * -(-G["k_m_l"] // 256)
+    if G["tail"]:
+        tiles += -(-G["tail"] // 256)
+    return tiles * 2 * ((m + 18) // 4) * 2048
+
+
 def run_precision(args):
+    """Config 3: LB_IP's sliding sums exact (64-bit fixed point) against the
+    FP32 matrix-core contraction with certified slack, on the plain large
+    target at m in {128, 1024}: LB error, pruning, LB_IP time, matrix-core
+    rate, and identity of cutoff / search answers with each other and with
+    the exact scan."""
     import fsmeld_amd as F
     from fsmeld_amd import workloads
     t = workloads.plain_large()["target"]
     T = F.Target(t)
     rows = []
-    for m, w in ((128, 6), (1024, 51)):
+    lengths = [int(x) for x in args.prec_m.split(",")]
+    chosen = args.prec_metrics.split(",")
+    for m in lengths:
+        w = m // 20
+        flops = _mfma_flops(t.size, m)
         for metric, arg in workloads.CONFIG2_METRICS:
+            if metric not in chosen:
+                continue
             q, _ = F.synth_query(3 * m + 1, t, m, 0.1)
             c = T.cutoff(q, m, 10, metric, w, arg)
+            c32 = T.cutoff(q, m, 10, metric, w, arg, lbip_f32=True)
             s = T.search(q, m, c.threshold, metric, w, arg)
             e = T.exact(q, m, c.threshold, metric, w, arg)
             extra = np.setdiff1d(s.indices, e.indices).size
             missing = np.setdiff1d(e.indices, s.indices)
             at_cut = bool(np.all(e.distances[np.isin(e.indices, missing)] == c.threshold))
+            ms = {False: [], True: []}
+            runs = {}
+            for rep in range(3):
+                for f32 in (False, True):
+                    r = T.search(q, m, c.threshold, metric, w, arg, lbip_f32=f32)
+                    ms[f32].append(r.stats["ms_lbip"])
+                    runs[f32] = r
+            s32 = runs[True]
             lb, _, _ = T.lbip(q, m, metric, w, arg)
-            win = np.flatnonzero(np.isfinite(lb))[::65537]
+            lb32, _, _ = T.lbip(q, m, metric, w, arg, lbip_f32=True)
+            fin = np.isfinite(lb)
+            za, zb = lb32[fin], lb[fin]
+            if metric in ("dtw", "erp"):
+                za, zb = za * za, zb * zb
+            rel = (zb - za) / np.maximum(zb, 1e-300)
+            win = np.flatnonzero(fin)[::65537]
             d = T.eld(q, m, metric, w, arg, win)
             viol = int(np.sum(lb[win] > d + 1e-9 * np.maximum(1.0, d)))
-            st = s.stats
-            rows.append(dict(m=m, w=w, metric=metric, cutoff=c.threshold,
-                             lbip_pruned_frac=round(1.0 - st["after_lbip"] / st["valid"], 6),
-                             glb_pruned_frac=round(1.0 - st["after_glb"] / st["valid"], 6),
-                             matches=int(st["matches"]),
-                             search_equals_exact=(extra == 0 and (missing.size == 0 or at_cut)),
-                             lb_above_eld=viol, lb_checked=int(win.size),
-                             search_ms=round(st["ms_total"], 2)))
+            viol32 = int(np.sum(lb32[win] > d + 1e-9 * np.maximum(1.0, d)))
+            st, st32 = s.stats, s32.stats
+            t64, t32 = float(np.median(ms[False])), float(np.median(ms[True]))
+            rows.append(dict(
+                m=m, w=w, metric=metric, cutoff=c.threshold,
+                lbip_pruned_frac=round(1.0 - st["after_lbip"] / st["valid"], 6),
+                lbip_f32_pruned_frac=round(1.0 - st32["after_lbip"] / st32["valid"], 6),
+                glb_pruned_frac=round(1.0 - st["after_glb"] / st["valid"], 6),
+                matches=int(st["matches"]),
+                search_equals_exact=(extra == 0 and (missing.size == 0 or at_cut)),
+                f32_equals_exact_path=bool(
+                    np.array_equal(s32.indices, s.indices)
+                    and np.array_equal(s32.distances, s.distances)
+                    and c32.threshold == c.threshold and np.array_equal(c32.indices, c.indices)),
+                f32_lb_rel_below_exact_max=float(rel.max()),
+                f32_lb_rel_below_exact_mean=float(rel.mean()),
+                f32_lb_above_exact=int(np.sum(rel < -1e-12)),
+                lb_above_eld=viol, f32_lb_above_eld=viol32, lb_checked=int(win.size),
+                lbip_ms=round(t64, 3), lbip_f32_ms=round(t32, 3),
+                mfma_tflops=round(flops / t32 / 1e9, 2),
+                mfma_frac=round(flops / t32 / 1e9 / 157.3, 4),
+                search_ms=round(st["ms_total"], 2), search_f32_ms=round(st32["ms_total"], 2)))
             log(f"m={m} {metric}: {rows[-1]}")
     print(json.dumps({
-        "metric": "LB_IP precision at long queries (config 3): pruning, identity, admissibility",
-        "value": float(np.mean([r["lbip_pruned_frac"] for r in rows])), "unit": "fraction pruned",
-        "n_gpus": 1, "higher_is_better": True, "dtype": "f64 (LB_IP sums: exact 64-bit fixed point)",
+        "metric": "LB_IP precision at long queries (config 3): exact fixed-point sums against "
+                  "the FP32 matrix-core contraction with certified slack",
+        "value": float(np.mean([r["lbip_f32_pruned_frac"] for r in rows])),
+        "unit": "fraction pruned by the FP32 bound",
+        "n_gpus": 1, "higher_is_better": True,
+        "dtype": "LB_IP sums: exact 64-bit fixed point (f64 penalties) and FP32 MFMA "
+                 "(v_mfma_f32_16x16x4_f32); GLB / ELD f64",
         "data": "synthetic",
         "config": {"workload": "precision: plain large 2^26 target, m in {128, 1024}, w = 5% m, "
                                "five metrics, cutoffs k=10"},
+        "mfma_peak_tflops": 157.3,
         "rows": rows}), flush=True)
 
 

@@ -51,7 +51,8 @@ class _Desc(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("metric", ctypes.c_int), ("m", ctypes.c_int),
                 ("w", ctypes.c_int), ("arg", ctypes.c_double), ("cutoff", ctypes.c_double),
                 ("k", ctypes.c_int), ("exclude_begin", ctypes.c_int64),
-                ("exclude_end", ctypes.c_int64), ("raw", ctypes.c_int)]
+                ("exclude_end", ctypes.c_int64), ("raw", ctypes.c_int),
+                ("lbip_f32", ctypes.c_int)]
 
 
 class _Stats(ctypes.Structure):
@@ -65,7 +66,7 @@ class _Stats(ctypes.Structure):
                 ("ms_eld", ctypes.c_double), ("ms_total", ctypes.c_double),
                 ("eld_cells", ctypes.c_uint64), ("ms_stats_finish", ctypes.c_double),
                 ("ms_env", ctypes.c_double), ("near_rechecked", ctypes.c_uint64),
-                ("near_changed", ctypes.c_uint64)]
+                ("near_changed", ctypes.c_uint64), ("lbip_f32", ctypes.c_int)]
 
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -185,10 +186,11 @@ def _take_result(r) -> QueryResult:
 
 
 def _spec_desc(spec) -> _Desc:
-    """Descriptor of a query spec: dict(mode, m, metric, w, arg, cutoff | k, exclude, raw)."""
+    """Descriptor of a query spec: dict(mode, m, metric, w, arg, cutoff | k, exclude, raw,
+    lbip_f32)."""
     return _desc(spec["mode"], spec["metric"], spec["m"], spec["w"], spec.get("arg", 0.0),
                  spec.get("cutoff", 0.0), spec.get("k", 0), spec.get("exclude"),
-                 spec.get("raw", False))
+                 spec.get("raw", False), spec.get("lbip_f32", False))
 
 
 def query_bytes(n, metric, m, w) -> int:
@@ -197,9 +199,11 @@ def query_bytes(n, metric, m, w) -> int:
                                     else int(metric), int(m), int(w)))
 
 
-def _desc(mode, metric, m, w, arg, cutoff_value=0.0, k=0, exclude=None, raw=False) -> _Desc:
+def _desc(mode, metric, m, w, arg, cutoff_value=0.0, k=0, exclude=None, raw=False,
+          lbip_f32=False) -> _Desc:
     d = _Desc()
     d.raw = 1 if raw else 0
+    d.lbip_f32 = 1 if lbip_f32 else 0
     d.mode = MODES[mode]
     d.metric = METRICS[metric] if isinstance(metric, str) else int(metric)
     d.m, d.w, d.arg = int(m), int(w), float(arg)
@@ -246,11 +250,12 @@ class Target:
 
     # ---- query-level ----
     def query(self, mode, query, m, metric, w, arg, cutoff_value=0.0, k=0,
-              exclude=None, raw=False) -> QueryResult:
+              exclude=None, raw=False, lbip_f32=False) -> QueryResult:
         """raw=True: raw (non-normalised) matching, the reference's
-        NormalizationMode::Raw (paper App. D)."""
+        NormalizationMode::Raw (paper App. D).  lbip_f32=True: LB_IP's sliding
+        sums from the FP32 matrix-core contraction with certified slack."""
         q = _f64(query)
-        d = _desc(mode, metric, m, w, arg, cutoff_value, k, exclude, raw)
+        d = _desc(mode, metric, m, w, arg, cutoff_value, k, exclude, raw, lbip_f32)
         r = _VP()
         _check(_lib.fsm_query(self._h, ctypes.byref(d), _ptr(q), int(q.size), ctypes.byref(r)))
         return _take_result(r)
@@ -299,12 +304,14 @@ class Target:
         return t
 
     def search(self, query, m, cutoff_value, metric, w, arg, exclude=None,
-               raw=False) -> QueryResult:
+               raw=False, lbip_f32=False) -> QueryResult:
         return self.query("search", query, m, metric, w, arg, cutoff_value=cutoff_value,
-                          exclude=exclude, raw=raw)
+                          exclude=exclude, raw=raw, lbip_f32=lbip_f32)
 
-    def cutoff(self, query, m, k, metric, w, arg, exclude=None, raw=False) -> QueryResult:
-        return self.query("cutoff", query, m, metric, w, arg, k=k, exclude=exclude, raw=raw)
+    def cutoff(self, query, m, k, metric, w, arg, exclude=None, raw=False,
+               lbip_f32=False) -> QueryResult:
+        return self.query("cutoff", query, m, metric, w, arg, k=k, exclude=exclude, raw=raw,
+                          lbip_f32=lbip_f32)
 
     def exact_topk(self, query, m, k, metric, w, arg, exclude=None, raw=False) -> QueryResult:
         """FSMELD_exact with top_k = k: every window at or below the k-th
@@ -326,9 +333,9 @@ class Target:
                                            _ptr(sg), _ptr(inv, _U8)))
         return mu, sg, inv.astype(bool)
 
-    def lbip(self, query, m, metric, w, arg, raw=False):
+    def lbip(self, query, m, metric, w, arg, raw=False, lbip_f32=False):
         q = _f64(query)
-        d = _desc("search", metric, m, w, arg, raw=raw)
+        d = _desc("search", metric, m, w, arg, raw=raw, lbip_f32=lbip_f32)
         n = len(self) - m + 1
         lb = np.empty(n)
         seg = _geometry(len(self), m)["S"]

@@ -74,6 +74,14 @@ bool raw_matching() {
     return e && std::string(e) == "raw";
 }
 
+// FSMELD_LBIP=f32 computes LB_IP's sliding sums on the FP32 matrix cores
+// with a certified slack (a valid, slightly looser bound); anything else, or
+// unset, the exact fixed-point sums
+bool lbip_f32() {
+    const char* e = std::getenv("FSMELD_LBIP");
+    return e && std::string(e) == "f32";
+}
+
 bool quiet_matches() {  // src/fsmeld.cpp:998-1001
     const char* e = std::getenv("FSMELD_QUIET_MATCHES");
     return e && std::string(e) != "0" && std::string(e) != "false";
@@ -167,6 +175,7 @@ int main(int argc, char** argv) {
         d.exclude_begin = begin;
         d.exclude_end = end;
         d.raw = raw_matching() ? 1 : 0;
+        d.lbip_f32 = lbip_f32() ? 1 : 0;
         if (command == "cutoff") {
             const auto k = parse_arg<long long>(argv[5], "k");
             if (k < 1 || k > std::numeric_limits<int>::max() ||

@@ -193,6 +193,7 @@ int validate(const fsm_query_desc* d, int64_t N, const double* q, int64_t qlen, 
     const double mean = sum / d->m;
     const double sigma = std::sqrt(sq / d->m - mean * mean);
     if (d->raw != 0 && d->raw != 1) return fail(FSM_ERR_ARG, "Unknown normalization mode");
+    if (d->lbip_f32 != 0 && d->lbip_f32 != 1) return fail(FSM_ERR_ARG, "lbip_f32 must be 0 or 1");
     if (!d->raw && (!std::isfinite(sigma) || sigma <= 1e-12))
         return fail(FSM_ERR_ARG,
                     "Query cannot be z-normalized: non-finite or near-zero standard deviation.");
@@ -541,6 +542,7 @@ QueryParams params_of(const fsm_query_desc* d) {
     const bool excl = !(d->exclude_begin == -1 && d->exclude_end == -1);
     P.xb = excl ? d->exclude_begin : -1;
     P.xe = excl ? d->exclude_end : -1;
+    P.lbip_f32 = d->lbip_f32 != 0;
     return P;
 }
 
@@ -873,6 +875,8 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
     st.forsize = G.forsize;
     st.tail = G.tail;
     st.m_k = G.m_k;
+    if (!lbip_f32_supported(G)) P.lbip_f32 = 0;
+    st.lbip_f32 = P.lbip_f32 && (d->mode == FSM_MODE_SEARCH || d->mode == FSM_MODE_CUTOFF);
     auto bail = [&](int code) { return code; };
     unsigned long long C[16] = {};
 

@@ -57,6 +57,10 @@ struct QueryParams {
     double arg;
     double cutoff;
     int64_t xb, xe;  // exclusion [xb, xe), xb < 0: none
+    // LB_IP's sliding sums: 0 exact fixed point, 1 the FP32 matrix-core
+    // contraction with certified slack (segments of m_k <= 8192; longer ones
+    // keep the fixed-point sums)
+    int lbip_f32;
 };
 
 // Window statistics live segment-major: entry s*kml + j is window j of
@@ -142,6 +146,8 @@ struct LbipArgs {
 };
 size_t lbip_slab_bytes(const Geometry& G);
 size_t lbip_lds_bytes(const Geometry& G);
+// true when the FP32 contraction serves queries of this geometry
+bool lbip_f32_supported(const Geometry& G);
 hipError_t launch_lbip(hipStream_t st, const LbipArgs& a);
 
 // Compaction: windows g in [0, ts_n) that are valid, not excluded and (for

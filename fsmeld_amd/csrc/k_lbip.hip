@@ -89,15 +89,22 @@ __host__ __device__ inline size_t zone_bytes(int npad, int nt, int kml) {
     return c > t ? c : t;
 }
 
+// FP32 matrix-core bounds (kDual only): words read past the end of the
+// upper mask bitmap (at most 7, see f32_bounds)
+constexpr int kMaskPadWords = 8;
+
 size_t lbip_lds_bytes(const Geometry& G) {
     const int nthr = G.m_k - 2 * G.m + 2;
     const int npad = npad_of(nthr);
     const int nw = (G.m_k + 63) / 64;
     const int nt = lbip_threads(npad, G.S);
     const int lay = lbip_layout(G.m_k);
+    // (+ kMaskPadWords: the FP32 contraction reads a few words past the
+    // upper mask's end, whose bits only ever meet zero penalties)
     if (lay != kSlab)
         return sizeof(double) * (size_t)G.m_k * (lay == kDual ? 2 : 1) +
-               zone_bytes(npad, nt, (int)G.kml) + 2 * sizeof(uint64_t) * (size_t)nw;
+               zone_bytes(npad, nt, (int)G.kml) +
+               sizeof(uint64_t) * (2 * (size_t)nw + kMaskPadWords);
     // kSlab: the keys in global scratch; the threshold scratch, then the
     // penalty prefix sums and the mask bitmaps, in LDS
     const size_t late = sizeof(int64_t) * 2 * (size_t)(G.m + 1) + sizeof(uint64_t) * 4 * (size_t)nw;
@@ -106,6 +113,8 @@ size_t lbip_lds_bytes(const Geometry& G) {
 
 // Per-block global scratch: the threshold's full-sort buffer (npad keys);
 // with kSlab also the van Herk arrays and both envelopes.
+bool lbip_f32_supported(const Geometry& G) { return lbip_layout(G.m_k) == kDual; }
+
 size_t lbip_slab_bytes(const Geometry& G) {
     const int nthr = G.m_k - 2 * G.m + 2;
     const int npad = npad_of(nthr);
@@ -919,6 +928,69 @@ __device__ inline int64_t list_sum(const int* P, int n, const uint64_t* gw, cons
     return acc;
 }
 
+// ------------------------------------------ FP32 matrix-core bounds ---
+// The sliding inner products of src/sip.cpp as a Hankel-mask x shifted-penalty
+// contraction on the matrix cores (v_mfma_f32_16x16x4_f32).  For a tile of 256
+// windows j0 + 16 r + n (r, n < 16):
+//   z[j0 + 16 r + n] = sum_k A[r][k] B[k][n],  A[r][k] = g[j0 + 16 r + k],
+//   B[k][n] = h[k - n] (zero outside [0, m)),  k < 4 nst, nst = ceil((m + 15) / 4),
+// the lower side (g_l, h_l) and the upper side (g_u, h_u) in accumulators of
+// their own.  Operand lanes: A[lane % 16][lane / 16], B[lane / 16][lane % 16];
+// result D[4 (lane / 16) + i][lane % 16] in element i.  Each lane keeps 64 mask
+// bits of its row in registers (16 steps of 4) and reads B from the padded
+// penalty rows hf[15 + k - n].
+//
+// Certification: the penalties enter rounded down to FP32 and every product is
+// g h with g in {0, 1}, so the terms are non-negative and exact.  Any order of
+// at most n = 4 nst FP32 additions of them, each within one ulp (u = 2^-23,
+// whatever the unit's rounding), leaves the computed s^ within gamma_n =
+// n u / (1 - n u) of the terms' exact sum s, so s >= s^ / (1 + gamma_n) >=
+// s^ (1 - gamma_n); subnormal results add at most 2^-149 per addition.  The
+// bound is s^ (1 - gamma_n - 2^-48) - n 2^-149 per side, clamped at 0 (the
+// 2^-48 covers the FP64 multiply and add that form it): a certified lower
+// bound of the FP32 penalties' sum, itself at most the FP64 penalties' sum.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// the 64 mask bits from bit 64 wi + sh on, words lo = [wi], hi = [wi + 1]
+__device__ __forceinline__ uint64_t bits64(uint64_t lo, uint64_t hi, int sh) {
+    return (lo >> sh) | ((hi << 1) << (63 - sh));
+}
+
+__host__ __device__ inline int f32_steps(int m) { return (m + 18) / 4; }
+__host__ __device__ inline int f32_hlen(int m) { return 4 * f32_steps(m) + 16; }
+
+// Both sides' sums of the tile at j0 for this lane's four windows.
+__device__ __forceinline__ void f32_tile(const uint64_t* GLw, const uint64_t* GUw,
+                                         const float* HLf, const float* HUf, int j0, int nst,
+                                         int lane, f32x4& al, f32x4& au) {
+    const int ra = lane & 15, kq = lane >> 4;
+    const int p = j0 + 16 * ra + kq;  // A[ra][kq]'s bit at step 0
+    const int sh = p & 63;
+    int wi = p >> 6;
+    uint64_t l0 = GLw[wi], u0 = GUw[wi];
+    const float* hl = HLf + 15 + kq - ra;
+    const float* hu = HUf + 15 + kq - ra;
+    al = f32x4{0.f, 0.f, 0.f, 0.f};
+    au = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int t0 = 0; t0 < nst; t0 += 16) {
+        const uint64_t l1 = GLw[wi + 1], u1 = GUw[wi + 1];
+        const uint64_t RL = bits64(l0, l1, sh), RU = bits64(u0, u1, sh);
+        const uint32_t rl[2] = {(uint32_t)RL, (uint32_t)(RL >> 32)};
+        const uint32_t ru[2] = {(uint32_t)RU, (uint32_t)(RU >> 32)};
+#pragma unroll
+        for (int tt = 0; tt < 16; tt++) {
+            if (t0 + tt >= nst) break;
+            const float a_l = (float)((rl[tt >> 3] >> (4 * (tt & 7))) & 1u);
+            const float a_u = (float)((ru[tt >> 3] >> (4 * (tt & 7))) & 1u);
+            al = __builtin_amdgcn_mfma_f32_16x16x4f32(a_l, hl[4 * (t0 + tt)], al, 0, 0, 0);
+            au = __builtin_amdgcn_mfma_f32_16x16x4f32(a_u, hu[4 * (t0 + tt)], au, 0, 0, 0);
+        }
+        l0 = l1;
+        u0 = u1;
+        wi++;
+    }
+}
+
 // ------------------------------------------------------------------ kernel ---
 // kDual / kSingle keep the normalised envelopes in LDS (X, and X2 = -L_n for
 // kDual), then the penalty prefix sums and mask transitions in X's space;
@@ -928,9 +1000,12 @@ __device__ inline int64_t list_sum(const int* P, int n, const uint64_t* gw, cons
 // global scratch, with U_n, L_n and the keys there too.
 // One kernel per layout, so that none carries another's registers; every
 // workgroup size fits four waves per SIMD.
-template <int NT, int LAYOUT>
+// F32 (kDual only): the bounds from the FP32 matrix-core contraction
+// (f32_tile) instead of the exact fixed-point run sums.
+template <int NT, int LAYOUT, bool F32 = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
     k_lbip(LbipArgs a) {
+    static_assert(!F32 || LAYOUT == kDual, "FP32 bounds need the kDual layout");
     extern __shared__ unsigned char lds_raw[];
     __shared__ ArgBest sh_best[NT / 64];
     __shared__ double sh_sum[NT / 64];
@@ -1210,96 +1285,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
             if (a.TLo) a.TLo[s] = t_l;
         }
 
-        // 5. penalties in fixed point: hI = floor(h * 2^F), exact prefix sums
-        double hs = 0.0;
-        for (int i = threadIdx.x; i < m; i += blockDim.x) {
-            double hl, hu;
-            penalties(metric, Q.qn[i], t_l, t_u, C, &hl, &hu);
-            hs += hl + hu;
-        }
-        hs = block_sum(hs, sh_sum);
-        const bool finite = __builtin_isfinite(hs);
-        int F = 0;
-        if (finite && hs > 0.0) {
-            F = 60 - ilogb(hs);
-            if (F > 1000) F = 1000;
-        }
-        {
-            const int per = (m + NT - 1) / NT;
-            const int i0 = threadIdx.x * per;
-            const int i1 = (i0 + per < m) ? i0 + per : m;
-            int64_t tl_sum = 0, tu_sum = 0;
-            for (int i = i0; i < i1; i++) {
-                double hl, hu;
-                penalties(metric, Q.qn[i], t_l, t_u, C, &hl, &hu);
-                if (finite) {
-                    tl_sum += (int64_t)floor(ldexp(hl, F));
-                    tu_sum += (int64_t)floor(ldexp(hu, F));
-                }
-            }
-            const int64_t ol = block_exscan(tl_sum, sh_scan);
-            const int64_t ou = block_exscan(tu_sum, sh_scan);
-            int64_t cl = ol, cu = ou;
-            if (threadIdx.x == 0) {
-                CHl[0] = 0;
-                CHu[0] = 0;
-            }
-            for (int i = i0; i < i1; i++) {
-                double hl, hu;
-                penalties(metric, Q.qn[i], t_l, t_u, C, &hl, &hu);
-                if (finite) {
-                    cl += (int64_t)floor(ldexp(hl, F));
-                    cu += (int64_t)floor(ldexp(hu, F));
-                }
-                CHl[i + 1] = cl;
-                CHu[i + 1] = cu;
-            }
-        }
-
-        // 6. mask transitions: bitmaps, and for kDual / kSingle their
-        //    ascending lists (positions t with g[t] != g[t - 1]) after the
-        //    bitmaps in X's space, one word per thread (nw <= NT), when they
-        //    fit (lcap entries each)
-        for (int wi = threadIdx.x; wi < nw; wi += NT) {
-            const uint64_t pl = wi ? GLw[wi - 1] >> 63 : 0ull;
-            const uint64_t pu = wi ? GUw[wi - 1] >> 63 : 0ull;
-            TLw[wi] = GLw[wi] ^ ((GLw[wi] << 1) | pl);
-            TUw[wi] = GUw[wi] ^ ((GUw[wi] << 1) | pu);
-        }
-        __syncthreads();
-        int* PL = (int*)(TUw + nw);
-        const int lcap = (LAYOUT == kDual) ? mk : mk / 2;
-        int* PU = PL + lcap;
-        int nl = 0, nu = 0;
-        bool lists = false;
-        if constexpr (xl) {
-            const int wi = threadIdx.x;
-            const int64_t c = (wi < nw) ? ((int64_t)__builtin_popcountll(TLw[wi]) |
-                                           ((int64_t)__builtin_popcountll(TUw[wi]) << 32))
-                                        : 0;
-            int64_t tot;
-            const int64_t off = block_exscan(c, sh_scan, &tot);  // low / high halves
-            nl = (int)(tot & 0xffffffff);
-            nu = (int)(tot >> 32);
-            lists = nl <= lcap && nu <= lcap;
-            if (lists && wi < nw) {
-                int ol = (int)(off & 0xffffffff), ou = (int)(off >> 32);
-                for (uint64_t b = TLw[wi]; b; b &= b - 1) PL[ol++] = wi * 64 + __builtin_ctzll(b);
-                for (uint64_t b = TUw[wi]; b; b &= b - 1) PU[ou++] = wi * 64 + __builtin_ctzll(b);
-            }
-            __syncthreads();
-        }
-
-        // 7. per-window bound
-        const double scale = ldexp(1.0, -F);
         ArgBest rep{__builtin_nan(""), -1};
-        auto bound = [&](int j, int64_t z) {
+        // the bound of window j from a lower bound zd of its z_l + z_u
+        auto bound = [&](int j, bool finite, double zd) {
             const int64_t g = gbase + (j - jfirst);
             double lb;
             if (INV[j] || !finite) {
                 lb = __builtin_nan("");
             } else {
-                const double zd = __ll2double_rd(z) * scale;
                 if (metric == DTW || metric == ERP) lb = sqrt(fmax(0.0, zd));
                 else if (metric == LCSS) lb = fmax(0.0, zd / (double)m);
                 else lb = zd;
@@ -1307,20 +1300,147 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
             }
             a.LB[g] = lb;
         };
-        if (lists) {
-            for (int j0 = jfirst + wid * 64; j0 < kml; j0 += NT) {
-                const int j = j0 + lane;
-                const int jc = j < kml ? j : kml - 1;
-                int64_t z = list_sum(PL, nl, GLw, CHl, j0, jc, m) +
-                            list_sum(PU, nu, GUw, CHu, j0, jc, m);
-                const int hi = jc + m - 1;
-                if ((GLw[hi >> 6] >> (hi & 63)) & 1) z += CHl[m];
-                if ((GUw[hi >> 6] >> (hi & 63)) & 1) z += CHu[m];
-                if (j < kml) bound(j, z);
+        if constexpr (F32) {
+            // 5'. penalties rounded down to FP32 in zero-padded rows:
+            //     HLf[15 + s] = h_l[s], HUf[15 + s] = h_u[s] (X is dead)
+            const int nst = f32_steps(m), hlen = f32_hlen(m);
+            float* HLf = (float*)X;
+            float* HUf = HLf + hlen;
+            double hs = 0.0;
+            for (int i = threadIdx.x; i < hlen; i += NT) {
+                const int sidx = i - 15;
+                float fl = 0.f, fu = 0.f;
+                if (sidx >= 0 && sidx < m) {
+                    double hl, hu;
+                    penalties(metric, Q.qn[sidx], t_l, t_u, C, &hl, &hu);
+                    hs += hl + hu;
+                    fl = __double2float_rd(hl);
+                    fu = __double2float_rd(hu);
+                }
+                HLf[i] = fl;
+                HUf[i] = fu;
+            }
+            hs = block_sum(hs, sh_sum);  // (its barriers publish the rows)
+            const bool finite = __builtin_isfinite(hs);
+            // 7'. tiles of 256 windows, one per wave at a time
+            const double nr = 4.0 * (double)nst;
+            const double gam = nr * 0x1p-23 / (1.0 - nr * 0x1p-23);
+            const double keep = 1.0 - gam - 0x1p-48;
+            const double sub = nr * 0x1p-149;
+            constexpr float kFltMax = 3.4028234663852886e38f;  // an overflowed sum
+            auto lo = [&](float v) { return (double)(v <= kFltMax ? v : kFltMax) * keep - sub; };
+            const int ntiles = (kml - jfirst + 255) / 256;
+            for (int tile = wid; tile < ntiles; tile += NT / 64) {
+                const int j0 = jfirst + 256 * tile;
+                f32x4 al, au;
+                f32_tile(GLw, GUw, HLf, HUf, j0, nst, lane, al, au);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int j = j0 + 16 * (4 * (lane >> 4) + i) + (lane & 15);
+                    if (j < kml) bound(j, finite, fmax(0.0, lo(al[i]) + lo(au[i])));
+                }
             }
         } else {
-            for (int j = jfirst + threadIdx.x; j < kml; j += blockDim.x)
-                bound(j, trans_sum(GLw, TLw, CHl, j, m) + trans_sum(GUw, TUw, CHu, j, m));
+            // 5. penalties in fixed point: hI = floor(h * 2^F), exact prefix sums
+            double hs = 0.0;
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                double hl, hu;
+                penalties(metric, Q.qn[i], t_l, t_u, C, &hl, &hu);
+                hs += hl + hu;
+            }
+            hs = block_sum(hs, sh_sum);
+            const bool finite = __builtin_isfinite(hs);
+            int F = 0;
+            if (finite && hs > 0.0) {
+                F = 60 - ilogb(hs);
+                if (F > 1000) F = 1000;
+            }
+            {
+                const int per = (m + NT - 1) / NT;
+                const int i0 = threadIdx.x * per;
+                const int i1 = (i0 + per < m) ? i0 + per : m;
+                int64_t tl_sum = 0, tu_sum = 0;
+                for (int i = i0; i < i1; i++) {
+                    double hl, hu;
+                    penalties(metric, Q.qn[i], t_l, t_u, C, &hl, &hu);
+                    if (finite) {
+                        tl_sum += (int64_t)floor(ldexp(hl, F));
+                        tu_sum += (int64_t)floor(ldexp(hu, F));
+                    }
+                }
+                const int64_t ol = block_exscan(tl_sum, sh_scan);
+                const int64_t ou = block_exscan(tu_sum, sh_scan);
+                int64_t cl = ol, cu = ou;
+                if (threadIdx.x == 0) {
+                    CHl[0] = 0;
+                    CHu[0] = 0;
+                }
+                for (int i = i0; i < i1; i++) {
+                    double hl, hu;
+                    penalties(metric, Q.qn[i], t_l, t_u, C, &hl, &hu);
+                    if (finite) {
+                        cl += (int64_t)floor(ldexp(hl, F));
+                        cu += (int64_t)floor(ldexp(hu, F));
+                    }
+                    CHl[i + 1] = cl;
+                    CHu[i + 1] = cu;
+                }
+            }
+
+            // 6. mask transitions: bitmaps, and for kDual / kSingle their
+            //    ascending lists (positions t with g[t] != g[t - 1]) after the
+            //    bitmaps in X's space, one word per thread (nw <= NT), when they
+            //    fit (lcap entries each)
+            for (int wi = threadIdx.x; wi < nw; wi += NT) {
+                const uint64_t pl = wi ? GLw[wi - 1] >> 63 : 0ull;
+                const uint64_t pu = wi ? GUw[wi - 1] >> 63 : 0ull;
+                TLw[wi] = GLw[wi] ^ ((GLw[wi] << 1) | pl);
+                TUw[wi] = GUw[wi] ^ ((GUw[wi] << 1) | pu);
+            }
+            __syncthreads();
+            int* PL = (int*)(TUw + nw);
+            const int lcap = (LAYOUT == kDual) ? mk : mk / 2;
+            int* PU = PL + lcap;
+            int nl = 0, nu = 0;
+            bool lists = false;
+            if constexpr (xl) {
+                const int wi = threadIdx.x;
+                const int64_t c = (wi < nw) ? ((int64_t)__builtin_popcountll(TLw[wi]) |
+                                               ((int64_t)__builtin_popcountll(TUw[wi]) << 32))
+                                            : 0;
+                int64_t tot;
+                const int64_t off = block_exscan(c, sh_scan, &tot);  // low / high halves
+                nl = (int)(tot & 0xffffffff);
+                nu = (int)(tot >> 32);
+                lists = nl <= lcap && nu <= lcap;
+                if (lists && wi < nw) {
+                    int ol = (int)(off & 0xffffffff), ou = (int)(off >> 32);
+                    for (uint64_t b = TLw[wi]; b; b &= b - 1) PL[ol++] = wi * 64 + __builtin_ctzll(b);
+                    for (uint64_t b = TUw[wi]; b; b &= b - 1) PU[ou++] = wi * 64 + __builtin_ctzll(b);
+                }
+                __syncthreads();
+            }
+
+            // 7. per-window bound (z rounded down to FP64; the scaling by 2^-F
+            //    is exact)
+            const double scale = ldexp(1.0, -F);
+            auto zfix = [&](int64_t z) { return __ll2double_rd(z) * scale; };
+            if (lists) {
+                for (int j0 = jfirst + wid * 64; j0 < kml; j0 += NT) {
+                    const int j = j0 + lane;
+                    const int jc = j < kml ? j : kml - 1;
+                    int64_t z = list_sum(PL, nl, GLw, CHl, j0, jc, m) +
+                                list_sum(PU, nu, GUw, CHu, j0, jc, m);
+                    const int hi = jc + m - 1;
+                    if ((GLw[hi >> 6] >> (hi & 63)) & 1) z += CHl[m];
+                    if ((GUw[hi >> 6] >> (hi & 63)) & 1) z += CHu[m];
+                    if (j < kml) bound(j, finite, zfix(z));
+                }
+            } else {
+                for (int j = jfirst + threadIdx.x; j < kml; j += blockDim.x)
+                    bound(j, finite,
+                          zfix(trans_sum(GLw, TLw, CHl, j, m) + trans_sum(GUw, TUw, CHu, j, m)));
+            }
         }
         if (a.rep_lb) {
             rep = block_best<false>(rep, sh_best);
@@ -1338,9 +1458,19 @@ hipError_t launch_lbip(hipStream_t st, const LbipArgs& a) {
     const int npad = npad_of(a.G.m_k - 2 * a.G.m + 2);
     const dim3 g((unsigned)a.blocks);
     const int nt = lbip_threads(npad, a.G.S);
+    const bool f32 = a.p.lbip_f32 != 0;
     switch (lbip_layout(a.G.m_k)) {
         case kDual:
-            if (nt == 128) hipLaunchKernelGGL((k_lbip<128, kDual>), g, dim3(128), lds, st, a);
+            if (f32) {
+                if (nt == 128)
+                    hipLaunchKernelGGL((k_lbip<128, kDual, true>), g, dim3(128), lds, st, a);
+                else if (nt == 256)
+                    hipLaunchKernelGGL((k_lbip<256, kDual, true>), g, dim3(256), lds, st, a);
+                else if (nt == 512)
+                    hipLaunchKernelGGL((k_lbip<512, kDual, true>), g, dim3(512), lds, st, a);
+                else
+                    hipLaunchKernelGGL((k_lbip<1024, kDual, true>), g, dim3(1024), lds, st, a);
+            } else if (nt == 128) hipLaunchKernelGGL((k_lbip<128, kDual>), g, dim3(128), lds, st, a);
             else if (nt == 256) hipLaunchKernelGGL((k_lbip<256, kDual>), g, dim3(256), lds, st, a);
             else if (nt == 512) hipLaunchKernelGGL((k_lbip<512, kDual>), g, dim3(512), lds, st, a);
             else hipLaunchKernelGGL((k_lbip<1024, kDual>), g, dim3(1024), lds, st, a);

@@ -71,6 +71,13 @@ typedef struct {
                               1: raw matching (NormalizationMode::Raw, paper App. D):
                               query and windows as they are, a window is invalid only
                               when it holds a non-finite sample */
+    int lbip_f32;          /* LB_IP's sliding inner products (src/sip.cpp:16-52):
+                              0: exact 64-bit fixed-point sums of the FP64 penalties
+                              rounded down;
+                              1: an FP32 contraction on the matrix cores with a
+                              certified rounding slack subtracted (a looser but still
+                              valid lower bound), for m_k <= 8192 (m <= 1024); longer
+                              segments keep the fixed-point sums (stats.lbip_f32) */
 } fsm_query_desc;
 
 typedef struct {
@@ -98,6 +105,8 @@ typedef struct {
      * and were recomputed on the host in FP64, whose value then decided;
      * how many of those host values differed from the device's */
     uint64_t near_rechecked, near_changed;
+    /* 1 when LB_IP's sums came from the FP32 matrix-core contraction */
+    int lbip_f32;
 } fsm_stats;
 
 /* ---- device and errors ---- */

@@ -54,3 +54,30 @@ def test_synth_is_seeded_and_shaped():
     q, off = F.synth_query(5, a, 64, 0.0)
     assert np.array_equal(q, a[off:off + 64])
     assert np.isfinite(a[off:off + 64]).all() and np.all(np.diff(a[off:off + 64]) != 0)
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """The binding's descriptor and statistics structs have the C header's
+    size and field offsets (gcc on include/fsmeld.h)."""
+    structs = {"fsm_query_desc": F._Desc, "fsm_stats": F._Stats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "fsmeld.h"',
+             "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)],
+                   check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], capture_output=True, text=True,
+                             check=True).stdout.splitlines():
+        cname, fname, val = ln.split()
+        got[(cname, fname)] = int(val)
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)

@@ -84,3 +84,7 @@ def test_cli_cutoff_search_exact(gpu, tmp_path, metric, arg):
     out_b = run(["search", tf, qf, m, cut, metric, w, arg], env_b)
     idx_b = re.findall(r"^A\[(\d+)\]=(\d+)$", out_b, re.M)
     assert np.array_equal([int(v) for _, v in idx_b], want)
+    # LB_IP on the FP32 matrix cores (FSMELD_LBIP=f32): the same matches
+    out_f = run(["search", tf, qf, m, cut, metric, w, arg], dict(env, FSMELD_LBIP="f32"))
+    idx_f = re.findall(r"^A\[(\d+)\]=(\d+)$", out_f, re.M)
+    assert np.array_equal([int(v) for _, v in idx_f], want)
