@@ -131,6 +131,10 @@ def main():
     ap.add_argument("--rates", default="50,200,1000")
     ap.add_argument("--requests", type=int, default=60, help="arrivals per rate (serve)")
     ap.add_argument("--workers", type=int, default=8, help="server workers (serve)")
+    ap.add_argument("--autotune", type=int, default=-1,
+                    help="sample searches for the autotuner before timing (0: built-in shapes; "
+                         "default: 32 for batch, 0 for plain, where it picks the built-in "
+                         "shapes within noise)")
     ap.add_argument("--prec-m", default="128,1024", help="query lengths (precision)")
     ap.add_argument("--prec-metrics", default="dtw,erp,edr,lcss,msm",
                     help="metrics (precision)")
@@ -182,6 +186,15 @@ def main():
                 json.dump({"key": key, "cutoffs": cutoffs}, f)
     else:
         log(f"cutoffs read from {args.cutoff_cache}")
+    tuning = None
+    if args.autotune > 0:  # (default off here)
+        # launch shapes from the autotuner on the step's own searches (before
+        # timing; shapes change speed only, the tuner checks the matches)
+        t0 = time.perf_counter()
+        tuning = T.autotune([dict(mode="search", query=q, m=m, metric=metric, w=w, arg=arg,
+                                  cutoff=cutoffs[metric]) for metric, arg in cfg["metrics"]],
+                            repeats=2)
+        log(f"autotune {time.perf_counter() - t0:.1f}s: {tuning}")
 
     def step():
         out = []
@@ -338,7 +351,9 @@ def main():
                                f"w={w}; dtw, erp g=0, edr 0.25, lcss 0.25, msm 0.5; cutoffs "
                                f"from k={cfg['k']}; one step = 5 searches, each recomputing "
                                f"its window statistics and envelope",
-                   "n": n, "m": m, "w": w, "k": cfg["k"], "parallelism": "single GPU"},
+                   "n": n, "m": m, "w": w, "k": cfg["k"], "parallelism": "single GPU",
+                   "launch_shapes": ("autotuned on the step's searches before timing: "
+                                     f"{tuning}" if tuning else "built-in")},
         "candidates_verified_per_s": round(valid * nq / elapsed, 1),
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
@@ -385,16 +400,38 @@ def run_batch(args):
                                 f"batching n={n} q={len(cfg['queries'])} k={cfg['k']}")
     # warm every window length's statistics path once (not timed)
     T.batch(specs[:16], concurrency=4)
-    t0 = time.perf_counter()
-    plain = []
-    for i, sp in enumerate(specs):
-        plain.append(T.query("search", sp["query"], sp["m"], sp["metric"], sp["w"], sp["arg"],
-                             cutoff_value=sp["cutoff"]))
-        if i % 32 == 31:
-            log(f"plain {i + 1}/{len(specs)} after {time.perf_counter() - t0:.1f}s")
-    plain_s = time.perf_counter() - t0
+
+    def run_plain(tag):
+        t0 = time.perf_counter()
+        out = []
+        for i, sp in enumerate(specs):
+            out.append(T.query("search", sp["query"], sp["m"], sp["metric"], sp["w"],
+                               sp["arg"], cutoff_value=sp["cutoff"]))
+            if i % 32 == 31:
+                log(f"plain ({tag}) {i + 1}/{len(specs)} after {time.perf_counter() - t0:.1f}s")
+        el = time.perf_counter() - t0
+        log(f"plain ({tag}): {len(specs)} queries in {el:.2f}s")
+        return out, el
+
+    plain0, plain0_s = run_plain("built-in shapes")
+    tuning, tune_s = None, 0.0
+    if args.autotune < 0:
+        args.autotune = 32
+    if args.autotune > 0:
+        # the autotuner on an evenly spread sample of the batch (not timed
+        # with the batch); plain and batched runs below both use its shapes
+        step = max(1, len(specs) // args.autotune)
+        t0 = time.perf_counter()
+        tuning = T.autotune(specs[::step][:args.autotune], repeats=2)
+        tune_s = time.perf_counter() - t0
+        log(f"autotune ({len(specs[::step][:args.autotune])} searches) {tune_s:.1f}s: {tuning}")
+        plain, plain_s = run_plain("tuned shapes")
+    else:
+        plain, plain_s = plain0, plain0_s
+    for a, b in zip(plain, plain0):
+        if not np.array_equal(a.indices, b.indices):
+            raise SystemExit("tuned shapes changed a search's matches")
     plain_lat = [r.stats["ms_total"] for r in plain]
-    log(f"plain: {len(specs)} queries in {plain_s:.2f}s")
     modes = []
     identical = True
     for c in [int(x) for x in args.concurrency.split(",")]:
@@ -419,6 +456,11 @@ def run_batch(args):
                                f"[64, 2048], w = 5% m, metric uniform over five, cutoffs k=10"},
         "plain": dict(queries_per_s=round(len(specs) / plain_s, 3), p50_ms=_pct(plain_lat, 50),
                       p99_ms=_pct(plain_lat, 99)),
+        "plain_builtin_shapes": dict(
+            queries_per_s=round(len(specs) / plain0_s, 3),
+            p50_ms=_pct([r.stats["ms_total"] for r in plain0], 50),
+            p99_ms=_pct([r.stats["ms_total"] for r in plain0], 99)),
+        "autotune": dict(samples=args.autotune, seconds=round(tune_s, 2), shapes=tuning),
         "batched": modes, "results_identical_to_plain": identical}), flush=True)
 
 

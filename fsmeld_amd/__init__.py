@@ -69,6 +69,11 @@ class _Stats(ctypes.Structure):
                 ("near_changed", ctypes.c_uint64), ("lbip_f32", ctypes.c_int)]
 
 
+class _Tuning(ctypes.Structure):
+    _fields_ = [("lbip_threads", ctypes.c_int * 4), ("eld_refill", ctypes.c_int * 7),
+                ("glb_blocks", ctypes.c_int)]
+
+
 _D = ctypes.POINTER(ctypes.c_double)
 _I64 = ctypes.POINTER(ctypes.c_int64)
 _U8 = ctypes.POINTER(ctypes.c_uint8)
@@ -120,6 +125,11 @@ _sig("fsm_query_bytes", ctypes.c_uint64, [ctypes.c_int64, ctypes.c_int, ctypes.c
 _sig("fsm_query_batch", ctypes.c_int, [_VP, ctypes.c_int64, ctypes.POINTER(_Desc),
                                        ctypes.POINTER(_D), _I64, ctypes.c_int,
                                        ctypes.POINTER(_VP), ctypes.POINTER(ctypes.c_int)])
+_sig("fsm_tuning_get", ctypes.c_int, [_VP, ctypes.POINTER(_Tuning)])
+_sig("fsm_tuning_set", ctypes.c_int, [_VP, ctypes.POINTER(_Tuning)])
+_sig("fsm_autotune", ctypes.c_int, [_VP, ctypes.c_int64, ctypes.POINTER(_Desc),
+                                    ctypes.POINTER(_D), _I64, ctypes.c_int,
+                                    ctypes.POINTER(_Tuning)])
 _sig("fsm_server_create", ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_VP)])
 _sig("fsm_server_submit", ctypes.c_int, [_VP, ctypes.POINTER(_Desc), _D, ctypes.c_int64,
                                          ctypes.POINTER(ctypes.c_uint64)])
@@ -280,6 +290,41 @@ class Target:
             else:
                 out.append(FsmeldError(status[i], "query failed"))
         return out
+
+    @staticmethod
+    def _tuning_dict(tu) -> dict:
+        return dict(lbip_threads=list(tu.lbip_threads), eld_refill=list(tu.eld_refill),
+                    glb_blocks=int(tu.glb_blocks))
+
+    def tuning(self) -> dict:
+        """The target's launch shapes (fsm_tuning_get); 0 = built-in rule."""
+        tu = _Tuning()
+        _check(_lib.fsm_tuning_get(self._h, ctypes.byref(tu)))
+        return self._tuning_dict(tu)
+
+    def set_tuning(self, tuning: dict):
+        """Sets the target's launch shapes (fsm_tuning_set)."""
+        tu = _Tuning()
+        for i, v in enumerate(tuning.get("lbip_threads", [])):
+            tu.lbip_threads[i] = int(v)
+        for i, v in enumerate(tuning.get("eld_refill", [])):
+            tu.eld_refill[i] = int(v)
+        tu.glb_blocks = int(tuning.get("glb_blocks", 0))
+        _check(_lib.fsm_tuning_set(self._h, ctypes.byref(tu)))
+
+    def autotune(self, specs, repeats=2) -> dict:
+        """Times the sample searches (dicts as for batch(), mode "search")
+        under each candidate launch shape, applies the fastest and returns
+        them (fsm_autotune)."""
+        n = len(specs)
+        qs = [_f64(sp["query"]) for sp in specs]
+        descs = (_Desc * max(n, 1))(*[_spec_desc(sp) for sp in specs])
+        qptr = (_D * max(n, 1))(*[_ptr(q) for q in qs])
+        lens = np.array([q.size for q in qs] or [0], dtype=np.int64)
+        tu = _Tuning()
+        _check(_lib.fsm_autotune(self._h, n, descs, qptr, _ptr(lens, _I64), int(repeats),
+                                 ctypes.byref(tu)))
+        return self._tuning_dict(tu)
 
     def trim(self):
         """Frees this handle's working buffers and the target's cached window

@@ -72,7 +72,9 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
-bool make_geometry(int64_t N, int m, Geometry* G) {
+}  // namespace
+
+bool fsm::make_geometry(int64_t N, int m, Geometry* G) {
     long long sel = 1;
     while (sel < 8LL * m) sel *= 2;  // src/fsmeld.cpp:28-37
     if (N < 2LL * m - 1) return false;
@@ -89,6 +91,8 @@ bool make_geometry(int64_t N, int m, Geometry* G) {
     G->S = G->forsize + (G->tail > 0 ? 1 : 0);
     return true;
 }
+
+namespace {
 
 // Centred clipped envelope on the host for the query (lower_upper_lemire,
 // src/lower_upper_lemire.cpp:8-44; the query holds only finite values).
@@ -247,6 +251,9 @@ struct CacheEntry {
     }
 };
 struct TargetCache {
+    // launch shapes the autotuner chose (fsm_tuning_set / fsm_autotune); every
+    // value 0 means the built-in rule.  Written while no query runs.
+    fsm_tuning tune{};
     std::mutex mu;
     std::map<int, std::shared_ptr<CacheEntry>> stats, env;  // by m, by w
     std::map<int, uint64_t> seen_stats, seen_env;            // last request, by key
@@ -507,6 +514,8 @@ int run_lbip(fsm_target* t, const Geometry& G, const QueryTables& Q, const Query
     a.slab = t->slab.p;
     a.slab_bytes_per_block = per;
     a.blocks = blocks;
+    const int cls = lbip_tune_class(G);
+    a.nt = cls >= 0 ? t->cache->tune.lbip_threads[cls] : 0;
     HIP_TRY(launch_lbip(t->stream, a));
     return FSM_OK;
 }
@@ -522,6 +531,9 @@ VerifyArgs verify_args(fsm_target* t, const Geometry& G, const QueryTables& Q,
     v.q = Q;
     v.p = P;
     v.G = G;
+    const fsm_tuning& tu = t->cache->tune;
+    v.glb_blocks = tu.glb_blocks;
+    v.eld_refill = (P.metric >= 1 && P.metric <= 6) ? tu.eld_refill[P.metric] : 0;
     return v;
 }
 
@@ -1509,6 +1521,25 @@ int fsm_query_batch(fsm_target* t, int64_t n, const fsm_query_desc* descs,
     for (int j = 1; j < k; j++) th.emplace_back(work, hs[j]);
     work(hs[0]);
     for (auto& x : th) x.join();
+    return FSM_OK;
+}
+
+int fsm_tuning_get(const fsm_target* t, fsm_tuning* out) {
+    if (!t || !out) return fail(FSM_ERR_ARG, "null argument");
+    *out = t->cache->tune;
+    return FSM_OK;
+}
+
+int fsm_tuning_set(fsm_target* t, const fsm_tuning* in) {
+    if (!t || !in) return fail(FSM_ERR_ARG, "null argument");
+    for (int v : in->lbip_threads)
+        if (v != 0 && v != 128 && v != 256 && v != 512 && v != 1024)
+            return fail(FSM_ERR_ARG, "LB_IP workgroup sizes are 128, 256, 512 or 1024");
+    for (int v : in->eld_refill)
+        if (v < 0 || v > 64) return fail(FSM_ERR_ARG, "ELD refill thresholds lie in [0, 64]");
+    if (in->glb_blocks < 0 || in->glb_blocks > (1 << 20))
+        return fail(FSM_ERR_ARG, "GLB workgroups lie in [0, 2^20]");
+    t->cache->tune = *in;
     return FSM_OK;
 }
 

@@ -33,6 +33,10 @@ struct Geometry {
     int64_t S;  // forsize + (tail > 0)
 };
 
+// The macro-segmentation of a target of N samples for length m; false when
+// the target is too short (N < 2m - 1) or m_k exceeds 2^30.
+bool make_geometry(int64_t N, int m, Geometry* G);
+
 // Device-side query tables, uploaded once per query.
 struct QueryTables {
     const double* qn;   // z-normalised query [m]
@@ -143,9 +147,16 @@ struct LbipArgs {
     void* slab;        // per-block global scratch
     size_t slab_bytes_per_block;
     int blocks;
+    int nt;            // workgroup size for the kDual layout (0: the built-in rule)
 };
 size_t lbip_slab_bytes(const Geometry& G);
-size_t lbip_lds_bytes(const Geometry& G);
+// nt: a kDual workgroup size (lbip_threads_ok), 0 for the built-in rule
+size_t lbip_lds_bytes(const Geometry& G, int nt = 0);
+// the autotuner's class of G (0: m_k <= 1024, 1: 2048, 2: 4096, 3: 8192) or
+// -1 when G's layout has a fixed workgroup size
+int lbip_tune_class(const Geometry& G);
+// true when LB_IP may run G's segments in workgroups of nt threads
+bool lbip_threads_ok(const Geometry& G, int nt);
 // true when the FP32 contraction serves queries of this geometry
 bool lbip_f32_supported(const Geometry& G);
 hipError_t launch_lbip(hipStream_t st, const LbipArgs& a);
@@ -186,6 +197,10 @@ struct VerifyArgs {
     QueryTables q;
     QueryParams p;
     Geometry G;
+    // launch shapes chosen by the autotuner (0: the built-in values): GLB's
+    // persistent workgroups; the register-band ELD's idle lanes per refill
+    int glb_blocks;
+    int eld_refill;
 };
 // GLB filter over in[0 .. *in_count): survivors appended to out (out_count),
 // with, when out_total != null, each survivor's certified consumption bound

@@ -664,10 +664,11 @@ k_eld(VerifyArgs a, const double* __restrict__ LQ,
     bool busy = false, drained = false;
     // the wave's pool of queue positions [pb, pe) (the same in every lane)
     unsigned long long pb = 0, pe = 0;
+    const int refill = a.eld_refill > 0 ? a.eld_refill : kRefillMin<METRIC>;
     while (true) {
         const uint64_t idle = __ballot(!busy);
         const int nidle = __builtin_popcountll(idle);
-        if ((pb < pe || !drained) && (nidle >= kRefillMin<METRIC> || nidle == 64)) {
+        if ((pb < pe || !drained) && (nidle >= refill || nidle == 64)) {
             const unsigned long long left = pe - pb;
             const bool take = left < (unsigned long long)nidle && !drained;
             unsigned long long nb = 0;

@@ -677,7 +677,7 @@ hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
                       unsigned long long* out_count, double* out_total, double* vals,
                       unsigned long long* queue) {
     if (!queue) return hipErrorInvalidValue;
-    const dim3 g(verify_blocks()), b(256);
+    const dim3 g(a.glb_blocks > 0 ? a.glb_blocks : verify_blocks()), b(256);
     hipLaunchKernelGGL(k_glb_queue_reset, dim3(1), dim3(1), 0, st, queue);
     const QueryTables& q = a.q;
     // staging: 3 arrays of up to kGlbStage doubles (39 KiB, four blocks per CU)

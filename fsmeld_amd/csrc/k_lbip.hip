@@ -93,11 +93,11 @@ __host__ __device__ inline size_t zone_bytes(int npad, int nt, int kml) {
 // upper mask bitmap (at most 7, see f32_bounds)
 constexpr int kMaskPadWords = 8;
 
-size_t lbip_lds_bytes(const Geometry& G) {
+size_t lbip_lds_bytes(const Geometry& G, int nt_want) {
     const int nthr = G.m_k - 2 * G.m + 2;
     const int npad = npad_of(nthr);
     const int nw = (G.m_k + 63) / 64;
-    const int nt = lbip_threads(npad, G.S);
+    const int nt = nt_want ? nt_want : lbip_threads(npad, G.S);
     const int lay = lbip_layout(G.m_k);
     // (+ kMaskPadWords: the FP32 contraction reads a few words past the
     // upper mask's end, whose bits only ever meet zero penalties)
@@ -114,6 +114,20 @@ size_t lbip_lds_bytes(const Geometry& G) {
 // Per-block global scratch: the threshold's full-sort buffer (npad keys);
 // with kSlab also the van Herk arrays and both envelopes.
 bool lbip_f32_supported(const Geometry& G) { return lbip_layout(G.m_k) == kDual; }
+
+int lbip_tune_class(const Geometry& G) {
+    if (lbip_layout(G.m_k) != kDual) return -1;
+    return G.m_k <= 1024 ? 0 : (G.m_k <= 2048 ? 1 : (G.m_k <= 4096 ? 2 : 3));
+}
+
+// kDual runs in any of the four workgroup sizes whose LDS fits a CU; the
+// mask transition lists take one bitmap word per thread (nw <= nt holds for
+// m_k <= 8192 and nt >= 128)
+bool lbip_threads_ok(const Geometry& G, int nt) {
+    if (lbip_layout(G.m_k) != kDual) return false;
+    if (nt != 128 && nt != 256 && nt != 512 && nt != 1024) return false;
+    return (G.m_k + 63) / 64 <= nt && lbip_lds_bytes(G, nt) <= 160 * 1024;
+}
 
 size_t lbip_slab_bytes(const Geometry& G) {
     const int nthr = G.m_k - 2 * G.m + 2;
@@ -1454,10 +1468,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 }
 
 hipError_t launch_lbip(hipStream_t st, const LbipArgs& a) {
-    const size_t lds = lbip_lds_bytes(a.G);
     const int npad = npad_of(a.G.m_k - 2 * a.G.m + 2);
     const dim3 g((unsigned)a.blocks);
-    const int nt = lbip_threads(npad, a.G.S);
+    const int nt = (a.nt && lbip_threads_ok(a.G, a.nt)) ? a.nt : lbip_threads(npad, a.G.S);
+    const size_t lds = lbip_lds_bytes(a.G, lbip_layout(a.G.m_k) == kDual ? nt : 0);
     const bool f32 = a.p.lbip_f32 != 0;
     switch (lbip_layout(a.G.m_k)) {
         case kDual:

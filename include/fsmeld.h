@@ -161,6 +161,37 @@ int fsm_query_batch(fsm_target* t, int64_t n, const fsm_query_desc* descs,
                     const double* const* queries, const int64_t* query_lens, int concurrency,
                     fsm_result** results, int* status);
 
+/* ---- launch shapes and the autotuner (no reference counterpart) ----
+ * Every field 0 keeps the engine's built-in rule; values outside a field's
+ * valid set are ignored where they would not fit (an LB_IP workgroup whose
+ * LDS exceeds a CU's).  Shapes change speed only, never results. */
+typedef struct {
+    int lbip_threads[4]; /* LB_IP workgroup size (128, 256, 512, 1024) for segments of
+                            m_k <= 1024, 2048, 4096, 8192 (longer: fixed 1024) */
+    int eld_refill[7];   /* register-band ELD: idle lanes of a wave that trigger its
+                            refill from the candidate queue, by FSM_METRIC_* (index 0
+                            unused) */
+    int glb_blocks;      /* GLB: persistent workgroups of 256 threads */
+} fsm_tuning;
+
+/* The target's launch shapes (shared by all its handles); set them while no
+ * query runs on any handle of the target. */
+int fsm_tuning_get(const fsm_target* t, fsm_tuning* out);
+int fsm_tuning_set(fsm_target* t, const fsm_tuning* in);
+
+/* Project-written autotuner: times the sample searches (descs[i].mode must be
+ * FSM_MODE_SEARCH) under each candidate launch shape -- per LB_IP segment
+ * class the workgroup sizes that fit, per metric the ELD refill thresholds
+ * {4, 8, 12, 16, 24}, the GLB grid {1024, 2048, 4096, 8192} -- by the HIP-
+ * event stage times of the stage each shape drives (the smallest of
+ * `repeats` runs per search), keeps the fastest, checks that every candidate
+ * returns the same matches, applies the result to the target and returns it
+ * in *out.  Classes and metrics without a sample search keep the built-in
+ * rule. */
+int fsm_autotune(fsm_target* t, int64_t n, const fsm_query_desc* descs,
+                 const double* const* queries, const int64_t* query_lens, int repeats,
+                 fsm_tuning* out);
+
 /* Device bytes a handle holds while running a query of (metric, m, w) over n
  * samples (working buffers only; the samples themselves are shared). */
 uint64_t fsm_query_bytes(int64_t n, int metric, int m, int w);

@@ -59,7 +59,7 @@ def test_synth_is_seeded_and_shaped():
 def test_ctypes_structs_match_the_header(tmp_path):
     """The binding's descriptor and statistics structs have the C header's
     size and field offsets (gcc on include/fsmeld.h)."""
-    structs = {"fsm_query_desc": F._Desc, "fsm_stats": F._Stats}
+    structs = {"fsm_query_desc": F._Desc, "fsm_stats": F._Stats, "fsm_tuning": F._Tuning}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "fsmeld.h"',
              "int main(void) {"]
     for cname, cls in structs.items():
