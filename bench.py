@@ -135,6 +135,10 @@ def main():
                     help="sample searches for the autotuner before timing (0: built-in shapes; "
                          "default: 32 for batch, 0 for plain, where it picks the built-in "
                          "shapes within noise)")
+    ap.add_argument("--phase-gaps", action="store_true",
+                    help="batch: leave the device idle 0.25 s between phases (warm-up, plain, "
+                         "autotune, tuned plain, each concurrency), so that a kernel trace "
+                         "splits into them")
     ap.add_argument("--prec-m", default="128,1024", help="query lengths (precision)")
     ap.add_argument("--prec-metrics", default="dtw,erp,edr,lcss,msm",
                     help="metrics (precision)")
@@ -398,10 +402,15 @@ def run_batch(args):
     n = len(cfg["target"])
     specs = _specs_with_cutoffs(T, cfg["queries"], cfg["k"], args.cutoff_cache,
                                 f"batching n={n} q={len(cfg['queries'])} k={cfg['k']}")
+    def gap():
+        if args.phase_gaps:
+            time.sleep(0.25)
+
     # warm every window length's statistics path once (not timed)
     T.batch(specs[:16], concurrency=4)
 
     def run_plain(tag):
+        gap()
         t0 = time.perf_counter()
         out = []
         for i, sp in enumerate(specs):
@@ -421,6 +430,7 @@ def run_batch(args):
         # the autotuner on an evenly spread sample of the batch (not timed
         # with the batch); plain and batched runs below both use its shapes
         step = max(1, len(specs) // args.autotune)
+        gap()
         t0 = time.perf_counter()
         tuning = T.autotune(specs[::step][:args.autotune], repeats=2)
         tune_s = time.perf_counter() - t0
@@ -435,6 +445,7 @@ def run_batch(args):
     modes = []
     identical = True
     for c in [int(x) for x in args.concurrency.split(",")]:
+        gap()
         t0 = time.perf_counter()
         got = T.batch(specs, concurrency=c)
         el = time.perf_counter() - t0
