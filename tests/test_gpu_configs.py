@@ -117,6 +117,41 @@ def test_plain_large_prefix(gpu, plain_large_prefix, metric, arg):
     assert abs(int(s.stats["after_glb"]) - int(ost["after_glb"])) <= 0.02 * v
 
 
+@pytest.fixture(scope="module")
+def plain_large_full():
+    c = workloads.plain_large()
+    return c["query"], F.Target(c["target"])
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("metric,arg", workloads.CONFIG2_METRICS)
+def test_plain_large_full_size(gpu, plain_large_full, metric, arg):
+    """The benchmark's own workload at full size (2^26, m = 512, w = 25, k =
+    100), through properties that hold at any size: the cutoff threshold is
+    the k-th smallest exact distance of all windows (the brute-force top-k),
+    its ranking is the top-k's first k in (distance, index) order, and search
+    at that threshold returns the exact scan's windows (up to windows exactly
+    at the threshold, the reference's own ulp case) with the same distance
+    bits."""
+    q, T = plain_large_full
+    m, w, k = 512, 25, 100
+    g = T.cutoff(q, m, k, metric, w, arg)
+    top = T.exact_topk(q, m, k, metric, w, arg)
+    assert bits(g.threshold) == bits(top.threshold)
+    order = np.lexsort((top.indices, top.distances))[:k]
+    assert np.array_equal(g.indices, top.indices[order])
+    assert np.array_equal(bits(g.distances), bits(top.distances[order]))
+    s = T.search(q, m, g.threshold, metric, w, arg)
+    e = T.exact(q, m, g.threshold, metric, w, arg)
+    assert np.setdiff1d(s.indices, e.indices).size == 0
+    common = np.isin(e.indices, s.indices)
+    assert np.array_equal(bits(e.distances[common]), bits(s.distances))
+    assert np.all(e.distances[~common] == g.threshold)
+    assert s.stats["valid"] == e.stats["valid"] == top.stats["valid"]
+    print(f"\n{metric}: threshold {g.threshold:.17g}, {s.indices.size} matches, "
+          f"{int((~common).sum())} exact-only at the threshold")
+
+
 # --------------------------------------------------------------- batching ---
 @pytest.fixture(scope="module")
 def batch_case():
