@@ -329,7 +329,24 @@ def main():
             log(f"cpu baseline {metric}: segments [{s0},{s0 + seg}) in {dt:.2f}s; same windows "
                 f"as the GPU there: {ok} ({ids.size} matches)")
         est = float(np.mean(per_metric_s))
+        # one thread (the reference is single-threaded) on a smaller sample
+        seg1 = max(1, min(seg, seg // 20))
+        c1 = time.perf_counter()
+        for metric, arg in cfg["metrics"]:
+            O.search(t, q, m, cutoffs[metric], metric, w, arg, segments=(s0, s0 + seg1),
+                     threads=1)
+        est1 = (time.perf_counter() - c1) / len(cfg["metrics"]) * G["S"] / seg1
+        log(f"cpu baseline, one thread: {seg1} segments per metric, {est1:.1f} s per query")
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f
+                              if ln.startswith("model name")), "")
+        except OSError:
+            pass
         cpu = dict(value=round(1.0 / est, 6), unit="queries/s", cores=threads, kind="port",
+                   cpu_model=model, one_thread_value=round(1.0 / est1, 7),
+                   one_thread_sample=f"macro-segments [{s0},{s0 + seg1}) per metric, scaled",
                    sample=f"oracle search on macro-segments [{s0},{s0 + seg}) of {G['S']} (around "
                           f"the query's source offset) per metric (5 metrics, k=100 cutoffs), "
                           f"OpenMP over segments, scaled to the full target; "
@@ -486,12 +503,18 @@ def run_serve(args):
                                 f"serving n={n} q={len(cfg['queries'])} k={cfg['k']}")
     budget = 96 << 30
     rng = np.random.default_rng(7)
+    biggest = max(specs, key=lambda sp: sp["m"])
 
     def serve(rate, workers):
         gaps = rng.exponential(1.0 / rate, args.requests)
         order = rng.integers(0, len(specs), args.requests)
         lat, done_t = [], []
         with F.Server(T, mem_budget=budget, max_workers=workers) as S:
+            # warm-up (not recorded): one of the largest queries per worker,
+            # so every worker's buffers exist before the arrivals start, as in
+            # a server that has been up for a while
+            for tk in [S.submit(biggest) for _ in range(workers)]:
+                S.poll(tk, wait_ms=-1)
             tickets = []
             lock = threading.Lock()
 
@@ -527,7 +550,6 @@ def run_serve(args):
         return dict(offered_rate=rate, completed=len(lat), queries_per_s=round(len(lat) / span, 3),
                     p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99))
 
-    serve(50, args.workers)  # warm every handle (not reported)
     rows = []
     for rate in [float(x) for x in args.rates.split(",")]:
         a = serve(rate, args.workers)

@@ -560,11 +560,13 @@ QueryParams params_of(const fsm_query_desc* d) {
 
 int ensure_eld_scratch(fsm_target* t, int metric, int m, int w);
 
-// Tighter cutoff bound (see run_filter_verify): samples the GLB survivors'
-// certified bounds (lb[0 .. C[2])), selects about max(8k, 4096) survivors of
-// smallest bound (largest for LCSS, whose bound counts possible matches),
-// computes their exact distances and lowers P->cutoff to their k-th smallest
-// when that is smaller.  Uses list_a, out_dist and counters 4, 6, 7.
+// Tighter cutoff bound (see run_filter_verify): selects the max(8k, 4096)
+// GLB survivors of smallest certified bound (lb[0 .. C[2]); largest for LCSS,
+// whose bound counts possible matches), ties included, by an exact radix
+// select of that order statistic -- so the set, and the bound that follows,
+// do not depend on the order in which GLB appended the survivors -- computes
+// their exact distances and lowers P->cutoff to their k-th smallest when that
+// is smaller.  Uses list_a, out_dist, hist and counters 6, 7.
 int tighten_cutoff(fsm_target* t, const Geometry& G, const QueryTables& Q, QueryParams* P,
                    int k, uint64_t* evals) {
     unsigned long long* C = t->counters.as<unsigned long long>();
@@ -573,20 +575,18 @@ int tighten_cutoff(fsm_target* t, const Geometry& G, const QueryTables& Q, Query
     HIP_TRY(hipStreamSynchronize(t->stream));
     const int64_t want = std::max<int64_t>(8 * (int64_t)k, 4096);
     if ((int64_t)n_l <= 4 * want) return FSM_OK;
-    const int64_t step = std::max<int64_t>(1, (int64_t)n_l / 65536);
-    const int64_t ns = ((int64_t)n_l + step - 1) / step;
-    HIP_TRY(t->out_dist.ensure(sizeof(double) * (size_t)std::max<int64_t>(ns, want * 4)));
-    HIP_TRY(launch_gather_stride(t->stream, t->lb.as<double>(), (int64_t)n_l, step,
-                                 t->out_dist.as<double>()));
-    std::vector<double> smp((size_t)ns);
-    HIP_TRY(hipMemcpyAsync(smp.data(), t->out_dist.p, sizeof(double) * ns, hipMemcpyDeviceToHost,
-                           t->stream));
-    HIP_TRY(hipStreamSynchronize(t->stream));
+    // tau: the want-th smallest bound (LCSS: the want-th largest, the
+    // (n - want + 1)-th smallest of the n finite ones)
     const double sign = (P->metric == LCSS) ? -1.0 : 1.0;
-    for (double& x : smp) x *= sign;
-    const int64_t pick = std::min<int64_t>(ns - 1, std::max<int64_t>(1, want * ns / (int64_t)n_l));
-    std::nth_element(smp.begin(), smp.begin() + pick, smp.end());
-    const double tau = smp[(size_t)pick];
+    HIP_TRY(t->hist.ensure(sizeof(unsigned long long) * 2048));
+    double tau = 0.0;
+    int64_t nfin = 0;
+    HIP_TRY(kth_smallest(t->stream, t->lb.as<double>(), C + 2, want,
+                         t->hist.as<unsigned long long>(), &tau, &nfin));
+    if (nfin < 4 * want) return FSM_OK;
+    if (sign < 0)
+        HIP_TRY(kth_smallest(t->stream, t->lb.as<double>(), C + 2, nfin - want + 1,
+                             t->hist.as<unsigned long long>(), &tau, &nfin));
     unsigned long long zero[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(C + 6, zero, sizeof(zero), hipMemcpyHostToDevice, t->stream));
     HIP_TRY(launch_select_le(t->stream, t->list_b.as<int64_t>(), t->lb.as<double>(), C + 2, sign,

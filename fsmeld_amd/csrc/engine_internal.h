@@ -168,9 +168,6 @@ hipError_t launch_compact(hipStream_t st, const Geometry& G, const QueryParams& 
                           const uint8_t* INV, const double* LB, bool use_lb, int64_t* out,
                           unsigned long long* counters);
 
-// Every step-th key of keys[0 .. n) into out.
-hipError_t launch_gather_stride(hipStream_t st, const double* keys, int64_t n, int64_t step,
-                                double* out);
 // list[i] for i < *count with sign * keys[i] <= tau, appended to out.
 hipError_t launch_select_le(hipStream_t st, const int64_t* list, const double* keys,
                             const unsigned long long* count, double sign, double tau,

@@ -95,13 +95,6 @@ hipError_t launch_compact(hipStream_t st, const Geometry& G, const QueryParams& 
 }
 
 // ------------------------------------------------- cutoff: tighter bound ---
-// Every step-th key of keys[0 .. n) into out (count = ceil(n / step)).
-__global__ void k_gather_stride(const double* __restrict__ keys, int64_t n, int64_t step,
-                                double* __restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i * step < n) out[i] = keys[i * step];
-}
-
 // Entries of list[0 .. *count) whose sign * keys[i] <= tau, appended to out.
 __global__ void __launch_bounds__(256) k_select_le(const int64_t* __restrict__ list,
                                                    const double* __restrict__ keys,
@@ -219,15 +212,6 @@ hipError_t launch_select_le_kv(hipStream_t st, const int64_t* list, const double
                                double* out_keys, unsigned long long* out_count) {
     hipLaunchKernelGGL(k_select_le_kv, dim3(1024), dim3(256), 0, st, list, keys, count, tau, out,
                        out_keys, out_count);
-    return hipGetLastError();
-}
-
-hipError_t launch_gather_stride(hipStream_t st, const double* keys, int64_t n, int64_t step,
-                                double* out) {
-    const int64_t cnt = (n + step - 1) / step;
-    if (cnt <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_stride, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st,
-                       keys, n, step, out);
     return hipGetLastError();
 }
 

@@ -243,3 +243,75 @@ def test_wide_radius_commands(gpu, m, w, n, metrics):
         win = np.flatnonzero(~stats[2])[::97]
         d = oracle_dist(t, q, m, metric, w, arg, win, stats)
         assert np.all(~(lb[win] > d + 1e-9 * np.maximum(1.0, d)))
+
+
+# ------------------------------------------------- batching / serving, full size ---
+def _same(a, b):
+    assert np.array_equal(a.indices, b.indices)
+    assert np.array_equal(bits(a.distances), bits(b.distances))
+    for key in ("valid", "after_lbip", "after_glb", "matches", "threshold"):
+        x, y = a.stats[key], b.stats[key]
+        assert x == y or (x != x and y != y), key
+
+
+@pytest.mark.timeout(600)
+def test_batching_full_size(gpu):
+    """The batching workload at its full size (2^24 target; 48 of its 256
+    queries, m log-uniform in [64, 2048]): k = 10 cutoffs and searches through
+    fsm_query_batch equal the same queries run one at a time, with the
+    autotuner's shapes as with the built-in ones, and every search equals the
+    exact scan at its cutoff (up to windows exactly at the cutoff)."""
+    cfg = workloads.batching()
+    t = cfg["target"]
+    qs = cfg["queries"][::5][:48]
+    T = F.Target(t)
+    cspecs = [dict(mode="cutoff", query=x["query"], m=x["m"], metric=x["metric"], w=x["w"],
+                   arg=x["arg"], k=cfg["k"]) for x in qs]
+    cuts = T.batch(cspecs, concurrency=8)
+    specs = []
+    for sp, c in zip(cspecs, cuts):
+        assert isinstance(c, F.QueryResult), c
+        alone = T.query("cutoff", sp["query"], sp["m"], sp["metric"], sp["w"], sp["arg"],
+                        k=sp["k"])
+        _same(c, alone)
+        specs.append(dict(sp, mode="search", cutoff=c.threshold))
+    got = T.batch(specs, concurrency=8)
+    T.autotune(specs[:12], repeats=1)
+    tuned = T.batch(specs, concurrency=4)
+    for sp, g, u in zip(specs, got, tuned):
+        alone = T.search(sp["query"], sp["m"], sp["cutoff"], sp["metric"], sp["w"], sp["arg"])
+        _same(g, alone)
+        _same(u, alone)
+        e = T.exact(sp["query"], sp["m"], sp["cutoff"], sp["metric"], sp["w"], sp["arg"])
+        assert np.setdiff1d(g.indices, e.indices).size == 0
+        assert np.all(e.distances[~np.isin(e.indices, g.indices)] == sp["cutoff"])
+    T.close()
+
+
+@pytest.mark.timeout(600)
+def test_serving_full_size(gpu):
+    """The serving workload at its full size (the 2^26 target; 24 of its
+    queries, m log-uniform in [64, 1024], cutoff / search / exact mixed,
+    submitted at once to 8 workers): each served result equals the same query
+    run alone."""
+    cfg = workloads.serving()
+    t = cfg["target"]
+    T = F.Target(t)
+    specs = []
+    for i, x in enumerate(cfg["queries"][:24]):
+        mode = ["cutoff", "search", "exact"][i % 3]
+        sp = dict(mode=mode, query=x["query"], m=x["m"], metric=x["metric"], w=x["w"],
+                  arg=x["arg"], k=cfg["k"])
+        if mode != "cutoff":
+            sp["cutoff"] = T.cutoff(x["query"], x["m"], cfg["k"], x["metric"], x["w"],
+                                    x["arg"]).threshold
+        specs.append(sp)
+    alone = [T.query(sp["mode"], sp["query"], sp["m"], sp["metric"], sp["w"], sp["arg"],
+                     cutoff_value=sp.get("cutoff", 0.0), k=sp["k"]) for sp in specs]
+    with F.Server(T, mem_budget=96 << 30, max_workers=8) as S:
+        tickets = [S.submit(sp) for sp in specs]
+        for tk, a in zip(tickets, alone):
+            state, r, _ = S.poll(tk, wait_ms=120000)
+            assert state == "done" and isinstance(r, F.QueryResult), (state, r)
+            _same(r, a)
+    T.close()
