@@ -51,6 +51,11 @@ int fail(int code, const std::string& msg) {
                         std::string(#expr) + ": " + hipGetErrorString(e_));           \
     } while (0)
 
+// Device buffer that grows on demand.  Sizes are rounded up (64 KiB, or
+// 64 MiB from 64 MiB on) so that requests that differ a little -- the window
+// statistics of two lengths, S k_m_l entries each -- reuse the allocation:
+// hipFree waits for the device, and a free / malloc per query stalled
+// concurrent handles.
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
@@ -59,8 +64,10 @@ struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e == hipSuccess) cap = bytes;
+        const size_t g = bytes >= (64u << 20) ? (64u << 20) : (64u << 10);
+        const size_t want = (bytes + g - 1) / g * g;
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) cap = want;
         return e;
     }
     void release() {
@@ -410,11 +417,10 @@ int cached(fsm_target* t, std::map<int, std::shared_ptr<CacheEntry>> TargetCache
     }
     if (!*made) HIP_TRY(hipStreamWaitEvent(t->stream, e->ready, 0));
     *out = std::move(e);
-    // this handle's own buffers are not needed while it uses a shared entry
-    // (fsm_query_bytes counts one set per handle)
-    own->a.release();
-    own->b.release();
-    own->c.release();
+    // (the handle keeps its own buffers, which fsm_query_bytes counts, for
+    // its next private length: freeing them here and allocating them again
+    // made every switch between shared and private entries two device-wide
+    // waits)
     return FSM_OK;
 }
 
@@ -1549,15 +1555,21 @@ uint64_t fsm_query_bytes(int64_t n, int metric, int m, int w) {
     const uint64_t rows = (uint64_t)G.S * (uint64_t)G.kml;
     const size_t per = lbip_slab_bytes(G);
     const uint64_t blocks = (uint64_t)std::min<int64_t>(G.S, per > (1u << 20) ? 1024 : 2048);
-    uint64_t b = 17 * rows                          // window mean, std, invalid flag
-                 + 16 * (uint64_t)n                 // target envelope
-                 + 48 * (uint64_t)G.ts_n            // LB, two candidate lists, matches,
+    // each buffer as DevBuf allocates it (rounded up)
+    auto r = [](uint64_t x) {
+        const uint64_t g = x >= (64u << 20) ? (64u << 20) : (64u << 10);
+        return (x + g - 1) / g * g;
+    };
+    uint64_t b = 2 * r(8 * rows) + r(rows)          // window mean, std, invalid flag
+                 + 2 * r(8 * (uint64_t)n)           // target envelope
+                 + 6 * r(8 * (uint64_t)G.ts_n)      // LB, two candidate lists, matches,
                                                     // cutoff's second GLB bounds
-                 + 32 * (uint64_t)G.S               // segment thresholds, representatives
-                 + per * blocks                     // LB_IP per-block scratch
-                 + 80 * (uint64_t)(m + 1) + 128;    // query tables, counters
+                 + 4 * r(8 * (uint64_t)G.S)         // segment thresholds, representatives
+                 + r(per * blocks)                  // LB_IP per-block scratch
+                 + r(96 * (uint64_t)(m + 1)) + r(128) + r(16384);  // query tables, counters,
+                                                                   // radix-select counts
     if (eld_uses_scratch(metric, m, w))
-        b += sizeof(double) * 2 * (uint64_t)(2 * w + 2) * (uint64_t)eld_scratch_threads();
+        b += r(sizeof(double) * 2 * (uint64_t)(2 * w + 2) * (uint64_t)eld_scratch_threads());
     return b;
 }
 
