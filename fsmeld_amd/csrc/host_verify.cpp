@@ -50,19 +50,35 @@ struct Rows {
     }
 };
 
+// Inner loops.  Each row's cells in column order form one dependency chain
+// through the left neighbour; everything else a cell needs comes from the
+// previous row.  So the loops below take the first column (and the first
+// row) apart and run the interior branch-free, with the cell written as
+// min(min(up-side terms), left + cost) (LCSS: max): min and max are exact
+// and the values are never NaN or -0, so any grouping of the reference's
+// min3 / max3 gives its bits, and only the last add and min stay on the
+// chain.
+
 double dtw(const double* x, const double* y, int m, int w) {
     Rows r(m, w, kOut);
-    for (int i = 0; i < m; i++) {
-        if (i) r.next(kOut, i);
-        for (int j = std::max(0, i - w); j <= std::min(m - 1, i + w); j++) {
-            if (i == 0 && j == 0) {
-                r.cur[0] = sq(x[0], y[0]);
-                continue;
-            }
-            const double up = i ? r.prev[j] : kOut;
-            const double left = j ? r.cur[j - 1] : kOut;
-            const double diag = (i && j) ? r.prev[j - 1] : kOut;
-            r.cur[j] = min3(up, left, diag) + sq(x[i], y[j]);
+    r.cur[0] = sq(x[0], y[0]);
+    for (int j = 1; j <= std::min(m - 1, w); j++)  // row 0: up and diag are out of the matrix
+        r.cur[j] = min3(kOut, r.cur[j - 1], kOut) + sq(x[0], y[j]);
+    for (int i = 1; i < m; i++) {
+        r.next(kOut, i);
+        const double* P = r.prev;
+        double* C = r.cur;
+        const double xi = x[i];
+        int j = std::max(0, i - w);
+        const int hi = std::min(m - 1, i + w);
+        if (j == 0) {  // column 0: left and diag are out of the matrix
+            C[0] = min3(P[0], kOut, kOut) + sq(xi, y[0]);
+            j = 1;
+        }
+        double left = C[j - 1];
+        for (; j <= hi; j++) {
+            left = std::min(std::min(P[j], P[j - 1]), left) + sq(xi, y[j]);
+            C[j] = left;
         }
     }
     const double f = r.cur[m - 1];
@@ -72,21 +88,31 @@ double dtw(const double* x, const double* y, int m, int w) {
 
 double lcss(const double* x, const double* y, int m, int w, double eps) {
     Rows r(m, w, 0.0);
-    for (int i = 0; i < m; i++) {
-        if (i) r.next(0.0, i);
-        const int lo = std::max(0, i - w), hi = std::min(m, i + w + 1);
-        for (int j = lo; j < hi; j++) {
-            const bool hit = std::abs(x[i] - y[j]) <= eps;
-            double v;
-            if (i + j == 0) v = hit ? 1 : 0;
-            else if (i == 0) v = hit ? 1 : ((j > lo) ? r.cur[j - 1] : 0);
-            else if (j == 0) v = hit ? 1 : r.prev[j];
-            else {
-                const double v3 = (j > lo) ? r.cur[j - 1] : 0;
-                v = hit ? max3(r.prev[j - 1] + 1, r.prev[j], v3)
-                        : max3(r.prev[j - 1], r.prev[j], v3);
-            }
-            r.cur[j] = v;
+    {  // row 0
+        const int hi = std::min(m, w + 1);
+        for (int j = 0; j < hi; j++) {
+            const bool hit = std::abs(x[0] - y[j]) <= eps;
+            r.cur[j] = hit ? 1 : ((j > 0) ? r.cur[j - 1] : 0);
+        }
+    }
+    for (int i = 1; i < m; i++) {
+        r.next(0.0, i);
+        const double* P = r.prev;
+        double* C = r.cur;
+        const double xi = x[i];
+        int j = std::max(0, i - w);
+        const int hi = std::min(m, i + w + 1);
+        if (j == 0) {
+            C[0] = (std::abs(xi - y[0]) <= eps) ? 1 : P[0];
+            j = 1;
+        }
+        // the band's first cell reads 0 on its left, which the cleared
+        // column lo - 1 holds
+        double left = C[j - 1];
+        for (; j < hi; j++) {
+            const double h = (std::abs(xi - y[j]) <= eps) ? 1.0 : 0.0;
+            left = std::max(std::max(P[j - 1] + h, P[j]), left);
+            C[j] = left;
         }
     }
     return 1 - r.cur[m - 1] / m;
@@ -95,16 +121,24 @@ double lcss(const double* x, const double* y, int m, int w, double eps) {
 // ERP and EDR run on the (m + 1) x (m + 1) grid with a gap row and column
 double erp(const double* x, const double* y, int m, int w, double g) {
     Rows r(m + 1, w, kOut);
+    std::vector<double> gy((size_t)m);
+    for (int j = 0; j < m; j++) gy[j] = sq(y[j], g);
     r.cur[0] = 0.0;
-    for (int j = 1; j <= std::min(m, w); j++) r.cur[j] = r.cur[j - 1] + sq(y[j - 1], g);
+    for (int j = 1; j <= std::min(m, w); j++) r.cur[j] = r.cur[j - 1] + gy[j - 1];
     for (int i = 1; i <= m; i++) {
         r.next(kOut, i);
-        if (i <= w) r.cur[0] = r.prev[0] + sq(x[i - 1], g);
-        for (int j = std::max(1, i - w); j <= std::min(m, i + w); j++) {
-            const double match = r.prev[j - 1] + sq(x[i - 1], y[j - 1]);
-            const double ins = r.cur[j - 1] + sq(y[j - 1], g);
-            const double del = r.prev[j] + sq(x[i - 1], g);
-            r.cur[j] = min3(match, ins, del);
+        const double* P = r.prev;
+        double* C = r.cur;
+        const double xi = x[i - 1], gx = sq(xi, g);
+        if (i <= w) C[0] = P[0] + gx;
+        int j = std::max(1, i - w);
+        const int hi = std::min(m, i + w);
+        double left = C[j - 1];
+        for (; j <= hi; j++) {
+            const double match = P[j - 1] + sq(xi, y[j - 1]);
+            const double del = P[j] + gx;
+            left = std::min(std::min(match, del), left + gy[j - 1]);
+            C[j] = left;
         }
     }
     const double f = r.cur[m];
@@ -118,10 +152,17 @@ double edr(const double* x, const double* y, int m, int w, double eps) {
     for (int j = 1; j <= std::min(m, w); j++) r.cur[j] = (double)j;
     for (int i = 1; i <= m; i++) {
         r.next(kOut, i);
-        if (i <= w) r.cur[0] = (double)i;
-        for (int j = std::max(1, i - w); j <= std::min(m, i + w); j++) {
-            const double sub = (std::abs(x[i - 1] - y[j - 1]) <= eps) ? 0.0 : 1.0;
-            r.cur[j] = min3(r.prev[j] + 1.0, r.cur[j - 1] + 1.0, r.prev[j - 1] + sub);
+        const double* P = r.prev;
+        double* C = r.cur;
+        const double xi = x[i - 1];
+        if (i <= w) C[0] = (double)i;
+        int j = std::max(1, i - w);
+        const int hi = std::min(m, i + w);
+        double left = C[j - 1];
+        for (; j <= hi; j++) {
+            const double sub = (std::abs(xi - y[j - 1]) <= eps) ? 0.0 : 1.0;
+            left = std::min(std::min(P[j] + 1.0, P[j - 1] + sub), left + 1.0);
+            C[j] = left;
         }
     }
     return r.cur[m];
@@ -135,10 +176,17 @@ double swale(const double* x, const double* y, int m, int w, double eps) {
     for (int j = 1; j <= std::min(m, w); j++) r.cur[j] = r.cur[j - 1] + kP;
     for (int i = 1; i <= m; i++) {
         r.next(kOut, i);
-        if (i <= w) r.cur[0] = r.prev[0] + kP;
-        for (int j = std::max(1, i - w); j <= std::min(m, i + w); j++) {
-            const double mv = (std::abs(x[i - 1] - y[j - 1]) <= eps) ? kR : kP;
-            r.cur[j] = min3(r.prev[j - 1] + mv, r.prev[j] + kP, r.cur[j - 1] + kP);
+        const double* P = r.prev;
+        double* C = r.cur;
+        const double xi = x[i - 1];
+        if (i <= w) C[0] = P[0] + kP;
+        int j = std::max(1, i - w);
+        const int hi = std::min(m, i + w);
+        double left = C[j - 1];
+        for (; j <= hi; j++) {
+            const double mv = (std::abs(xi - y[j - 1]) <= eps) ? kR : kP;
+            left = std::min(std::min(P[j - 1] + mv, P[j] + kP), left + kP);
+            C[j] = left;
         }
     }
     return r.cur[m];
@@ -150,12 +198,18 @@ double msm(const double* x, const double* y, int m, int w, double c) {
     for (int j = 1; j < m && j <= w; j++) r.cur[j] = r.cur[j - 1] + split_cost(y[j], x[0], y[j - 1], c);
     for (int i = 1; i < m; i++) {
         r.next(kOut, i);
-        if (i <= w) r.cur[0] = r.prev[0] + split_cost(x[i], x[i - 1], y[0], c);
-        for (int j = std::max(1, i - w); j < std::min(m, i + w + 1); j++) {
-            const double c1 = r.prev[j - 1] + std::abs(x[i] - y[j]);
-            const double c2 = r.prev[j] + split_cost(x[i], x[i - 1], y[j], c);
-            const double c3 = r.cur[j - 1] + split_cost(y[j], x[i], y[j - 1], c);
-            r.cur[j] = min3(c1, c2, c3);
+        const double* P = r.prev;
+        double* C = r.cur;
+        const double xi = x[i], xp = x[i - 1];
+        if (i <= w) C[0] = P[0] + split_cost(xi, xp, y[0], c);
+        int j = std::max(1, i - w);
+        const int hi = std::min(m, i + w + 1);
+        double left = C[j - 1];
+        for (; j < hi; j++) {
+            const double c1 = P[j - 1] + std::abs(xi - y[j]);
+            const double c2 = P[j] + split_cost(xi, xp, y[j], c);
+            left = std::min(std::min(c1, c2), left + split_cost(y[j], xi, y[j - 1], c));
+            C[j] = left;
         }
     }
     return r.cur[m - 1];

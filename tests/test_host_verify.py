@@ -52,3 +52,20 @@ def test_host_eld_equals_oracle(host):
         got = lib.shim_host_eld(metric, q.ctypes.data_as(P), t.ctypes.data_as(P), m, w, arg)
         want = O.eld(metric, q, t, w, arg)
         assert np.float64(got).tobytes() == np.float64(want).tobytes(), (metric, m, w, arg)
+
+
+def test_host_eld_equals_oracle_long(host):
+    """Longer rows (m up to 700), so the band's interior, its edges and both
+    grid borders all occur in one pair."""
+    lib, P = host
+    rng = np.random.default_rng(17)
+    for trial in range(90):
+        m = int(rng.integers(100, 700))
+        w = int(rng.choice([0, 1, int(rng.integers(2, m)), m - 1]))
+        metric = 1 + trial % 6
+        arg = {1: 0.0, 2: 0.3, 3: 0.5, 4: 0.3, 5: 0.7, 6: 0.3}[metric]
+        q = np.cumsum(rng.normal(size=m))
+        t = np.ascontiguousarray(q + rng.normal(size=m) * (0.05 if trial % 2 else 1.0))
+        got = lib.shim_host_eld(metric, q.ctypes.data_as(P), t.ctypes.data_as(P), m, w, arg)
+        want = O.eld(metric, q, t, w, arg)
+        assert np.float64(got).tobytes() == np.float64(want).tobytes(), (metric, m, w, arg)
