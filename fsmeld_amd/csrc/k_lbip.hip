@@ -40,12 +40,14 @@ __host__ __device__ inline int npad_of(int n) {
 
 // Workgroup size per segment: 128 threads up to 1024 keys when there are
 // enough segments to fill the device with two-wave workgroups (short segments
-// are then dominated by the workgroup's barriers), 256 up to 1024, 512 up to
-// 4096 (two workgroups and 16 waves per CU with both envelopes in LDS), 1024
-// beyond, so that long segments spread their per-position work over more
+// are then dominated by the workgroup's barriers), 256 up to 2048 (the
+// autotuner's choice for 2048 keys on every workload it was run on), 512 up
+// to 4096 (two workgroups and 16 waves per CU with both envelopes in LDS),
+// 1024 beyond, so that long segments spread their per-position work over more
 // waves.  (npad, the keys' power of two, equals m_k.)
 inline int lbip_threads(int npad, int64_t segments) {
     if (npad <= 1024) return segments >= 4096 ? 128 : 256;
+    if (npad <= 2048) return 256;
     return npad <= 4096 ? 512 : 1024;
 }
 
@@ -74,7 +76,8 @@ __host__ __device__ inline size_t thr_lds_bytes(int npad, int nt) {
 // Layouts by segment length.  kDual (m_k <= 8192): U_n and -L_n both in LDS
 // (two m_k arrays), from one pass over the window statistics; with m_k = 4096
 // and 512 threads, 71 KiB, two workgroups per CU.  kSingle (m_k = 16384): one
-// m_k array, U_n for the upper threshold and mask, then -L_n (two passes);
+// m_k array, U_n for the upper threshold and mask, then -L_n (from one pass
+// over the statistics that parks -L_n in the block's global scratch);
 // 156 KiB with 1024 threads.  Beside the arrays: a zone for the chunk
 // extremes and then the threshold scratch, and the two mask bitmaps.
 // kSlab (longer segments): van Herk arrays and both envelopes in the block's
@@ -133,6 +136,8 @@ size_t lbip_slab_bytes(const Geometry& G) {
     const int nthr = G.m_k - 2 * G.m + 2;
     const int npad = npad_of(nthr);
     size_t b = sizeof(double) * (size_t)npad;
+    if (lbip_layout(G.m_k) == kSingle)  // -L_n while U_n's threshold is chosen
+        b += sizeof(double) * (size_t)G.m_k;
     if (lbip_layout(G.m_k) == kSlab)
         b += sizeof(double) * (8 * (size_t)G.kml + 2 * (size_t)G.m_k);
     return (b + 255) & ~(size_t)255;
@@ -1130,24 +1135,23 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
             mask(X2, t_l, false, GLw);
             __syncthreads();
         } else if constexpr (LAYOUT == kSingle) {
+            // one pass over the statistics for both envelopes: U_n into X,
+            // -L_n into the block's global scratch (after the full-sort
+            // buffer), copied into X once U_n's mask is built
             double* Cx = (double*)zone;
+            double* Lg = slab + npad;
             KeyStats ku, kl;
             if (m >= 64) {
                 chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx);
                 __syncthreads();
             }
-            norm_env_lds<1, NT>(MU, SG, t, a.Uraw + seg0, nullptr, Cx, m, mk, kml, w, X, nullptr,
-                                &ku, nullptr);
+            norm_env_lds<3, NT>(MU, SG, t, a.Uraw + seg0, a.Lraw + seg0, Cx, m, mk, kml, w, X,
+                                Lg, &ku, &kl);
             __syncthreads();
             t_u = threshold(keys, ku, true);
             mask(X, t_u, true, GUw);
             __syncthreads();
-            if (m >= 64) {  // the zone held threshold scratch since
-                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx);
-                __syncthreads();
-            }
-            norm_env_lds<2, NT>(MU, SG, t, nullptr, a.Lraw + seg0, Cx, m, mk, kml, w, nullptr, X,
-                                nullptr, &kl);
+            for (int i = threadIdx.x; i < mk; i += NT) X[i] = Lg[i];
             __syncthreads();
             t_l = threshold(keys, kl, false);
             mask(X, t_l, false, GLw);
