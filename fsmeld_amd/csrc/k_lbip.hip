@@ -306,7 +306,8 @@ __device__ void bitonic_sort(double* keys, int n) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             const bool mirror = j == (k >> 1);
             for (int q = threadIdx.x; q < (n >> 1); q += blockDim.x) {
-                const int i = ((q / j) * (j << 1)) + (q % j);  // bit j of i clear
+                // bit j of i clear (j a power of two: q / j, q % j as bit operations)
+                const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
                 const int p = mirror ? (i ^ (k - 1)) : (i + j);
                 const double a = keys[i], b = keys[p];
                 keys[i] = vmin(a, b);
