@@ -849,6 +849,44 @@ __device__ double select_threshold(const double* keys, int nthr, int npad, const
             const int o = cst[b];
             if (o >= 0) cand[o + atomicAdd(&fil[b], 1)] = v;
         }
+        if (K <= 64) {
+            // one wave sorts the candidates in its registers (a bitonic
+            // network over lanes, no block barriers) and takes the argmax
+            __threadfence_block();
+            __syncthreads();
+            const int lane = tid & 63;
+            if (tid < 64) {
+                double v = lane < K ? cand[lane] : __builtin_inf();
+                for (int k = 2; k <= 64; k <<= 1) {
+                    for (int j = k >> 1; j > 0; j >>= 1) {
+                        const double o = __shfl_xor(v, j, 64);
+                        // ascending within blocks of k whose bit k is clear
+                        const bool take_min = ((lane & j) == 0) == ((lane & k) == 0);
+                        v = take_min ? vmin(v, o) : vmax(v, o);
+                    }
+                }
+                ArgBest x{__builtin_nan(""), 0};
+                int64_t p = 0;
+                if (lane < K) {
+                    const int b = bucket_of(v, lo, scale, nbk);
+                    p = (int64_t)bst[b] + (lane - cst[b]);
+                    x = ArgBest{hf(v) * (double)(p + 1), p};
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    ArgBest y;
+                    y.v = __shfl_xor(x.v, o, 64);
+                    y.i = __shfl_xor(x.i, o, 64);
+                    x = better_max(x, y);
+                }
+                const uint64_t win = __ballot(lane < K && x.v == x.v && p == x.i);
+                const double key = __shfl(v, win ? __builtin_ctzll(win) : 0, 64);
+                if (lane == 0) *sh_key = (x.v == x.v) ? key : kmin;
+            }
+            __syncthreads();
+            const double t = *sh_key;
+            __syncthreads();
+            return t;
+        }
         for (int j = K + tid; j < kpad; j += NT) cand[j] = __builtin_inf();
         __threadfence_block();
         __syncthreads();
