@@ -808,14 +808,41 @@ __device__ double select_threshold(const double* keys, int nthr, int npad, const
         bmax[b] = 0ull;
     }
     __syncthreads();
-    for (int i = tid; i < nthr; i += NT) {
-        const double v = keys[i];
-        if (v != v) continue;
-        const int b = bucket_of(v, lo, scale, nbk);
-        const unsigned long long o = ord_key(v);
-        atomicAdd(&cnt[b], 1);
-        atomicMin(&bmin[b], o);
-        atomicMax(&bmax[b], o);
+    {
+        // each thread a contiguous run of keys (positions): neighbouring
+        // positions' envelope values mostly share a bucket, so a thread
+        // folds its run per bucket and updates the shared counts once per
+        // bucket change instead of once per key (lanes hitting one bucket
+        // serialise on its LDS word)
+        const int per = (nthr + NT - 1) / NT;
+        const int i1 = (tid + 1) * per < nthr ? (tid + 1) * per : nthr;
+        int rb = -1, rc = 0;
+        unsigned long long rmn = ~0ull, rmx = 0ull;
+        for (int i = tid * per; i < i1; i++) {
+            const double v = keys[i];
+            if (v != v) continue;
+            const int b = bucket_of(v, lo, scale, nbk);
+            const unsigned long long o = ord_key(v);
+            if (b != rb) {
+                if (rc) {
+                    atomicAdd(&cnt[rb], rc);
+                    atomicMin(&bmin[rb], rmn);
+                    atomicMax(&bmax[rb], rmx);
+                }
+                rb = b;
+                rc = 0;
+                rmn = ~0ull;
+                rmx = 0ull;
+            }
+            rc++;
+            rmn = o < rmn ? o : rmn;
+            rmx = o > rmx ? o : rmx;
+        }
+        if (rc) {
+            atomicAdd(&cnt[rb], rc);
+            atomicMin(&bmin[rb], rmn);
+            atomicMax(&bmax[rb], rmx);
+        }
     }
     __syncthreads();
     const int c = (tid < nbk) ? cnt[tid] : 0;
