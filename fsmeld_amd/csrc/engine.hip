@@ -130,12 +130,13 @@ void host_envelope(const std::vector<double>& a, int r, std::vector<double>& L,
 }
 
 struct HostQuery {
-    std::vector<double> qn, LQ, UQ, qs, S1, S2, SD, TU, TL, XG, LQE, UQE;
+    std::vector<double> qn, LQ, UQ, qs, S1, S2, SD, TU, TL, XG, MQ, HQ;
 };
+
 
 // src/fsmeld.cpp:622-643 (normalisation and sorted copy), :721 (envelope).
 // Raw matching (paper App. D) takes the query as it is.
-void prepare_query(const double* q_raw, int m, int w, double arg, HostQuery* H,
+void prepare_query(const double* q_raw, int m, int w, double arg, int metric, HostQuery* H,
                    bool raw = false) {
     H->qn.resize(m);
     if (raw) {
@@ -154,11 +155,15 @@ void prepare_query(const double* q_raw, int m, int w, double arg, HostQuery* H,
     host_envelope(H->qn, w, H->LQ, H->UQ);
     H->XG.resize(m);
     for (int i = 0; i < m; i++) H->XG[i] = (H->qn[i] - arg) * (H->qn[i] - arg);  // dist(x, g)
-    H->LQE.resize(m);
-    H->UQE.resize(m);
+    // envelope midpoints and half widths, the half width widened by eps
+    // (EDR / LCSS) or divided by c (MSM)
+    H->MQ.resize(m);
+    H->HQ.resize(m);
+    const double rho = glb_msm_rho(arg);
     for (int i = 0; i < m; i++) {
-        H->LQE[i] = H->LQ[i] - arg;
-        H->UQE[i] = H->UQ[i] + arg;
+        H->MQ[i] = (H->LQ[i] + H->UQ[i]) * 0.5;
+        const double h = (H->UQ[i] - H->LQ[i]) * 0.5;
+        H->HQ[i] = (metric == EDR || metric == LCSS) ? h + arg : (metric == MSM ? h * rho : h);
     }
     H->qs = H->qn;
     std::sort(H->qs.begin(), H->qs.end());
@@ -333,7 +338,7 @@ int upload_query(fsm_target* t, const HostQuery& H, int m, QueryTables* Q) {
     HIP_TRY(t->qtab.ensure(sizeof(double) * stride * 12));
     std::vector<double> buf(stride * 12, 0.0);
     const std::vector<double>* parts[12] = {&H.qn, &H.LQ, &H.UQ, &H.qs, &H.S1, &H.S2,
-                                            &H.SD, &H.TU, &H.TL, &H.XG, &H.LQE, &H.UQE};
+                                            &H.SD, &H.TU, &H.TL, &H.XG, &H.MQ, &H.HQ};
     for (int k = 0; k < 12; k++)
         std::copy(parts[k]->begin(), parts[k]->end(), buf.begin() + k * stride);
     HIP_TRY(hipMemcpyAsync(t->qtab.p, buf.data(), sizeof(double) * buf.size(),
@@ -349,8 +354,8 @@ int upload_query(fsm_target* t, const HostQuery& H, int m, QueryTables* Q) {
     Q->TU = d + 7 * stride;
     Q->TL = d + 8 * stride;
     Q->XG = d + 9 * stride;
-    Q->LQE = d + 10 * stride;
-    Q->UQE = d + 11 * stride;
+    Q->MQ = d + 10 * stride;
+    Q->HQ = d + 11 * stride;
     Q->qabs = 0.0;
     for (double v : H.qn) Q->qabs = std::max(Q->qabs, std::fabs(v));
     // the staging vector must outlive the async copy
@@ -881,7 +886,7 @@ int fsm_query(fsm_target* t, const fsm_query_desc* d, const double* q, int64_t q
     if (!make_geometry(t->N, d->m, &G))
         return fail(FSM_ERR_ARG, "Macro segment exceeds the supported FFT length.");
     HostQuery H;
-    prepare_query(q, d->m, d->w, d->arg, &H, d->raw != 0);
+    prepare_query(q, d->m, d->w, d->arg, d->metric, &H, d->raw != 0);
     QueryTables Q{};
     rc = upload_query(t, H, d->m, &Q);
     if (rc) return rc;
@@ -1146,7 +1151,7 @@ int fsm_stage_lbip(fsm_target* t, const fsm_query_desc* d, const double* q, doub
     Geometry G;
     if (!make_geometry(t->N, d->m, &G)) return fail(FSM_ERR_ARG, "invalid m");
     HostQuery H;
-    prepare_query(q, d->m, d->w, d->arg, &H, d->raw != 0);
+    prepare_query(q, d->m, d->w, d->arg, d->metric, &H, d->raw != 0);
     QueryTables Q{};
     if ((rc = upload_query(t, H, d->m, &Q))) return rc;
     QueryParams P = params_of(d);
@@ -1175,7 +1180,7 @@ static int stage_verify(fsm_target* t, const fsm_query_desc* d, const double* q,
         if (win[i] < 0 || win[i] >= G.ts_n) return fail(FSM_ERR_ARG, "window index out of range");
     if (n == 0) return FSM_OK;
     HostQuery H;
-    prepare_query(q, d->m, d->w, d->arg, &H, d->raw != 0);
+    prepare_query(q, d->m, d->w, d->arg, d->metric, &H, d->raw != 0);
     QueryTables Q{};
     if ((rc = upload_query(t, H, d->m, &Q))) return rc;
     QueryParams P = params_of(d);
@@ -1347,7 +1352,7 @@ int fsm_stage_lbip_dev(fsm_target* t, const fsm_query_desc* d, const double* q, 
     StreamJoin j(stream, t->stream);
     HIP_TRY(j.enter());
     HostQuery H;
-    prepare_query(q, d->m, d->w, d->arg, &H, d->raw != 0);
+    prepare_query(q, d->m, d->w, d->arg, d->metric, &H, d->raw != 0);
     QueryTables Q{};
     if ((rc = upload_query(t, H, d->m, &Q))) return rc;
     QueryParams P = params_of(d);
@@ -1383,7 +1388,7 @@ static int stage_verify_dev(fsm_target* t, const fsm_query_desc* d, const double
     HIP_TRY(hipStreamSynchronize(t->stream));
     if (bad) return fail(FSM_ERR_ARG, "window index out of range");
     HostQuery H;
-    prepare_query(q, d->m, d->w, d->arg, &H, d->raw != 0);
+    prepare_query(q, d->m, d->w, d->arg, d->metric, &H, d->raw != 0);
     QueryTables Q{};
     if ((rc = upload_query(t, H, d->m, &Q))) return rc;
     QueryParams P = params_of(d);

@@ -49,10 +49,18 @@ struct QueryTables {
     const double* TU;   // fl(qs - arg), fl(qs + arg) [m]
     const double* TL;
     const double* XG;   // (qn - arg)^2 [m], the ERP gap cost of each query sample
-    const double* LQE;  // fl(LQ - arg), fl(UQ + arg) [m]: the envelope widened by eps
-    const double* UQE;
+    const double* MQ;   // fl(LQ + UQ) / 2, the envelope midpoints [m]
+    const double* HQ;   // h = fl(UQ - LQ) / 2: fl(h + arg) (EDR, LCSS), fl(h glb_msm_rho(arg))
+                        // (MSM), h (others) [m]
     double qabs;        // max |qn|, which also bounds |LQ| and |UQ|
 };
+
+// 1 / c for GLB's MSM terms, which it measures in units of c.  Outside this
+// range 1 / c would leave the normal range, and the fast pass is not used;
+// c = 0 gives 0, which makes every term 0 (as min(d, 0) is).
+__host__ __device__ inline double glb_msm_rho(double c) {
+    return (c >= 1e-100 && c <= 1e100) ? 1.0 / c : 0.0;
+}
 
 struct QueryParams {
     int metric;

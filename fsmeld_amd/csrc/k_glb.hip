@@ -298,18 +298,23 @@ constexpr int kGlbChunk = (METRIC == MSM || METRIC == DTW) ? 32 : 16;
 // envelope at every (2w + 1)-th position instead of every sample.
 constexpr int kGlbSampleW = 4;
 
-// Contribution of one position with envelope distance e >= 0 (fast path).
-// xg2 = (x - g)^2 of the moving sample (ERP).  DTW / ERP / MSM only; the
-// counting metrics take their own form in glb_fast.
-template <int METRIC>
-__device__ inline void fast_term(double e, double xg2, double arg, double& s) {
-    if (METRIC == DTW) {
-        s = fma(e, e, s);
-    } else if (METRIC == ERP) {
-        s += fmin(e * e, xg2);
-    } else {
-        s += fmin(e, arg);
-    }
+// Pass 1's distance of a candidate sample a to the query envelope [L, U] is
+// |d| - h with d = fl(fl(t - mu) r - M), M = fl(L + U) / 2, h = fl(U - L) / 2
+// (exactly max(L - a, a - U, 0) when positive, in exact arithmetic), h
+// replaced by fl(h + eps) for EDR / LCSS and the whole divided by c, i.e.
+// fl(|d| fl(1 / c) - fl(h fl(1 / c))) clamped to [0, 1], for MSM.  Bound on
+// each term's distance from the reference's term (u = 2^-53, |a| <= amax,
+// |L|, |U| <= QM): |d| is within 3.02 u |a| + 2.01 u QM of |a - M| (two
+// roundings in a, u QM in M, one after the product), and |d| - h within
+// 4.03 u |a| + 4.02 u QM of the exact distance (h: u QM, the difference: u (|a|
+// + QM)); MSM's 1 / c and h / c add 2.03 u QM and the clamped fma u c.  The
+// reference's own normalisation and subtraction are within 3.02 u |a| + 1.01 u QM
+// of the exact distance.  ERP's a - g, fl(fl(t - mu) r - g) against the
+// reference's fl(a' - g): 6.04 u |a| + 2.02 u |g|.  So every DTW / ERP / MSM
+// term argument is within 7.06 u (amax + QM + |arg|) < 8.2e-16 (amax + QM +
+// |arg|); 1e-220 covers results in the subnormal range (c and 1 / c <= 1e100).
+__device__ inline double glb_delta1(double amax, double QM, double arg) {
+    return 8.2e-16 * (amax + QM + fabs(arg)) + 1e-220;
 }
 
 // ERP's term from the distance itself: min(e^2, xg^2) = min(e, |xg|)^2 for
@@ -331,23 +336,21 @@ __device__ inline void erp_term(double e, double xg, double& s) {
 //
 // EDR / LCSS count positions.  With eps > 0 a position mismatches (EDR) or
 // fails to match (LCSS) exactly when R = max(L - a, a - U) - eps > 0 for its
-// envelope [L, U] (the clamp at 0 cannot reach eps), and the fast pass takes
-// R from the envelope widened by eps, LQE = fl(LQ - eps) and UQE = fl(UQ + eps),
-// in one max of two differences.  Against the reference's rounded quantities
-// (its own normalisation fl(fl(t - mu) / sd) and subtraction) the computed R
-// is off by at most 7.01 u |a| + 3 u max|q| + 2 u eps (pass 1: 3 u |a| from a,
-// u (max|q| + eps) from the widened envelope, u |R + eps| from the difference,
-// 3 u |a| + u max|q| on the reference's side), and pass 2, taken in raw units
-// against eps sd, by at most 5 u amax + 4 u max|q| + 2 u eps + u |mu| / sd.
-// eta = 1.2e-15 (|a| + max|q| + eps) (pass 2: plus 1.3e-16 |mu| / sd, scaled
-// by sd) covers both; a position with R > eta is certainly out, with R < -eta
-// certainly in, so counting R > eta and R >= -eta gives both ends of the
-// reference's count.
+// envelope [L, U] (the clamp at 0 cannot reach eps).  Pass 1 takes R as
+// fl(|d| - fl(h + eps)) (glb_delta1's d and h): against the reference's rounded
+// quantities (its own normalisation fl(fl(t - mu) / sd) and subtraction) it is
+// off by at most 7.05 u |a| + 6.05 u max|q| + 2.01 u eps (|d|: 3.02 u |a| + 2.01
+// u QM; h + eps: u QM + u (QM + eps); the difference: u (|a| + 2 QM + eps); the
+// reference's side: 3.02 u |a| + 1.01 u QM).  Pass 2 compares max(lo - qr, qr -
+// hi) in raw units with fl(eps sd +- eta sd), within 5 u amax + 4 u max|q| + 2 u
+// eps + u |mu| / sd.  eta = 1.2e-15 (|a| + max|q| + eps) (pass 2: plus 1.3e-16
+// |mu| / sd, scaled by sd) covers both; a position with R > eta is certainly
+// out, with R < -eta certainly in, so counting R > eta and R >= -eta gives both
+// ends of the reference's count.
 template <int METRIC>
 __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __restrict__ Q,
-                        const double* __restrict__ LQ, const double* __restrict__ UQ,
-                        const double* __restrict__ XG, const double* __restrict__ LQE,
-                        const double* __restrict__ UQE, const double* __restrict__ t,
+                        const double* __restrict__ XG, const double* __restrict__ MQ,
+                        const double* __restrict__ HQ, const double* __restrict__ t,
                         const double* __restrict__ Lr, const double* __restrict__ Ur,
                         double mu, double sd, bool active, double* total) {
     constexpr bool COUNT = (METRIC == EDR || METRIC == LCSS);
@@ -395,19 +398,31 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
     const double cetaC = kEtaC * (QM + arg) + 1e-300;
     const double eta1 = kEtaC * amax + cetaC;  // with a sampled amax
     int cgt1 = 0, cge1 = 0;
+    // MSM sums its terms in units of c (s1, s2), scaled back where they are
+    // used: each term is one clamp to [0, 1] of (|d| - h) / c
+    const double rho = glb_msm_rho(arg);
+    const double sc = (METRIC == MSM) ? arg : 1.0;
     auto term1 = [&](int k, auto track) {
-        const double av = (t[k] - mu) * r;
-        if constexpr (decltype(track)::value) amax = fmax(amax, fabs(av));
+        const double tm = t[k] - mu;
+        // the signed distance from the envelope's midpoint, d = a - MQ with
+        // a = (t - mu) r, in one rounding after the product
+        const double d = fma(tm, r, -MQ[k]);
+        double et = eta1;
+        if constexpr (decltype(track)::value) {
+            const double av = tm * r;
+            amax = fmax(amax, fabs(av));
+            et = fma(fabs(av), kEtaC, cetaC);
+        }
         if constexpr (COUNT) {
-            const double R = fmax(LQE[k] - av, av - UQE[k]);
-            const double et = decltype(track)::value ? fma(fabs(av), kEtaC, cetaC) : eta1;
+            const double R = fabs(d) - HQ[k];  // HQ: the half width widened by eps
             cgt1 += (R > et) ? 1 : 0;
             cge1 += (R >= -et) ? 1 : 0;
-        } else if constexpr (METRIC == ERP) {
-            erp_term(fmax(fmax(LQ[k] - av, av - UQ[k]), 0.0), av - arg, s1);
+        } else if constexpr (METRIC == MSM) {
+            s1 += fmin(fmax(fma(fabs(d), rho, -HQ[k]), 0.0), 1.0);
         } else {
-            const double e = fmax(fmax(LQ[k] - av, av - UQ[k]), 0.0);
-            fast_term<METRIC>(e, 0.0, arg, s1);
+            const double e = fmax(fabs(d) - HQ[k], 0.0);
+            if constexpr (METRIC == ERP) erp_term(e, fma(tm, r, -arg), s1);
+            else s1 = fma(e, e, s1);
         }
     };
     auto check1 = [&](int ie) {
@@ -419,7 +434,8 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
         } else if (METRIC == EDR) {
             dead = B + (double)cgt1 > bsf;
         } else {
-            const double lo = s1 - glb_sum_err<METRIC>(s1, n, 4e-16 * amax + 1e-300, amax + kq);
+            const double S = s1 * sc;
+            const double lo = S - glb_sum_err<METRIC>(S, n, glb_delta1(amax, QM, arg), amax + kq);
             dead = B + lo > thr;
         }
     };
@@ -439,27 +455,45 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
 
     // pass 2: query against the candidate envelope (raw window min / max over
     // positions clipped to the window).  The distance is taken in raw units,
-    // e = max(lo - qr, qr - hi, 0) * r with qr = fl(q sd + mu), and differs
-    // from the reference's fl(q - UT) / fl(LT - q) on its normalised envelope
-    // by at most 6.02 u amax + 5.02 u QM + 1.01 u |mu| / sd (u = 2^-53):
+    // x = max(lo - qr, qr - hi) with qr = fl(q sd + mu); e = max(x, 0) r
+    // differs from the reference's fl(q - UT) / fl(LT - q) on its normalised
+    // envelope by at most 6.02 u amax + 5.02 u QM + 1.01 u |mu| / sd (u = 2^-53):
     // 2 u |a| from the reference's normalisation, 3 u e from the subtraction,
     // r and the product, u |q + mu / sd| from qr, u e for the reference's own
-    // subtraction, e <= amax + QM).
+    // subtraction, e <= amax + QM).  DTW sums max(x, 0)^2 and multiplies by r^2
+    // once (a few u relative, inside glb_sum_err's psi).  MSM clamps x rc,
+    // rc = fl(r fl(1 / c)), to [0, 1]: four roundings on x / sd (4.02 u (amax +
+    // QM)) besides the subtraction's, 8.06 u amax + 7.05 u QM + u |mu| / sd in
+    // all with the reference's side.
     const double mur = fabs(mu) * r;
-    const double delta2 = 7e-16 * (amax + QM) + 1.2e-16 * mur + 1e-300;
+    const double delta2 = (METRIC == MSM)
+                              ? 9.2e-16 * (amax + QM) + 1.2e-16 * mur + 1e-220
+                              : 7e-16 * (amax + QM) + 1.2e-16 * mur + 1e-300;
+    // s2: DTW sums the squared raw distances (times r^2 where used), MSM in
+    // units of c as pass 1
     double s2 = 0.0;
+    const double sc2 = (METRIC == DTW) ? r * r : sc;
+    const double rc = r * rho;
     const double epsr = arg * sd;
     const double eta2r = sd * (kEtaC * (amax + QM + arg) + 1.3e-16 * mur) + 1e-300;
+    // counting: R = max(lo - qr, qr - hi) - eps sd against +-eta2r, as
+    // max(lo - qr, qr - hi) against the two sums
+    const double rgt2 = epsr + eta2r, rge2 = epsr - eta2r;
     int cgt2 = 0, cge2 = 0;
     auto term2 = [&](int k, double lo, double hi) {
         const double qr = fma(Q[k], sd, mu);
-        if constexpr (COUNT) {  // R in raw units: (max(lo - qr, qr - hi) - eps sd)
-            const double R = fmax(lo - qr, qr - hi) - epsr;
-            cgt2 += (R > eta2r) ? 1 : 0;
-            cge2 += (R >= -eta2r) ? 1 : 0;
-        } else {
-            const double e = fmax(fmax(lo - qr, qr - hi), 0.0) * r;
-            fast_term<METRIC>(e, (METRIC == ERP) ? XG[k] : 0.0, arg, s2);
+        const double x = fmax(lo - qr, qr - hi);  // raw units
+        if constexpr (COUNT) {
+            cgt2 += (x > rgt2) ? 1 : 0;
+            cge2 += (x >= rge2) ? 1 : 0;
+        } else if constexpr (METRIC == MSM) {
+            s2 += fmin(fmax(x * rc, 0.0), 1.0);
+        } else if constexpr (METRIC == DTW) {
+            const double e = fmax(x, 0.0);
+            s2 = fma(e, e, s2);
+        } else {  // ERP
+            const double e = fmax(x, 0.0) * r;
+            s2 += fmin(e * e, XG[k]);
         }
     };
     auto check2 = [&]() {
@@ -470,7 +504,8 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
         } else if (METRIC == EDR) {
             dead = B + (double)cgt2 > bsf;
         } else {
-            const double lo = s2 - glb_sum_err<METRIC>(s2, n, delta2, amax + kq);
+            const double S = s2 * sc2;
+            const double lo = S - glb_sum_err<METRIC>(S, n, delta2, amax + kq);
             dead = B + lo > thr;
         }
     };
@@ -530,13 +565,14 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
         Qlo = (double)(npos - cge2);
         Qhi = (double)(npos - cgt2);
     } else {
-        const double delta = 4e-16 * amax + 1e-300, K = amax + kq;
-        const double e1 = glb_sum_err<METRIC>(s1, n, delta, K);
-        const double e2 = glb_sum_err<METRIC>(s2, n, delta2, K);
-        Tlo = fmax(s1 - e1, 0.0);
-        Thi = s1 + e1;
-        Qlo = fmax(s2 - e2, 0.0);
-        Qhi = s2 + e2;
+        const double K = amax + kq;
+        const double S1 = s1 * sc, S2 = s2 * sc2;
+        const double e1 = glb_sum_err<METRIC>(S1, n, glb_delta1(amax, QM, arg), K);
+        const double e2 = glb_sum_err<METRIC>(S2, n, delta2, K);
+        Tlo = fmax(S1 - e1, 0.0);
+        Thi = S1 + e1;
+        Qlo = fmax(S2 - e2, 0.0);
+        Qhi = S2 + e2;
     }
     // pass 1's terms are the reference's rounded terms; the real-valued sum
     // is lower by at most psi relative
@@ -559,8 +595,8 @@ k_glb(VerifyArgs a, const double* __restrict__ Q,
                                              const double* __restrict__ LQ,
                                              const double* __restrict__ UQ,
                                              const double* __restrict__ XG,
-                                             const double* __restrict__ LQE,
-                                             const double* __restrict__ UQE,
+                                             const double* __restrict__ MQ,
+                                             const double* __restrict__ HQ,
                                              const int64_t* __restrict__ in,
                                              const unsigned long long* __restrict__ in_count,
                                              int64_t* __restrict__ out,
@@ -578,13 +614,15 @@ k_glb(VerifyArgs a, const double* __restrict__ Q,
     double* Ls = Ts + stage_len;
     double* Us = Ls + stage_len;
     const int64_t n = (int64_t)*in_count;
-    // the fast decision needs contributions >= 0 and, for EDR / LCSS, an
-    // envelope that counts as a match (eps > 0); values (vals) need the exact path
-    // (SWALE takes the exact path only)
+    // the fast decision needs contributions >= 0 (MSM: c = 0 or 1 / c
+    // normal, glb_msm_rho), for EDR / LCSS an envelope that counts as a match
+    // (eps > 0), and envelope midpoints that cannot overflow; values (vals)
+    // need the exact path (SWALE takes the exact path only)
+    const bool arg_ok = (METRIC == DTW || METRIC == ERP) ? true
+                        : (METRIC == MSM) ? (a.p.arg == 0.0 || glb_msm_rho(a.p.arg) > 0.0)
+                                          : a.p.arg > 0.0;
     const bool fast = METRIC != SWALE && vals == nullptr && a.p.cutoff < kGlbAbandon &&
-                      a.p.m > 2 &&
-                      ((METRIC == DTW || METRIC == ERP) || (METRIC == MSM && a.p.arg >= 0.0) ||
-                       a.p.arg > 0.0);
+                      a.p.m > 2 && a.q.qabs <= 1e300 && arg_ok;
     // Blocks take runs of blockDim.x consecutive candidates from the queue
     // until it is drained: per-candidate work varies (early exits), so a
     // fixed assignment leaves most of the GPU idle behind the slowest blocks.
@@ -648,10 +686,10 @@ k_glb(VerifyArgs a, const double* __restrict__ Q,
         }
         if (fast && staged) {
             const int64_t off = (active ? g : gfirst) - gfirst;
-            verdict = glb_fast<METRIC>(a, Q, LQ, UQ, XG, LQE, UQE, Ts + off, Ls + off, Us + off,
+            verdict = glb_fast<METRIC>(a, Q, XG, MQ, HQ, Ts + off, Ls + off, Us + off,
                                        mu, sd, active, &total);
         } else if (fast) {
-            verdict = glb_fast<METRIC>(a, Q, LQ, UQ, XG, LQE, UQE, a.T + g, a.Lraw + g,
+            verdict = glb_fast<METRIC>(a, Q, XG, MQ, HQ, a.T + g, a.Lraw + g,
                                        a.Uraw + g, mu, sd, active, &total);
         }
         bool keep = verdict > 0;
@@ -685,7 +723,7 @@ hipError_t launch_glb(hipStream_t st, const VerifyArgs& a, const int64_t* in,
     const int stage_len = (a.p.m + 512 <= kGlbStage) ? kGlbStage : 0;
     const size_t lds = sizeof(double) * 3 * (size_t)stage_len;
 #define FSM_GLB(MM) \
-    hipLaunchKernelGGL(k_glb<MM>, g, b, lds, st, a, q.qn, q.LQ, q.UQ, q.XG, q.LQE, q.UQE, in, \
+    hipLaunchKernelGGL(k_glb<MM>, g, b, lds, st, a, q.qn, q.LQ, q.UQ, q.XG, q.MQ, q.HQ, in, \
                        in_count, out, out_count, out_total, vals, queue, stage_len)
     switch (a.p.metric) {
         case DTW: FSM_GLB(DTW); break;

@@ -125,6 +125,22 @@ def test_glb_filter_decisions(gpu, small, metric, arg, m, w):
     """The search filter's keep decisions equal `glb <= bsf` of the oracle,
     with bsf set exactly on, and one ulp below, some windows' GLB values so
     that the decision interval of the fast pass straddles the threshold."""
+    glb_decisions(small, metric, arg, m, w)
+
+
+@pytest.mark.parametrize("metric,arg", [("msm", 0.0), ("msm", 1e-3), ("msm", 3.0), ("msm", 1e-120),
+                                        ("erp", 0.7), ("erp", -1.3), ("edr", 1e-3),
+                                        ("lcss", 2.0)])
+@pytest.mark.parametrize("m,w", [(64, 6), (40, 0)])
+def test_glb_filter_decisions_args(gpu, small, metric, arg, m, w):
+    """The same at the ends of the metric arguments: MSM with c = 0 (every
+    envelope term 0), tiny and large c, and c too small for the fast pass's
+    1 / c (the exact path decides); ERP with g != 0; EDR / LCSS with a tiny
+    and a wide eps."""
+    glb_decisions(small, metric, arg, m, w)
+
+
+def glb_decisions(small, metric, arg, m, w):
     t, T = small
     q, _ = F.synth_query(11, t, m, 0.1)
     mu, sg, inv = O.window_stats(t, m)
