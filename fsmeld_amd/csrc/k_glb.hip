@@ -362,6 +362,10 @@ __device__ __forceinline__ int glb_fast(const VerifyArgs& A, const double* __res
     const int i0 = (METRIC == LCSS) ? 0 : 1;
     const int i1 = (METRIC == LCSS) ? n : n - 1;
     const int npos = i1 - i0;
+    // DTW's pass 2 scales by r^2 and MSM's by r / c once: with sd in this
+    // range those factors stay normal (no underflow to 0 against an overflowed
+    // sum); other windows take the exact path
+    if ((METRIC == DTW || METRIC == MSM) && active && !(sd >= 1e-100 && sd <= 1e100)) return 0;
 
     double B = 0.0;
     if (METRIC != LCSS)

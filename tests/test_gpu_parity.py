@@ -140,6 +140,16 @@ def test_glb_filter_decisions_args(gpu, small, metric, arg, m, w):
     glb_decisions(small, metric, arg, m, w)
 
 
+@pytest.mark.parametrize("metric,arg", [("dtw", 0.0), ("msm", 0.5), ("erp", 0.0)])
+def test_glb_filter_decisions_huge_values(gpu, small, metric, arg):
+    """A target scaled by 1e120 (window std far above 1e100): DTW and MSM
+    windows leave the fast pass for the exact path, whose decisions still
+    equal the oracle's."""
+    t, _ = small
+    big = np.ascontiguousarray(t * 1e120)
+    glb_decisions((big, F.Target(big)), metric, arg, 64, 6)
+
+
 def glb_decisions(small, metric, arg, m, w):
     t, T = small
     q, _ = F.synth_query(11, t, m, 0.1)
