@@ -88,8 +88,16 @@ enum LbipLayout { kDual = 0, kSingle = 1, kSlab = 2 };
 __host__ __device__ inline int lbip_layout(int mk) {
     return mk <= 8192 ? kDual : (mk <= 16384 ? kSingle : kSlab);
 }
-__host__ __device__ inline size_t zone_bytes(int npad, int nt, int kml) {
-    const size_t c = 4 * sizeof(double) * (size_t)((kml + 63) / 64);
+// Sparse-table levels above the 64-window chunk extremes (kDual): a tile's
+// middle spans at most m / 64 - 1 whole chunks, so levels 1 .. floor(log2(m /
+// 64)) (ranges of 2^k chunks) answer it with two reads per quantity.
+__host__ __device__ inline int chunk_levels(int m) {
+    int k = 0;
+    while ((2 << k) <= m / 64) k++;
+    return k;
+}
+__host__ __device__ inline size_t zone_bytes(int npad, int nt, int kml, int levels) {
+    const size_t c = 4 * sizeof(double) * (size_t)((kml + 63) / 64) * (size_t)(1 + levels);
     const size_t t = thr_lds_bytes(npad, nt);
     return c > t ? c : t;
 }
@@ -108,7 +116,7 @@ size_t lbip_lds_bytes(const Geometry& G, int nt_want) {
     // upper mask's end, whose bits only ever meet zero penalties)
     if (lay != kSlab)
         return sizeof(double) * (size_t)G.m_k * (lay == kDual ? 2 : 1) +
-               zone_bytes(npad, nt, (int)G.kml) +
+               zone_bytes(npad, nt, (int)G.kml, lay == kDual ? chunk_levels(G.m) : 0) +
                sizeof(uint64_t) * (2 * (size_t)nw + kMaskPadWords);
     // kSlab: the keys in global scratch; the threshold scratch, then the
     // penalty prefix sums and the mask bitmaps, in LDS
@@ -549,18 +557,34 @@ __device__ __forceinline__ void tile_finish(const TileScan<MN, MX>& r, const Til
 
 // The tile's extremes of both statistics: the mean's minimum (MUMN) and / or
 // maximum (MUMX), the std's minimum and maximum.  C: the chunk extremes.
-template <bool MUMN, bool MUMX>
+template <bool MUMN, bool MUMX, bool LEVELS>
 __device__ __forceinline__ void tile_ext(const TileVals& vm, const TileVals& vs,
                                          const TileGeom& g, const double* C, int nch,
                                          double* mumin, double* mumax, double* sgmin,
                                          double* sgmax) {
     TileScan<MUMN, MUMX> a = tile_scan<MUMN, MUMX>(vm, g);
     TileScan<true, true> b = tile_scan<true, true>(vs, g);
-    for (int c = g.ca + 1; c < g.cb; c++) {  // whole chunks
-        if constexpr (MUMN) a.mmn = nmin(a.mmn, C[c]);
-        if constexpr (MUMX) a.mmx = nmax(a.mmx, C[nch + c]);
-        b.mmn = nmin(b.mmn, C[2 * nch + c]);
-        b.mmx = nmax(b.mmx, C[3 * nch + c]);
+    if constexpr (LEVELS) {
+        // whole chunks ca + 1 .. cb - 1 from the sparse table: level k holds
+        // ranges of 2^k chunks, two of which (overlapping) cover the middle;
+        // the NaN-propagating min / max are idempotent
+        const int len = g.cb - g.ca - 1;
+        if (len > 0) {
+            const int k = 31 - __builtin_clz(len);
+            const double* L = C + (size_t)k * 4 * nch;
+            const int c0 = g.ca + 1, c1 = g.cb - (1 << k);
+            if constexpr (MUMN) a.mmn = nmin(a.mmn, nmin(L[c0], L[c1]));
+            if constexpr (MUMX) a.mmx = nmax(a.mmx, nmax(L[nch + c0], L[nch + c1]));
+            b.mmn = nmin(b.mmn, nmin(L[2 * nch + c0], L[2 * nch + c1]));
+            b.mmx = nmax(b.mmx, nmax(L[3 * nch + c0], L[3 * nch + c1]));
+        }
+    } else {
+        for (int c = g.ca + 1; c < g.cb; c++) {  // whole chunks
+            if constexpr (MUMN) a.mmn = nmin(a.mmn, C[c]);
+            if constexpr (MUMX) a.mmx = nmax(a.mmx, C[nch + c]);
+            b.mmn = nmin(b.mmn, C[2 * nch + c]);
+            b.mmx = nmax(b.mmx, C[3 * nch + c]);
+        }
     }
     tile_finish<MUMN, MUMX>(a, g, mumin, mumax);
     tile_finish<true, true>(b, g, sgmin, sgmax);
@@ -571,7 +595,7 @@ __device__ __forceinline__ void tile_ext(const TileVals& vm, const TileVals& vs,
 // (std min, max); NaN when the chunk holds a NaN statistic.
 template <int NT>
 __device__ void chunk_extremes(const double* __restrict__ MU, const double* __restrict__ SG,
-                               int kml, int nch, double* C) {
+                               int kml, int nch, double* C, int levels) {
     const double inf = __builtin_inf(), nan = __builtin_nan("");
     const int lane = threadIdx.x & 63;
     for (int c = threadIdx.x >> 6; c < nch; c += NT / 64) {
@@ -595,6 +619,19 @@ __device__ void chunk_extremes(const double* __restrict__ MU, const double* __re
             C[nch + c] = nm ? nan : b;
             C[2 * nch + c] = ns ? nan : e;
             C[3 * nch + c] = ns ? nan : f;
+        }
+    }
+    // sparse-table level k: extremes of chunks c .. c + 2^k - 1 (entries
+    // whose range passes the last chunk are never read)
+    for (int k = 1; k <= levels; k++) {
+        __syncthreads();
+        const double* P = C + (size_t)(k - 1) * 4 * nch;
+        double* D = C + (size_t)k * 4 * nch;
+        const int h = 1 << (k - 1);
+        for (int x = threadIdx.x; x < 4 * nch; x += NT) {
+            const int q = x / nch, c = x - q * nch;
+            const double u = P[x], v = (c + h < nch) ? P[x + h] : u;
+            D[x] = (q & 1) ? nmax(u, v) : nmin(u, v);
         }
     }
 }
@@ -624,7 +661,7 @@ struct KeyStats {
 // segment's own, src/lower_upper_lemire.cpp:8-44) and the statistics'
 // extremes over the position's windows.  ku / kl: the keys (positions m - 1
 // .. m_k - m).
-template <int PART, int NT>
+template <int PART, int NT, bool LEVELS>
 __device__ void norm_env_lds(const double* __restrict__ MU, const double* __restrict__ SG,
                              const double* __restrict__ t, const double* __restrict__ Eu,
                              const double* __restrict__ El, const double* C, int m, int mk,
@@ -667,7 +704,7 @@ __device__ void norm_env_lds(const double* __restrict__ MU, const double* __rest
             // (not used within w of the edges)
             const double Uv = U ? Eu[i] : 0.0, Lv = L ? El[i] : 0.0;
             double mumin = 0.0, mumax = 0.0, sgmin, sgmax;
-            tile_ext<U, L>(vm, vs, g, C, nch, &mumin, &mumax, &sgmin, &sgmax);
+            tile_ext<U, L, LEVELS>(vm, vs, g, C, nch, &mumin, &mumax, &sgmin, &sgmax);
             finish(i, Uv, Lv, mumin, mumax, sgmin, sgmax);
         }
     } else {  // ranges of fewer than 64 windows: a direct loop per position
@@ -1110,6 +1147,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
     const int npad = npad_of(nthr);
     const int nw = (mk + 63) / 64;
     constexpr bool xl = LAYOUT != kSlab;
+    constexpr bool kLevels = LAYOUT == kDual;  // sparse-table levels in the zone
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 
     double* slab = (double*)((char*)a.slab + (size_t)blockIdx.x * a.slab_bytes_per_block);
@@ -1126,7 +1164,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
     int64_t *CHl, *CHu;
     uint64_t *GLw, *GUw, *TLw, *TUw;  // masks, transitions: bit p = g[p] xor g[p - 1]
     if (xl) {
-        GLw = (uint64_t*)(zone + zone_bytes(npad, NT, kml));
+        GLw = (uint64_t*)(zone + zone_bytes(npad, NT, kml, kLevels ? chunk_levels(m) : 0));
         GUw = GLw + nw;
         CHl = (int64_t*)X;  // X is dead once both masks are built
         CHu = CHl + (m + 1);
@@ -1190,12 +1228,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
         if constexpr (LAYOUT == kDual) {
             double* Cx = (double*)zone;
             if (m >= 64) {
-                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx);
+                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx, chunk_levels(m));
                 __syncthreads();
             }
             KeyStats ku, kl;
-            norm_env_lds<3, NT>(MU, SG, t, a.Uraw + seg0, a.Lraw + seg0, Cx, m, mk, kml, w, X,
-                                X2, &ku, &kl);
+            norm_env_lds<3, NT, true>(MU, SG, t, a.Uraw + seg0, a.Lraw + seg0, Cx, m, mk, kml, w,
+                                      X, X2, &ku, &kl);
             __syncthreads();  // (the zone's chunk extremes are dead from here)
             t_u = threshold(X + (m - 1), ku, true);
             mask(X, t_u, true, GUw);
@@ -1210,11 +1248,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
             double* Lg = slab + npad;
             KeyStats ku, kl;
             if (m >= 64) {
-                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx);
+                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx, 0);
                 __syncthreads();
             }
-            norm_env_lds<3, NT>(MU, SG, t, a.Uraw + seg0, a.Lraw + seg0, Cx, m, mk, kml, w, X,
-                                Lg, &ku, &kl);
+            norm_env_lds<3, NT, false>(MU, SG, t, a.Uraw + seg0, a.Lraw + seg0, Cx, m, mk, kml, w,
+                                       X, Lg, &ku, &kl);
             __syncthreads();
             t_u = threshold(keys, ku, true);
             mask(X, t_u, true, GUw);
