@@ -88,18 +88,33 @@ enum LbipLayout { kDual = 0, kSingle = 1, kSlab = 2 };
 __host__ __device__ inline int lbip_layout(int mk) {
     return mk <= 8192 ? kDual : (mk <= 16384 ? kSingle : kSlab);
 }
-// Sparse-table levels above the 64-window chunk extremes (kDual): a tile's
-// middle spans at most m / 64 - 1 whole chunks, so levels 1 .. floor(log2(m /
-// 64)) (ranges of 2^k chunks) answer it with two reads per quantity.
+// Sparse-table levels above the 64-window chunk extremes: a tile's middle
+// spans at most m / 64 - 1 whole chunks, so levels 1 .. floor(log2(m / 64))
+// (ranges of 2^k chunks) answer it with two reads per quantity (kDual).
 __host__ __device__ inline int chunk_levels(int m) {
     int k = 0;
     while ((2 << k) <= m / 64) k++;
+    return k;
+}
+// kSingle has no LDS to spare: as many levels as fit the threshold scratch
+// that shares the zone (a tile's middle then takes about len / 2^levels
+// reads per quantity).
+__host__ __device__ inline int chunk_levels_in(int m, int kml, size_t budget) {
+    const size_t c = 4 * sizeof(double) * (size_t)((kml + 63) / 64);
+    int k = chunk_levels(m);
+    while (k > 0 && c * (size_t)(1 + k) > budget) k--;
     return k;
 }
 __host__ __device__ inline size_t zone_bytes(int npad, int nt, int kml, int levels) {
     const size_t c = 4 * sizeof(double) * (size_t)((kml + 63) / 64) * (size_t)(1 + levels);
     const size_t t = thr_lds_bytes(npad, nt);
     return c > t ? c : t;
+}
+// The levels each layout keeps (kSlab computes its ranges otherwise).
+__host__ __device__ inline int lbip_levels(int layout, int m, int kml, int npad, int nt) {
+    if (layout == kDual) return chunk_levels(m);
+    if (layout == kSingle) return chunk_levels_in(m, kml, thr_lds_bytes(npad, nt));
+    return 0;
 }
 
 // FP32 matrix-core bounds (kDual only): words read past the end of the
@@ -116,7 +131,7 @@ size_t lbip_lds_bytes(const Geometry& G, int nt_want) {
     // upper mask's end, whose bits only ever meet zero penalties)
     if (lay != kSlab)
         return sizeof(double) * (size_t)G.m_k * (lay == kDual ? 2 : 1) +
-               zone_bytes(npad, nt, (int)G.kml, lay == kDual ? chunk_levels(G.m) : 0) +
+               zone_bytes(npad, nt, (int)G.kml, lbip_levels(lay, G.m, (int)G.kml, npad, nt)) +
                sizeof(uint64_t) * (2 * (size_t)nw + kMaskPadWords);
     // kSlab: the keys in global scratch; the threshold scratch, then the
     // penalty prefix sums and the mask bitmaps, in LDS
@@ -557,34 +572,36 @@ __device__ __forceinline__ void tile_finish(const TileScan<MN, MX>& r, const Til
 
 // The tile's extremes of both statistics: the mean's minimum (MUMN) and / or
 // maximum (MUMX), the std's minimum and maximum.  C: the chunk extremes.
-template <bool MUMN, bool MUMX, bool LEVELS>
+template <bool MUMN, bool MUMX, bool FULL>
 __device__ __forceinline__ void tile_ext(const TileVals& vm, const TileVals& vs,
-                                         const TileGeom& g, const double* C, int nch,
+                                         const TileGeom& g, const double* C, int nch, int levels,
                                          double* mumin, double* mumax, double* sgmin,
                                          double* sgmax) {
     TileScan<MUMN, MUMX> a = tile_scan<MUMN, MUMX>(vm, g);
     TileScan<true, true> b = tile_scan<true, true>(vs, g);
-    if constexpr (LEVELS) {
-        // whole chunks ca + 1 .. cb - 1 from the sparse table: level k holds
-        // ranges of 2^k chunks, two of which (overlapping) cover the middle;
-        // the NaN-propagating min / max are idempotent
-        const int len = g.cb - g.ca - 1;
-        if (len > 0) {
-            const int k = 31 - __builtin_clz(len);
-            const double* L = C + (size_t)k * 4 * nch;
-            const int c0 = g.ca + 1, c1 = g.cb - (1 << k);
-            if constexpr (MUMN) a.mmn = nmin(a.mmn, nmin(L[c0], L[c1]));
-            if constexpr (MUMX) a.mmx = nmax(a.mmx, nmax(L[nch + c0], L[nch + c1]));
-            b.mmn = nmin(b.mmn, nmin(L[2 * nch + c0], L[2 * nch + c1]));
-            b.mmx = nmax(b.mmx, nmax(L[3 * nch + c0], L[3 * nch + c1]));
+    // whole chunks ca + 1 .. cb - 1 from the sparse table: level k holds
+    // ranges of 2^k chunks; consecutive ones and a last one ending at cb - 1
+    // (overlapping: the NaN-propagating min / max are idempotent) cover the
+    // middle, two when the table reaches floor(log2(len)) (FULL: it always
+    // does)
+    const int len = g.cb - g.ca - 1;
+    if (len > 0) {
+        int k = 31 - __builtin_clz(len);
+        if constexpr (!FULL) k = k < levels ? k : levels;
+        const int step = 1 << k;
+        const double* L = C + (size_t)k * 4 * nch;
+        auto fold = [&](int c) {
+            if constexpr (MUMN) a.mmn = nmin(a.mmn, L[c]);
+            if constexpr (MUMX) a.mmx = nmax(a.mmx, L[nch + c]);
+            b.mmn = nmin(b.mmn, L[2 * nch + c]);
+            b.mmx = nmax(b.mmx, L[3 * nch + c]);
+        };
+        if constexpr (FULL) {
+            fold(g.ca + 1);
+        } else {
+            for (int c = g.ca + 1; c <= g.cb - step; c += step) fold(c);
         }
-    } else {
-        for (int c = g.ca + 1; c < g.cb; c++) {  // whole chunks
-            if constexpr (MUMN) a.mmn = nmin(a.mmn, C[c]);
-            if constexpr (MUMX) a.mmx = nmax(a.mmx, C[nch + c]);
-            b.mmn = nmin(b.mmn, C[2 * nch + c]);
-            b.mmx = nmax(b.mmx, C[3 * nch + c]);
-        }
+        fold(g.cb - step);
     }
     tile_finish<MUMN, MUMX>(a, g, mumin, mumax);
     tile_finish<true, true>(b, g, sgmin, sgmax);
@@ -661,11 +678,11 @@ struct KeyStats {
 // segment's own, src/lower_upper_lemire.cpp:8-44) and the statistics'
 // extremes over the position's windows.  ku / kl: the keys (positions m - 1
 // .. m_k - m).
-template <int PART, int NT, bool LEVELS>
+template <int PART, int NT, bool FULL>
 __device__ void norm_env_lds(const double* __restrict__ MU, const double* __restrict__ SG,
                              const double* __restrict__ t, const double* __restrict__ Eu,
                              const double* __restrict__ El, const double* C, int m, int mk,
-                             int kml, int w, double* Xu, double* Xl, KeyStats* ku,
+                             int kml, int w, int levels, double* Xu, double* Xl, KeyStats* ku,
                              KeyStats* kl) {
     constexpr bool U = PART & 1, L = PART & 2;
     const int lane = threadIdx.x & 63;
@@ -704,7 +721,7 @@ __device__ void norm_env_lds(const double* __restrict__ MU, const double* __rest
             // (not used within w of the edges)
             const double Uv = U ? Eu[i] : 0.0, Lv = L ? El[i] : 0.0;
             double mumin = 0.0, mumax = 0.0, sgmin, sgmax;
-            tile_ext<U, L, LEVELS>(vm, vs, g, C, nch, &mumin, &mumax, &sgmin, &sgmax);
+            tile_ext<U, L, FULL>(vm, vs, g, C, nch, levels, &mumin, &mumax, &sgmin, &sgmax);
             finish(i, Uv, Lv, mumin, mumax, sgmin, sgmax);
         }
     } else {  // ranges of fewer than 64 windows: a direct loop per position
@@ -1147,7 +1164,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
     const int npad = npad_of(nthr);
     const int nw = (mk + 63) / 64;
     constexpr bool xl = LAYOUT != kSlab;
-    constexpr bool kLevels = LAYOUT == kDual;  // sparse-table levels in the zone
+    const int levels = lbip_levels(LAYOUT, m, kml, npad, NT);  // sparse-table levels in the zone
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 
     double* slab = (double*)((char*)a.slab + (size_t)blockIdx.x * a.slab_bytes_per_block);
@@ -1164,7 +1181,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
     int64_t *CHl, *CHu;
     uint64_t *GLw, *GUw, *TLw, *TUw;  // masks, transitions: bit p = g[p] xor g[p - 1]
     if (xl) {
-        GLw = (uint64_t*)(zone + zone_bytes(npad, NT, kml, kLevels ? chunk_levels(m) : 0));
+        GLw = (uint64_t*)(zone + zone_bytes(npad, NT, kml, levels));
         GUw = GLw + nw;
         CHl = (int64_t*)X;  // X is dead once both masks are built
         CHu = CHl + (m + 1);
@@ -1228,12 +1245,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
         if constexpr (LAYOUT == kDual) {
             double* Cx = (double*)zone;
             if (m >= 64) {
-                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx, chunk_levels(m));
+                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx, levels);
                 __syncthreads();
             }
             KeyStats ku, kl;
             norm_env_lds<3, NT, true>(MU, SG, t, a.Uraw + seg0, a.Lraw + seg0, Cx, m, mk, kml, w,
-                                      X, X2, &ku, &kl);
+                                      levels, X, X2, &ku, &kl);
             __syncthreads();  // (the zone's chunk extremes are dead from here)
             t_u = threshold(X + (m - 1), ku, true);
             mask(X, t_u, true, GUw);
@@ -1248,11 +1265,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
             double* Lg = slab + npad;
             KeyStats ku, kl;
             if (m >= 64) {
-                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx, 0);
+                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx, levels);
                 __syncthreads();
             }
             norm_env_lds<3, NT, false>(MU, SG, t, a.Uraw + seg0, a.Lraw + seg0, Cx, m, mk, kml, w,
-                                       X, Lg, &ku, &kl);
+                                       levels, X, Lg, &ku, &kl);
             __syncthreads();
             t_u = threshold(keys, ku, true);
             mask(X, t_u, true, GUw);
