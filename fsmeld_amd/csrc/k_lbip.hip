@@ -614,28 +614,54 @@ template <int NT>
 __device__ void chunk_extremes(const double* __restrict__ MU, const double* __restrict__ SG,
                                int kml, int nch, double* C, int levels) {
     const double inf = __builtin_inf(), nan = __builtin_nan("");
-    const int lane = threadIdx.x & 63;
-    for (int c = threadIdx.x >> 6; c < nch; c += NT / 64) {
-        const int j = c * 64 + lane;
-        const bool in = j < kml;
-        const int jc = in ? j : kml - 1;
-        const double vm = MU[jc], vs = SG[jc];
-        const bool nm = __ballot(in && vm != vm) != 0ull;
-        const bool ns = __ballot(in && vs != vs) != 0ull;
-        const bool okm = in && vm == vm, oks = in && vs == vs;
-        double a = okm ? vm : inf, b = okm ? vm : -inf;
-        double e = oks ? vs : inf, f = oks ? vs : -inf;
-        wave_scan_nf<true, true>(a, b);
-        wave_scan_nf<true, true>(e, f);
-        a = lane63(a);
-        b = lane63(b);
-        e = lane63(e);
-        f = lane63(f);
-        if (lane == 0) {
-            C[c] = nm ? nan : a;
-            C[nch + c] = nm ? nan : b;
-            C[2 * nch + c] = ns ? nan : e;
-            C[3 * nch + c] = ns ? nan : f;
+    // eight threads per chunk, eight consecutive windows each (their loads
+    // in flight together), then a three-step reduction within the eight
+    const int part = threadIdx.x & 7;
+    for (int c0 = 0; c0 < nch; c0 += NT / 8) {
+        const int c = c0 + (int)(threadIdx.x >> 3);
+        double a = inf, b = -inf, e = inf, f = -inf;
+        int nanm = 0, nans = 0;
+        if (c < nch) {
+            const int j0 = c * 64 + part * 8;
+            double vm[8], vs[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int j = (j0 + i < kml) ? j0 + i : kml - 1;
+                vm[i] = MU[j];
+                vs[i] = SG[j];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                if (j0 + i < kml) {
+                    if (vm[i] == vm[i]) {
+                        a = vmin(a, vm[i]);
+                        b = vmax(b, vm[i]);
+                    } else {
+                        nanm = 1;
+                    }
+                    if (vs[i] == vs[i]) {
+                        e = vmin(e, vs[i]);
+                        f = vmax(f, vs[i]);
+                    } else {
+                        nans = 1;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+            a = vmin(a, __shfl_xor(a, o, 64));
+            b = vmax(b, __shfl_xor(b, o, 64));
+            e = vmin(e, __shfl_xor(e, o, 64));
+            f = vmax(f, __shfl_xor(f, o, 64));
+            nanm |= __shfl_xor(nanm, o, 64);
+            nans |= __shfl_xor(nans, o, 64);
+        }
+        if (part == 0 && c < nch) {
+            C[c] = nanm ? nan : a;
+            C[nch + c] = nanm ? nan : b;
+            C[2 * nch + c] = nans ? nan : e;
+            C[3 * nch + c] = nans ? nan : f;
         }
     }
     // sparse-table level k: extremes of chunks c .. c + 2^k - 1 (entries
