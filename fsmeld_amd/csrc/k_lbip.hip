@@ -105,8 +105,10 @@ __host__ __device__ inline int chunk_levels_in(int m, int kml, size_t budget) {
     while (k > 0 && c * (size_t)(1 + k) > budget) k--;
     return k;
 }
-__host__ __device__ inline size_t zone_bytes(int npad, int nt, int kml, int levels) {
-    const size_t c = 4 * sizeof(double) * (size_t)((kml + 63) / 64) * (size_t)(1 + levels);
+__host__ __device__ inline size_t zone_bytes(int npad, int nt, int kml, int levels, int layout) {
+    // kDual: the levels and the chunks' suffixes from one offset (chunk_extremes)
+    const size_t c = 4 * sizeof(double) * (size_t)((kml + 63) / 64) *
+                     (size_t)(1 + levels + (layout == kDual ? 1 : 0));
     const size_t t = thr_lds_bytes(npad, nt);
     return c > t ? c : t;
 }
@@ -131,7 +133,7 @@ size_t lbip_lds_bytes(const Geometry& G, int nt_want) {
     // upper mask's end, whose bits only ever meet zero penalties)
     if (lay != kSlab)
         return sizeof(double) * (size_t)G.m_k * (lay == kDual ? 2 : 1) +
-               zone_bytes(npad, nt, (int)G.kml, lbip_levels(lay, G.m, (int)G.kml, npad, nt)) +
+               zone_bytes(npad, nt, (int)G.kml, lbip_levels(lay, G.m, (int)G.kml, npad, nt), lay) +
                sizeof(uint64_t) * (2 * (size_t)nw + kMaskPadWords);
     // kSlab: the keys in global scratch; the threshold scratch, then the
     // penalty prefix sums and the mask bitmaps, in LDS
@@ -485,6 +487,7 @@ struct TileGeom {
     int loB, hiA, lo, hi, ca, cb;
     int il, ir, ih, it;
     bool inl, inr, inh, intl;
+    bool interior;  // neither range end clamped: middle = [p0 + 64 - m, p0]
 };
 __device__ __forceinline__ TileGeom tile_geom(int p0, int m, int kml, int mk, int lane) {
     TileGeom g;
@@ -506,6 +509,7 @@ __device__ __forceinline__ TileGeom tile_geom(int p0, int m, int kml, int mk, in
     g.ir = (g.hiA + 1 + lane < hiB) ? g.hiA + 1 + lane : hiB;
     g.ih = (g.loB + lane < g.hiA) ? g.loB + lane : g.hiA;
     g.it = ((g.cb << 6) + lane < g.hiA) ? (g.cb << 6) + lane : g.hiA;
+    g.interior = p0 + 64 - m >= 0 && p0 <= kml - 1;
     return g;
 }
 struct TileVals {
@@ -523,27 +527,34 @@ struct TileScan {
     uint64_t nanl, nanr;
     bool nanm;
 };
-template <bool MN, bool MX>
+// MID: the middle's partial chunks from the wave (h, t); otherwise only the
+// runs, the middle left at the identity for the caller to fill.
+template <bool MN, bool MX, bool MID>
 __device__ __forceinline__ TileScan<MN, MX> tile_scan(const TileVals& v, const TileGeom& g) {
     const double inf = __builtin_inf();
     TileScan<MN, MX> r;
     r.nanl = __ballot(g.inl && v.l != v.l);
     r.nanr = __ballot(g.inr && v.r != v.r);
-    r.nanm = __ballot((g.inh && v.h != v.h) || (g.intl && v.t != v.t)) != 0ull;
     const bool okl = g.inl && v.l == v.l, okr = g.inr && v.r == v.r;
-    const bool okh = g.inh && v.h == v.h, okt = g.intl && v.t == v.t;
     r.lmn = okl ? v.l : inf;
     r.lmx = okl ? v.l : -inf;
     r.rmn = okr ? v.r : inf;
     r.rmx = okr ? v.r : -inf;
-    const double tmn = okt ? v.t : inf, tmx = okt ? v.t : -inf;
-    r.mmn = vmin(okh ? v.h : inf, tmn);
-    r.mmx = vmax(okh ? v.h : -inf, tmx);
     wave_scan_nf<MN, MX>(r.lmn, r.lmx);  // lane j: [lo(B) - 1 - j, lo(B))
     wave_scan_nf<MN, MX>(r.rmn, r.rmx);  // lane j: (hi(A), hi(A) + 1 + j]
-    wave_scan_nf<MN, MX>(r.mmn, r.mmx);
-    if (MN) r.mmn = lane63(r.mmn);
-    if (MX) r.mmx = lane63(r.mmx);
+    r.mmn = inf;
+    r.mmx = -inf;
+    r.nanm = false;
+    if constexpr (MID) {
+        r.nanm = __ballot((g.inh && v.h != v.h) || (g.intl && v.t != v.t)) != 0ull;
+        const bool okh = g.inh && v.h == v.h, okt = g.intl && v.t == v.t;
+        const double tmn = okt ? v.t : inf, tmx = okt ? v.t : -inf;
+        r.mmn = vmin(okh ? v.h : inf, tmn);
+        r.mmx = vmax(okh ? v.h : -inf, tmx);
+        wave_scan_nf<MN, MX>(r.mmn, r.mmx);
+        if (MN) r.mmn = lane63(r.mmn);
+        if (MX) r.mmx = lane63(r.mmx);
+    }
     return r;
 }
 // The position's extremes from the scans, once the whole chunks are folded
@@ -572,13 +583,31 @@ __device__ __forceinline__ void tile_finish(const TileScan<MN, MX>& r, const Til
 
 // The tile's extremes of both statistics: the mean's minimum (MUMN) and / or
 // maximum (MUMX), the std's minimum and maximum.  C: the chunk extremes.
-template <bool MUMN, bool MUMX, bool FULL>
+// INT (an interior tile of kDual): the middle [p0 + 64 - m, p0] is chunk
+// ca's suffix from offset (64 - m) mod 64 (the block after the sparse
+// table, chunk_extremes), the whole chunks, and window p0 (mp, sp), with no
+// wave reduction.
+template <bool MUMN, bool MUMX, bool FULL, bool INT>
 __device__ __forceinline__ void tile_ext(const TileVals& vm, const TileVals& vs,
                                          const TileGeom& g, const double* C, int nch, int levels,
-                                         double* mumin, double* mumax, double* sgmin,
-                                         double* sgmax) {
-    TileScan<MUMN, MUMX> a = tile_scan<MUMN, MUMX>(vm, g);
-    TileScan<true, true> b = tile_scan<true, true>(vs, g);
+                                         double mp, double sp, double* mumin, double* mumax,
+                                         double* sgmin, double* sgmax) {
+    TileScan<MUMN, MUMX> a = tile_scan<MUMN, MUMX, !INT>(vm, g);
+    TileScan<true, true> b = tile_scan<true, true, !INT>(vs, g);
+    if constexpr (INT) {
+        if (g.ca < g.cb) {
+            const double* X = C + (size_t)(levels + 1) * 4 * nch;
+            const int c = g.ca;
+            if constexpr (MUMN) a.mmn = nmin(a.mmn, X[c]);
+            if constexpr (MUMX) a.mmx = nmax(a.mmx, X[nch + c]);
+            b.mmn = nmin(b.mmn, X[2 * nch + c]);
+            b.mmx = nmax(b.mmx, X[3 * nch + c]);
+        }
+        if constexpr (MUMN) a.mmn = nmin(a.mmn, mp);
+        if constexpr (MUMX) a.mmx = nmax(a.mmx, mp);
+        b.mmn = nmin(b.mmn, sp);
+        b.mmx = nmax(b.mmx, sp);
+    }
     // whole chunks ca + 1 .. cb - 1 from the sparse table: level k holds
     // ranges of 2^k chunks; consecutive ones and a last one ending at cb - 1
     // (overlapping: the NaN-propagating min / max are idempotent) cover the
@@ -609,18 +638,21 @@ __device__ __forceinline__ void tile_ext(const TileVals& vm, const TileVals& vs,
 
 // Extremes of the 64-window chunks [64 c, 64 c + 64) of the segment's mean
 // and std: C[c], C[nch + c] (mean min, max), C[2 nch + c], C[3 nch + c]
-// (std min, max); NaN when the chunk holds a NaN statistic.
+// (std min, max); NaN when the chunk holds a NaN statistic.  With s0 >= 0,
+// also the extremes of each chunk's windows from offset s0 on, in the same
+// layout after the sparse table's levels.
 template <int NT>
 __device__ void chunk_extremes(const double* __restrict__ MU, const double* __restrict__ SG,
-                               int kml, int nch, double* C, int levels) {
+                               int kml, int nch, double* C, int levels, int s0) {
     const double inf = __builtin_inf(), nan = __builtin_nan("");
     // eight threads per chunk, eight consecutive windows each (their loads
     // in flight together), then a three-step reduction within the eight
     const int part = threadIdx.x & 7;
     for (int c0 = 0; c0 < nch; c0 += NT / 8) {
         const int c = c0 + (int)(threadIdx.x >> 3);
-        double a = inf, b = -inf, e = inf, f = -inf;
-        int nanm = 0, nans = 0;
+        // [0]: the whole chunk, [1]: its windows from offset s0 on
+        double a[2] = {inf, inf}, b[2] = {-inf, -inf}, e[2] = {inf, inf}, f[2] = {-inf, -inf};
+        int nanm[2] = {0, 0}, nans[2] = {0, 0};
         if (c < nch) {
             const int j0 = c * 64 + part * 8;
             double vm[8], vs[8];
@@ -633,35 +665,43 @@ __device__ void chunk_extremes(const double* __restrict__ MU, const double* __re
 #pragma unroll
             for (int i = 0; i < 8; i++) {
                 if (j0 + i < kml) {
-                    if (vm[i] == vm[i]) {
-                        a = vmin(a, vm[i]);
-                        b = vmax(b, vm[i]);
-                    } else {
-                        nanm = 1;
-                    }
-                    if (vs[i] == vs[i]) {
-                        e = vmin(e, vs[i]);
-                        f = vmax(f, vs[i]);
-                    } else {
-                        nans = 1;
+#pragma unroll
+                    for (int x = 0; x < 2; x++) {
+                        if (x == 1 && (s0 < 0 || part * 8 + i < s0)) continue;
+                        if (vm[i] == vm[i]) {
+                            a[x] = vmin(a[x], vm[i]);
+                            b[x] = vmax(b[x], vm[i]);
+                        } else {
+                            nanm[x] = 1;
+                        }
+                        if (vs[i] == vs[i]) {
+                            e[x] = vmin(e[x], vs[i]);
+                            f[x] = vmax(f[x], vs[i]);
+                        } else {
+                            nans[x] = 1;
+                        }
                     }
                 }
             }
         }
+        const int nx = s0 >= 0 ? 2 : 1;
+        for (int x = 0; x < nx; x++) {
 #pragma unroll
-        for (int o = 1; o < 8; o <<= 1) {
-            a = vmin(a, __shfl_xor(a, o, 64));
-            b = vmax(b, __shfl_xor(b, o, 64));
-            e = vmin(e, __shfl_xor(e, o, 64));
-            f = vmax(f, __shfl_xor(f, o, 64));
-            nanm |= __shfl_xor(nanm, o, 64);
-            nans |= __shfl_xor(nans, o, 64);
-        }
-        if (part == 0 && c < nch) {
-            C[c] = nanm ? nan : a;
-            C[nch + c] = nanm ? nan : b;
-            C[2 * nch + c] = nans ? nan : e;
-            C[3 * nch + c] = nans ? nan : f;
+            for (int o = 1; o < 8; o <<= 1) {
+                a[x] = vmin(a[x], __shfl_xor(a[x], o, 64));
+                b[x] = vmax(b[x], __shfl_xor(b[x], o, 64));
+                e[x] = vmin(e[x], __shfl_xor(e[x], o, 64));
+                f[x] = vmax(f[x], __shfl_xor(f[x], o, 64));
+                nanm[x] |= __shfl_xor(nanm[x], o, 64);
+                nans[x] |= __shfl_xor(nans[x], o, 64);
+            }
+            if (part == 0 && c < nch) {
+                double* D = C + (x ? (size_t)(levels + 1) * 4 * nch : 0);
+                D[c] = nanm[x] ? nan : a[x];
+                D[nch + c] = nanm[x] ? nan : b[x];
+                D[2 * nch + c] = nans[x] ? nan : e[x];
+                D[3 * nch + c] = nans[x] ? nan : f[x];
+            }
         }
     }
     // sparse-table level k: extremes of chunks c .. c + 2^k - 1 (entries
@@ -743,11 +783,18 @@ __device__ void norm_env_lds(const double* __restrict__ MU, const double* __rest
         for (int p0 = (threadIdx.x >> 6) * 64; p0 < mk; p0 += NT) {
             const int i = p0 + lane;  // < mk: m_k is a multiple of 64 for m >= 64
             const TileGeom g = tile_geom(p0, m, kml, mk, lane);
-            const TileVals vm = tile_load(MU, g), vs = tile_load(SG, g);
             // (not used within w of the edges)
             const double Uv = U ? Eu[i] : 0.0, Lv = L ? El[i] : 0.0;
             double mumin = 0.0, mumax = 0.0, sgmin, sgmax;
-            tile_ext<U, L, FULL>(vm, vs, g, C, nch, levels, &mumin, &mumax, &sgmin, &sgmax);
+            if (FULL && g.interior) {  // (wave-uniform)
+                const TileVals vm{MU[g.il], MU[g.ir], 0.0, 0.0}, vs{SG[g.il], SG[g.ir], 0.0, 0.0};
+                tile_ext<U, L, FULL, true>(vm, vs, g, C, nch, levels, MU[g.hiA], SG[g.hiA],
+                                           &mumin, &mumax, &sgmin, &sgmax);
+            } else {
+                const TileVals vm = tile_load(MU, g), vs = tile_load(SG, g);
+                tile_ext<U, L, FULL, false>(vm, vs, g, C, nch, levels, 0.0, 0.0, &mumin, &mumax,
+                                            &sgmin, &sgmax);
+            }
             finish(i, Uv, Lv, mumin, mumax, sgmin, sgmax);
         }
     } else {  // ranges of fewer than 64 windows: a direct loop per position
@@ -1207,7 +1254,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
     int64_t *CHl, *CHu;
     uint64_t *GLw, *GUw, *TLw, *TUw;  // masks, transitions: bit p = g[p] xor g[p - 1]
     if (xl) {
-        GLw = (uint64_t*)(zone + zone_bytes(npad, NT, kml, levels));
+        GLw = (uint64_t*)(zone + zone_bytes(npad, NT, kml, levels, LAYOUT));
         GUw = GLw + nw;
         CHl = (int64_t*)X;  // X is dead once both masks are built
         CHu = CHl + (m + 1);
@@ -1271,7 +1318,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
         if constexpr (LAYOUT == kDual) {
             double* Cx = (double*)zone;
             if (m >= 64) {
-                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx, levels);
+                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx, levels, (64 - m) & 63);
                 __syncthreads();
             }
             KeyStats ku, kl;
@@ -1291,7 +1338,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
             double* Lg = slab + npad;
             KeyStats ku, kl;
             if (m >= 64) {
-                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx, levels);
+                chunk_extremes<NT>(MU, SG, kml, (kml + 63) / 64, Cx, levels, -1);
                 __syncthreads();
             }
             norm_env_lds<3, NT, false>(MU, SG, t, a.Uraw + seg0, a.Lraw + seg0, Cx, m, mk, kml, w,
