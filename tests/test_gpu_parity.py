@@ -192,6 +192,20 @@ def test_exact_command_abandon_is_safe(gpu, small, metric, arg, m, w):
 def test_lbip_values(gpu, metric, arg, m, w, logn):
     """One case per LB_IP workgroup shape and key count: 256 threads (m = 32,
     512), 1024 threads with 8192 (m = 1024) and 16384 (m = 2048) keys in LDS."""
+    lbip_values_case(metric, arg, m, w, logn)
+
+
+@pytest.mark.parametrize("metric,arg", [("dtw", 0.0), ("msm", 0.5), ("edr", 0.25)])
+@pytest.mark.parametrize("m,w,logn", [(100, 5, 16), (200, 10, 16), (500, 25, 17), (700, 35, 18),
+                                      (1500, 75, 19)])
+def test_lbip_values_unaligned(gpu, metric, arg, m, w, logn):
+    """Lengths that are not multiples of 64: a tile's window ranges start and
+    end inside 64-window chunks (the partial-chunk suffixes and the sparse
+    table's uneven middles), for every segment class from m_k = 1024 to 16384."""
+    lbip_values_case(metric, arg, m, w, logn)
+
+
+def lbip_values_case(metric, arg, m, w, logn):
     t = make_target(8, 1 << logn, nans=0, runs=0, spikes=0.0)  # NaN-free: deques exact
     T = F.Target(t)
     q, _ = F.synth_query(8, t, m, 0.1)
