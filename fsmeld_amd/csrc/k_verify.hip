@@ -1,4 +1,5 @@
 // k_verify.hip -- candidate compaction after LB_IP on gfx950.
+#include <cstdint>
 #include <cstring>
 
 #include "engine_internal.h"
@@ -17,12 +18,13 @@ __device__ inline bool lbip_exceeds_cutoff(double bound, double cutoff, int metr
 }
 
 // ------------------------------------------------------------- compaction ---
-// A block takes chunks of 256 * kCompactIpt windows; at step e its threads
-// test the 256 consecutive windows base + 256 e + t (coalesced loads), and
-// each wave's ballot marks the kept ones.  The (e, wave) masks, in that order,
-// list the chunk's windows ascending, so a scan of their popcounts and one
-// atomicAdd per chunk give every survivor its slot: each chunk's survivors
-// are written ascending and contiguously.
+// A block takes chunks of 256 * kCompactIpt windows; thread t tests the
+// kCompactIpt consecutive windows base + kCompactIpt t .. + kCompactIpt - 1
+// with vector loads (16 invalid flags in one 16-byte load, the bounds in
+// 16-byte pairs), so the chunk's windows are in thread order.  A block scan
+// of the threads' counts and one atomicAdd per chunk give every survivor its
+// slot; the survivors pass through LDS so that the stores are contiguous.
+// Each chunk's survivors are written ascending and contiguously.
 constexpr int kCompactIpt = 16;
 
 __global__ void __launch_bounds__(256) k_compact(Geometry G, QueryParams P,
@@ -30,53 +32,76 @@ __global__ void __launch_bounds__(256) k_compact(Geometry G, QueryParams P,
                                                  const double* __restrict__ LB, int use_lb,
                                                  int64_t* __restrict__ out,
                                                  unsigned long long* counters) {
-    constexpr int kMasks = kCompactIpt * (256 / 64);
-    static_assert(kMasks == 64, "one wave scans the masks");
-    __shared__ uint64_t sh_mask[kMasks];
-    __shared__ int sh_pre[kMasks];
+    __shared__ int64_t sh_idx[256 * kCompactIpt];
+    __shared__ int sh_wsum[256 / 64];
     __shared__ unsigned long long sh_base;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int64_t chunk = (int64_t)blockDim.x * kCompactIpt;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int64_t chunk = 256 * kCompactIpt;
     const int64_t nchunks = (G.ts_n + chunk - 1) / chunk;
-    const uint64_t below = (1ull << lane) - 1;  // lanes before this one
+    const int64_t full = G.forsize * G.kml;  // windows below: stat_index(g) = g
     unsigned long long nvalid = 0;
     for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const int64_t base = c * chunk;
-        uint32_t bits = 0;  // bit e: window base + 256 e + threadIdx.x is kept
-        for (int e = 0; e < kCompactIpt; e++) {
-            const int64_t g = base + (int64_t)blockDim.x * e + threadIdx.x;
-            bool keep = false;
-            if (g < G.ts_n) {
-                const bool valid = !INV[stat_index(G, g)] && !is_excluded(P, g);
-                nvalid += valid;
-                keep = valid && (!use_lb || !lbip_exceeds_cutoff(LB[g], P.cutoff, P.metric));
+        const int64_t g0 = c * chunk + (int64_t)tid * kCompactIpt;
+        // flags and bounds of the thread's windows (1 / +inf past the end)
+        uint8_t inv[kCompactIpt];
+        double lb[kCompactIpt];
+        if (g0 + kCompactIpt <= G.ts_n && g0 + kCompactIpt <= full) {
+            const uint4 v = *reinterpret_cast<const uint4*>(INV + g0);  // g0 % 16 == 0
+            const uint32_t wds[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int i = 0; i < kCompactIpt; i++) inv[i] = (uint8_t)(wds[i >> 2] >> (8 * (i & 3)));
+        } else {
+#pragma unroll
+            for (int i = 0; i < kCompactIpt; i++) {
+                const int64_t g = g0 + i;
+                inv[i] = g < G.ts_n ? INV[stat_index(G, g)] : (uint8_t)1;
             }
-            const uint64_t mask = __ballot(keep);
-            if (lane == 0) sh_mask[e * (256 / 64) + wid] = mask;
-            bits |= (uint32_t)keep << e;
         }
-        __syncthreads();
-        if (threadIdx.x < kMasks) {  // one wave: kMasks == 64
-            const int cnt = __builtin_popcountll(sh_mask[threadIdx.x]);
-            int inc = cnt;
-            for (int o = 1; o < 64; o <<= 1) {
-                const int y = __shfl_up(inc, o, 64);
-                if (lane >= o) inc += y;
+        if (use_lb) {
+            if (g0 + kCompactIpt <= G.ts_n) {
+                const double2* l2 = reinterpret_cast<const double2*>(LB + g0);
+#pragma unroll
+                for (int i = 0; i < kCompactIpt / 2; i++) {
+                    const double2 v = l2[i];
+                    lb[2 * i] = v.x;
+                    lb[2 * i + 1] = v.y;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < kCompactIpt; i++)
+                    lb[i] = g0 + i < G.ts_n ? LB[g0 + i] : 0.0;
             }
-            sh_pre[threadIdx.x] = inc - cnt;
-            if (threadIdx.x == kMasks - 1)
-                sh_base = inc ? atomicAdd(&counters[0], (unsigned long long)inc) : 0ull;
         }
-        __syncthreads();
-        while (bits) {
-            const int e = __builtin_ctz(bits);
-            bits &= bits - 1;
-            const int k = e * (256 / 64) + wid;
-            const unsigned long long pos =
-                sh_base + (unsigned long long)sh_pre[k] + __builtin_popcountll(sh_mask[k] & below);
-            out[pos] = base + (int64_t)blockDim.x * e + threadIdx.x;
+        uint32_t keep = 0;
+#pragma unroll
+        for (int i = 0; i < kCompactIpt; i++) {
+            const int64_t g = g0 + i;
+            const bool valid = g < G.ts_n && !inv[i] && !is_excluded(P, g);
+            nvalid += valid;
+            const bool kp =
+                valid && (!use_lb || !lbip_exceeds_cutoff(lb[i], P.cutoff, P.metric));
+            keep |= (uint32_t)kp << i;
         }
+        const int cnt = __builtin_popcount(keep);
+        int inc = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) sh_wsum[wid] = inc;
         __syncthreads();
+        int off = inc - cnt, tot = 0;
+#pragma unroll
+        for (int k = 0; k < 256 / 64; k++) {
+            if (k < wid) off += sh_wsum[k];
+            tot += sh_wsum[k];
+        }
+        if (tid == 0) sh_base = tot ? atomicAdd(&counters[0], (unsigned long long)tot) : 0ull;
+        for (uint32_t b = keep; b; b &= b - 1) sh_idx[off++] = g0 + __builtin_ctz(b);
+        __syncthreads();
+        const unsigned long long base = sh_base;
+        for (int x = tid; x < tot; x += 256) out[base + (unsigned long long)x] = sh_idx[x];
+        __syncthreads();  // before the next chunk's scan and staging
     }
     for (int o = 32; o > 0; o >>= 1) nvalid += __shfl_xor(nvalid, o, 64);
     if (lane == 0 && nvalid) atomicAdd(&counters[1], nvalid);
@@ -85,6 +110,9 @@ __global__ void __launch_bounds__(256) k_compact(Geometry G, QueryParams P,
 hipError_t launch_compact(hipStream_t st, const Geometry& G, const QueryParams& P,
                           const uint8_t* INV, const double* LB, bool use_lb, int64_t* out,
                           unsigned long long* counters) {
+    // the vector loads read the flags in 16-byte and the bounds in 16-byte
+    // pieces from the buffers' starts
+    if (((uintptr_t)INV & 15) || (use_lb && ((uintptr_t)LB & 15))) return hipErrorInvalidValue;
     const int64_t chunk = 256 * kCompactIpt;
     int64_t blocks = (G.ts_n + chunk - 1) / chunk;
     if (blocks > 2048) blocks = 2048;
