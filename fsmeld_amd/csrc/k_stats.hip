@@ -114,20 +114,43 @@ __global__ void __launch_bounds__(64) k_window_sums(const double* __restrict__ T
 }
 
 // mean = sum1 / m, std = sqrt(sum2 / m - mean^2), NaN when the window holds a
-// non-finite sample; invalid when so, or std is not finite or <= 1e-12
+// non-finite sample; invalid when so, or std is not finite or <= 1e-12.  A
+// thread takes four consecutive windows: 16-byte loads and stores of the
+// sums, one 4-byte store of their flags (the buffers start 16-byte aligned).
 __global__ void __launch_bounds__(256) k_window_finish(int64_t n, int m, double* __restrict__ MU,
                                                        double* __restrict__ SG,
                                                        uint8_t* __restrict__ INV) {
     const double dl = (double)m;
     const double nan = __builtin_nan("");
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const bool bad = (unsigned long long)__double_as_longlong(MU[i]) == kBadSum;
-        const double mi = bad ? nan : MU[i] / dl;
-        const double sg = bad ? nan : sqrt(SG[i] / dl - mi * mi);
+    auto one = [&](double s1, double s2, double* mi, double* sg) {
+        const bool bad = (unsigned long long)__double_as_longlong(s1) == kBadSum;
+        *mi = bad ? nan : s1 / dl;
+        *sg = bad ? nan : sqrt(s2 / dl - *mi * *mi);
+        return (uint32_t)(bad || !__builtin_isfinite(*sg) || *sg <= 1e-12);
+    };
+    const int64_t n4 = n / 4;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        double2* m2 = reinterpret_cast<double2*>(MU + 4 * q);
+        double2* s2 = reinterpret_cast<double2*>(SG + 4 * q);
+        const double2 a0 = m2[0], a1 = m2[1], b0 = s2[0], b1 = s2[1];
+        double mi[4], sg[4];
+        const uint32_t f = one(a0.x, b0.x, &mi[0], &sg[0]) | (one(a0.y, b0.y, &mi[1], &sg[1]) << 8) |
+                           (one(a1.x, b1.x, &mi[2], &sg[2]) << 16) |
+                           (one(a1.y, b1.y, &mi[3], &sg[3]) << 24);
+        m2[0] = double2{mi[0], mi[1]};
+        m2[1] = double2{mi[2], mi[3]};
+        s2[0] = double2{sg[0], sg[1]};
+        s2[1] = double2{sg[2], sg[3]};
+        *reinterpret_cast<uint32_t*>(INV + 4 * q) = f;
+    }
+    // the last n mod 4 windows
+    const int64_t i = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        double mi, sg;
+        INV[i] = (uint8_t)one(MU[i], SG[i], &mi, &sg);
         MU[i] = mi;
         SG[i] = sg;
-        INV[i] = bad || !__builtin_isfinite(sg) || sg <= 1e-12;
     }
 }
 
@@ -157,6 +180,7 @@ hipError_t launch_window_stats(hipStream_t st, const double* T, const Geometry& 
         if (e != hipSuccess) return e;
     }
     const int64_t n = G.S * G.kml;
+    if (((uintptr_t)MU | (uintptr_t)SG | (uintptr_t)INV) & 15) return hipErrorInvalidValue;
     const int64_t fb = std::min<int64_t>((n + 255) / 256, 8192);
     if (raw)
         hipLaunchKernelGGL(k_window_finish_raw, dim3((unsigned)fb), dim3(256), 0, st, n, MU, SG,
